@@ -1,0 +1,45 @@
+// Shared between the gfx950 pairwise kernels and their host launcher.
+#pragma once
+#include <stdint.h>
+
+#define CW_SEQ_PAD 64          // code-0 padding on each side of a per-wave LDS sequence
+#define CW_SMALL_COLS 128      // Myers-Miller nodes with N <= this keep their rows in LDS
+#define CW_STACK_CAP 48        // Myers-Miller stack entries per wave (depth <= log2(L)+2)
+#define CW_TAB_BYTES 4096      // int32 [32][32] substitution matrix in LDS
+#define CW_MAX_THREADS 256     // threads per workgroup (4 waves)
+#define CW_RMAX 8              // rows per lane the kernels are instantiated for
+
+struct DevArgs {
+    const uint8_t* codes;      // packed residue codes of all sequences
+    const int64_t* offs;       // sequence s starts at codes[offs[s]]
+    const int32_t* lens;       // gapped length of sequence s
+    const int32_t* pairs;      // pair p = (pairs[2p], pairs[2p+1]), 0-based
+    const int32_t* order;      // work order over pairs (may be null: identity)
+    int64_t pair_base;         // plain kernel: first work item of this launch
+    int64_t pair_end;          // number of work items
+    const int32_t* go_by_len;  // gap open for min(n,m) == L
+    const int32_t* matrix;     // [32][32]
+    int32_t ge, gap1, gap2;
+    int32_t rmax;              // rows-per-lane cap for stripes (1..CW_RMAX)
+    int32_t lcap;              // LDS bytes per sequence buffer (with padding)
+    int32_t wave_lds;          // LDS bytes per wave
+    int32_t rowcap;            // ints per scratch row array (>= maxlen + 1)
+    int32_t pad_;
+    int64_t scratch_stride;    // ints per wave slot (8 * rowcap)
+    int32_t* scratch;          // per-wave DP boundary rows
+    int32_t* out_count;        // per pair identity count (-1: empty sequence)
+    int32_t* out_info;         // optional per pair [maxScore,se1,se2,sb1,sb2,status,nodes,0]
+    int32_t* status;           // OR of per-pair status bits (0 = consistent)
+    unsigned int* queue;       // persistent kernel work counter
+};
+
+static inline int cw_wave_lds_bytes(int lcap)
+{
+    int b = 2 * lcap + 4 * (4 * (CW_SMALL_COLS + 1) + 6 * CW_STACK_CAP);
+    return (b + 15) & ~15;
+}
+
+static inline int cw_lcap(int maxlen)
+{
+    return ((maxlen + 2 * CW_SEQ_PAD + 1) + 15) & ~15;
+}

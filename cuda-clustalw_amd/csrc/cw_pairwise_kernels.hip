@@ -1,0 +1,551 @@
+// cw_pairwise_kernels.hip -- gfx950 kernels of ClustalW's full-DP pairwise stage.
+//
+// One wavefront (64 lanes) aligns one pair end to end:
+//   forward local Gotoh pass   (FullPairwiseAlign.cpp:280-341)  -> maxScore, (se1,se2)
+//   reverse pass                (FullPairwiseAlign.cpp:343-416)  -> (sb1,sb2)
+//   Myers-Miller recursion      (FullPairwiseAlign.cpp:418-612)  -> identity count
+// (paths relative to CPU/clustalw-2.0.11/src of coldfunction/CUDA-clustalW).
+//
+// DP layout: every pass is a "sweep" over a rows x cols rectangle.  Lanes own
+// rows (R consecutive rows per lane, a stripe of up to 64*R rows) and the wave
+// walks the columns as an anti-diagonal wavefront: at step t lane k works on
+// column t-k.  The H/E values of a lane's last row move to the next lane with
+// one DPP wave_shr:1 per value; the lane-0 input (the stripe's top boundary
+// row) and the lane-63 output (the stripe's bottom row) stream through 64-wide
+// register chunks rotated with DPP wave_shl:1 and read/written to memory one
+// coalesced chunk per 64 steps.  The substitution matrix sits in LDS, sequence
+// residues in per-wave LDS buffers.  All arithmetic is int32 and every
+// comparison keeps the reference's order, so results are bit-identical to the
+// CPU code:
+//   * forward max: first cell in row-major order (smallest i, then j);
+//   * reverse start: first cell in the reverse scan order with H >= maxScore;
+//   * diff midpoint: the two scans of :570-593 as parallel reductions;
+//   * the identity count is summed over the M==1 leaves of the recursion, which
+//     is order-free, so the recursion runs as a depth-first stack in LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cw_pairwise_kernels.h"
+
+namespace cwk {
+
+constexpr int WAVE = 64;
+constexpr int BIG = 0x7fffffff;
+
+enum { MODE_FWD = 0, MODE_REV = 1, MODE_GLB = 2 };
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// lane l receives src[l-1]; lane 0 receives old
+__device__ __forceinline__ int shr1(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xf, 0xf, false);
+}
+// lane l receives src[l+1]; lane 63 receives old
+__device__ __forceinline__ int shl1(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+// make LDS/global writes of this wave visible to its later reads
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+struct Sweep {
+    int nrows, ncols;
+    const uint8_t* rs; int rb, rd;   // residue of row r  : rs[rb + rd*r]
+    const uint8_t* cs; int cb, cd;   // residue of column c: cs[cb + cd*c]
+    int th0, ths, teo;               // top row   : H[0][c] = th0 + c*ths, E = H + teo (c>=1)
+    int lh0, lhs, lfo;               // left col  : H[r][0] = lh0 + r*lhs, F = H + lfo (r>=1)
+    int go, ge, goe;                 // H[0][0] = 0 in every pass
+    int thr;                         // MODE_REV threshold (maxScore)
+    const int* tab;                  // LDS substitution matrix [32][32]
+};
+
+struct Track {
+    int best;   // MODE_FWD: best value; MODE_REV: 1 when found
+    int key;    // (row << 16) | col of the tracked cell
+};
+
+__device__ __forceinline__ int left_h(const Sweep& S, int r) { return r == 0 ? 0 : S.lh0 + r * S.lhs; }
+
+// One stripe: rows row0+1 .. row0+P.  R rows per lane.  DIRECT: R==1 and P<64,
+// lanes >= P idle and lane P-1 writes its row straight to out (only ever the
+// single stripe of a sweep, so its top boundary is the formula row).
+template <int R, int MODE, bool DIRECT>
+__device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int first,
+                                                 const int* inH, const int* inE, int* outH,
+                                                 int* outE, int want_out, Track& T)
+{
+    const int lane = lane_id();
+    const int ncols = S.ncols;
+    const int k1 = DIRECT ? P : P - WAVE * (R - 1);    // lanes < k1 hold R rows
+    const bool dummy = (!DIRECT) && (R > 1) && (lane >= k1);
+    const bool lane_on = DIRECT ? (lane < P) : true;
+    const int d = dummy ? 1 : 0;
+    const int firstrow = row0 + 1 + (lane < k1 ? lane * R : k1 * R + (lane - k1) * (R - 1));
+
+    int sb[R], hL[R], hLg[R], f[R], tj[R], tv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int row = firstrow + r - d;
+        int rr = row < 1 ? 1 : (row > S.nrows ? S.nrows : row);
+        sb[r] = (int)S.rs[S.rb + S.rd * rr] * 32;
+        hL[r] = left_h(S, row);
+        f[r] = hL[r] + S.lfo;
+        hLg[r] = hL[r] - S.goe;
+        tv[r] = 0;
+        tj[r] = BIG;
+    }
+    int dprev = left_h(S, firstrow - 1);
+    const int ge = S.ge, goe = S.goe;
+
+    // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
+    auto chunk = [&](int ci, int& h, int& e) {
+        int col = 64 * ci + 1 + lane;
+        if (first) {
+            h = S.th0 + col * S.ths;
+            e = h + S.teo;
+        } else if (col <= ncols) {
+            h = inH[col];
+            e = inE[col];
+        } else {
+            h = 0;
+            e = 0;
+        }
+    };
+    int cinH, cinE, nxtH = 0, nxtE = 0;
+    chunk(0, cinH, cinE);
+    const int nchunks = (ncols + 63) >> 6;
+    if (nchunks > 1) chunk(1, nxtH, nxtE);
+
+    int outHv = 0, outEv = 0, coutH = 0, coutE = 0;
+    int caddr = S.cb + S.cd * (1 - lane);
+    const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
+
+    for (int t = 1; t <= tend; ++t) {
+        if (((t - 1) & 63) == 0 && t > 1) {
+            int ci = (t - 1) >> 6;
+            cinH = nxtH;
+            cinE = nxtE;
+            if (ci + 1 < nchunks) chunk(ci + 1, nxtH, nxtE);
+        }
+        const int inHv = shr1(cinH, outHv);
+        const int inEv = shr1(cinE, outEv);
+        cinH = shl1(cinH, cinH);
+        cinE = shl1(cinE, cinE);
+        const int c = t - lane;
+        const int b = (int)S.cs[caddr];
+        caddr += S.cd;
+        if (lane_on && c >= 1 && c <= ncols) {
+            int hup = inHv, eup = inEv, dg = dprev;
+            int hupg = inHv - goe;
+            dprev = inHv;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int s = S.tab[sb[r] + b];
+                int e = max(eup - ge, hupg);
+                f[r] = max(f[r] - ge, hLg[r]);
+                int h = max(max(dg + s, e), f[r]);
+                if (MODE == MODE_FWD) h = max(h, 0);
+                if (r == 0 && R > 1 && !DIRECT) {
+                    if (dummy) { h = hup; e = eup; }
+                }
+                dg = hL[r];
+                hL[r] = h;
+                hLg[r] = h - goe;
+                hup = h;
+                hupg = hLg[r];
+                eup = e;
+                if (MODE == MODE_FWD) {
+                    if (h > tv[r]) { tv[r] = h; tj[r] = c; }
+                } else if (MODE == MODE_REV) {
+                    if (h >= S.thr && tj[r] == BIG) tj[r] = c;
+                }
+            }
+            outHv = hup;
+            outEv = eup;
+            if (DIRECT && want_out && lane == P - 1) {
+                outH[c] = hup;
+                outE[c] = eup;
+            }
+        }
+        if (!DIRECT && want_out) {
+            coutH = shl1(outHv, coutH);
+            coutE = shl1(outEv, coutE);
+            const int oc = t - 63;
+            if (oc >= 1 && (((oc & 63) == 0) || oc == ncols)) {
+                const int col = oc - 63 + lane;
+                const int cstart = ((oc - 1) & ~63) + 1;
+                if (col >= cstart) {
+                    outH[col] = coutH;
+                    outE[col] = coutE;
+                }
+            }
+        }
+    }
+    // per-stripe reduction of the tracked cell
+    if (MODE == MODE_FWD) {
+        int v = 0, key = BIG;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = firstrow + r - d;
+            const bool valid = lane_on && !(r == 0 && dummy);
+            if (valid && tv[r] > v) { v = tv[r]; key = (row << 16) | tj[r]; }
+        }
+        const int V = wave_max(v);
+        const int K = wave_min(v == V ? key : BIG);
+        if (V > T.best) { T.best = V; T.key = K; }
+    } else if (MODE == MODE_REV) {
+        int key = BIG;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = firstrow + r - d;
+            const bool valid = lane_on && !(r == 0 && dummy);
+            if (valid && tj[r] != BIG) key = min(key, (row << 16) | tj[r]);
+        }
+        const int K = wave_min(key);
+        if (K != BIG) { T.best = 1; T.key = K; }
+    }
+}
+
+template <int MODE>
+__device__ void run_stripe(int R, bool direct, const Sweep& S, int row0, int P, int first,
+                           const int* inH, const int* inE, int* outH, int* outE, int want_out,
+                           Track& T)
+{
+    if (direct) { stripe<1, MODE, true>(S, row0, P, first, inH, inE, outH, outE, want_out, T); return; }
+    switch (R) {
+    case 1: stripe<1, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 2: stripe<2, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 3: stripe<3, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 4: stripe<4, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 5: stripe<5, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 6: stripe<6, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 7: stripe<7, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    default: stripe<8, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    }
+}
+
+// Full sweep over S.nrows rows in balanced stripes.  Intermediate stripe rows
+// ping-pong through g[0..3] (H0,E0,H1,E1); the last stripe of a MODE_GLB sweep
+// writes the final row to finH/finE (indexed by column 1..ncols).
+struct Bufs { int *h0, *e0, *h1, *e1; };   // ping-pong stripe rows in global scratch
+
+template <int MODE>
+__device__ void sweep(const Sweep& S, int rmax, const Bufs& g, int* finH, int* finE, Track& T)
+{
+    const int nrows = S.nrows;
+    const int per = WAVE * rmax;
+    const int nst = uni((nrows + per - 1) / per);
+    const int base = nrows / nst, rem = nrows % nst;
+    int row0 = 0;
+    for (int s = 0; s < nst; ++s) {
+        const int P = uni(base + (s < rem ? 1 : 0));
+        const int R = uni((P + WAVE - 1) / WAVE);
+        const bool last = (s == nst - 1);
+        const bool direct = (R == 1 && P < WAVE);
+        const int* inH = (s & 1) ? g.h1 : g.h0;
+        const int* inE = (s & 1) ? g.e1 : g.e0;
+        int* oH = last ? finH : ((s & 1) ? g.h0 : g.h1);
+        int* oE = last ? finE : ((s & 1) ? g.e0 : g.e1);
+        const int want = (!last || MODE == MODE_GLB) ? 1 : 0;
+        run_stripe<MODE>(R, direct, S, row0, P, s == 0, inH, inE, oH, oE, want, T);
+        wave_fence();
+        if (MODE == MODE_REV && T.best) break;
+        row0 += P;
+    }
+}
+
+struct Node { int A, B, M, N, tb, te; };
+
+// FullPairwiseAlign::diff (:418-612) restated as a depth-first walk; returns the
+// identity count summed over the M==1 leaves (what tracePath counts).
+__device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2, int go, int ge,
+                        int gap1, int gap2, int sb1, int sb2, int se1, int se2, int rmax,
+                        const Bufs& g, int* g2H, int* g2E, int* g3H, int* g3E,
+                        int* smallF, int* smallR, int small_cap, int* stack, int stack_cap,
+                        int* status, int* nodes_out)
+{
+    const int lane = lane_id();
+    int count = 0, sp = 0, nodes = 0;
+    if (lane == 0) {
+        int* e = stack;
+        e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
+    }
+    sp = 1;
+    wave_fence();
+    while (sp > 0) {
+        --sp;
+        const int* e = stack + 6 * sp;
+        const int A = uni(e[0]), B = uni(e[1]), M = uni(e[2]), N = uni(e[3]);
+        const int tb = uni(e[4]), te = uni(e[5]);
+        wave_fence();
+        if (N <= 0 || M <= 0) continue;                 // only del()/add(): no identities
+        if (M == 1) {                                     // :435-478
+            const int tegN = te + ge * N;                 // N >= 1
+            const int tbgN = tb + ge * N;
+            int midh0 = -(tb + ge) - tegN;
+            const int alt = -(te + ge) - tbgN;
+            if (alt > midh0) midh0 = alt;
+            const int a = s1[A + 1];
+            int bv = -BIG, bj = BIG;
+            for (int j = 1 + lane; j <= N; j += WAVE) {
+                const int k1 = N - j, k2 = j - 1;
+                const int v = base.tab[a * 32 + s2[B + j]] - (k1 <= 0 ? 0 : te + ge * k1) -
+                              (k2 <= 0 ? 0 : tb + ge * k2);
+                if (v > bv) { bv = v; bj = j; }
+            }
+            const int V = wave_max(bv);
+            const int J = wave_min(bv == V ? bj : BIG);
+            if (V > midh0) {
+                const int r2 = s2[B + J];
+                if (a != gap1 && r2 != gap2 && a == r2) ++count;
+            }
+            continue;
+        }
+        ++nodes;
+        const int midi = M / 2;
+        const bool small = N <= small_cap;
+        Track T{0, 0};
+        Sweep F = base;
+        F.nrows = midi; F.ncols = N;
+        F.rs = s1; F.rb = A; F.rd = 1;
+        F.cs = s2; F.cb = B; F.cd = 1;
+        F.th0 = -tb; F.ths = -ge; F.teo = -go;
+        F.lh0 = -tb; F.lhs = -ge; F.lfo = -go;
+        int* fH = small ? smallF : g2H;
+        int* fE = small ? smallF + (small_cap + 1) : g2E;
+        sweep<MODE_GLB>(F, rmax, g, fH, fE, T);
+        Sweep Rv = base;
+        Rv.nrows = M - midi; Rv.ncols = N;
+        Rv.rs = s1; Rv.rb = A + M + 1; Rv.rd = -1;
+        Rv.cs = s2; Rv.cb = B + N + 1; Rv.cd = -1;
+        Rv.th0 = -te; Rv.ths = -ge; Rv.teo = -go;
+        Rv.lh0 = -te; Rv.lhs = -ge; Rv.lfo = -go;
+        int* rH = small ? smallR : g3H;
+        int* rE = small ? smallR + (small_cap + 1) : g3E;
+        sweep<MODE_GLB>(Rv, rmax, g, rH, rE, T);
+        wave_fence();
+        // midpoint (:570-593)
+        const int HH0 = -tb - midi * ge;
+        const int RRN = -te - (M - midi) * ge;
+        int vmax = -BIG, jfirst = BIG, jc = -1, wmax = -BIG, jw = -1;
+        for (int j = lane; j <= N; j += WAVE) {
+            const int HHj = (j == 0) ? HH0 : fH[j];
+            const int DDj = (j == 0) ? HH0 : fE[j];
+            const int RRj = (j == N) ? RRN : rH[N - j];
+            const int SSj = (j == N) ? RRN : rE[N - j];
+            const int v = HHj + RRj;
+            const int w = DDj + SSj + go;
+            if (v > vmax) { vmax = v; jfirst = j; jc = -1; }
+            if (v == vmax && HHj != DDj && RRj == SSj) jc = j;
+            if (w >= wmax) { wmax = w; jw = j; }
+        }
+        const int Mv = wave_max(vmax);
+        const int J0 = wave_min(vmax == Mv ? jfirst : BIG);
+        const int JC = wave_max(vmax == Mv ? jc : -1);
+        const int Wv = wave_max(wmax);
+        const int JW = wave_max(wmax == Wv ? jw : -1);
+        int type, midj;
+        if (Wv > Mv) { type = 2; midj = JW; }
+        else { type = 1; midj = J0 > JC ? J0 : JC; }
+        midj = uni(midj);
+        type = uni(type);
+        wave_fence();
+        if (sp + 2 > stack_cap) { if (lane == 0) atomicOr(status, 2); break; }
+        if (lane == 0) {
+            int* c0 = stack + 6 * sp;
+            int* c1 = stack + 6 * (sp + 1);
+            if (type == 1) {
+                c0[0] = A; c0[1] = B; c0[2] = midi; c0[3] = midj; c0[4] = tb; c0[5] = go;
+                c1[0] = A + midi; c1[1] = B + midj; c1[2] = M - midi; c1[3] = N - midj; c1[4] = go; c1[5] = te;
+            } else {
+                c0[0] = A; c0[1] = B; c0[2] = midi - 1; c0[3] = midj; c0[4] = tb; c0[5] = 0;
+                c1[0] = A + midi + 1; c1[1] = B + midj; c1[2] = M - midi - 1; c1[3] = N - midj; c1[4] = 0; c1[5] = te;
+            }
+        }
+        sp += 2;
+        wave_fence();
+    }
+    *nodes_out = nodes;
+    return count;
+}
+
+__device__ void load_seq(uint8_t* dst, const uint8_t* src, int n, int pad)
+{
+    const int lane = lane_id();
+    // dst[-pad .. n+pad] : residue k (1-based) at dst[k]; padding = code 0
+    for (int k = lane - pad; k <= n + pad; k += WAVE) {
+        uint8_t v = 0;
+        if (k >= 1 && k <= n) v = src[k - 1];
+        dst[k] = v;
+    }
+}
+
+__device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, uint8_t* wlds)
+{
+    const int lane = lane_id();
+    const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
+    const int n = uni(A.lens[si]), m = uni(A.lens[sj]);
+    if (n == 0 || m == 0) {
+        if (lane == 0) A.out_count[p] = -1;
+        return;
+    }
+    const int pad = CW_SEQ_PAD;
+    uint8_t* s1 = wlds + pad;
+    uint8_t* s2 = wlds + A.lcap + pad;
+    int* smallF = (int*)(wlds + 2 * A.lcap);
+    int* smallR = smallF + 2 * (CW_SMALL_COLS + 1);
+    int* stack = smallR + 2 * (CW_SMALL_COLS + 1);
+    load_seq(s1, A.codes + A.offs[si], n, pad);
+    load_seq(s2, A.codes + A.offs[sj], m, pad);
+    wave_fence();
+
+    int* scr = A.scratch + (int64_t)slot * A.scratch_stride;
+    const int rc = A.rowcap;
+    const Bufs g{scr, scr + rc, scr + 2 * rc, scr + 3 * rc};
+    int* g2H = scr + 4 * rc;
+    int* g2E = scr + 5 * rc;
+    int* g3H = scr + 6 * rc;
+    int* g3E = scr + 7 * rc;
+
+    const int go = A.go_by_len[n < m ? n : m];
+    const int ge = A.ge;
+    Sweep S;
+    S.tab = tab;
+    S.go = go; S.ge = ge; S.goe = go + ge;
+    S.thr = 0;
+
+    // forward local pass (:280-341)
+    Sweep F = S;
+    F.nrows = n; F.ncols = m;
+    F.rs = s1; F.rb = 0; F.rd = 1;
+    F.cs = s2; F.cb = 0; F.cd = 1;
+    F.th0 = 0; F.ths = 0; F.teo = -go;
+    F.lh0 = 0; F.lhs = 0; F.lfo = -go;
+    Track T{0, 0};
+    sweep<MODE_FWD>(F, A.rmax, g, nullptr, nullptr, T);
+    const int maxScore = uni(T.best);
+    int se1 = 0, se2 = 0, sb1 = 1, sb2 = 1, status = 0;
+    if (maxScore > 0) {
+        se1 = uni(T.key >> 16);
+        se2 = uni(T.key & 0xffff);
+        // reverse pass (:343-416) in reversed coordinates
+        Sweep Rv = S;
+        Rv.nrows = se1; Rv.ncols = se2;
+        Rv.rs = s1; Rv.rb = se1 + 1; Rv.rd = -1;
+        Rv.cs = s2; Rv.cb = se2 + 1; Rv.cd = -1;
+        Rv.th0 = -1; Rv.ths = 0; Rv.teo = 0;
+        Rv.lh0 = -1; Rv.lhs = 0; Rv.lfo = 0;
+        Rv.thr = maxScore;
+        Track U{0, 0};
+        sweep<MODE_REV>(Rv, A.rmax, g, nullptr, nullptr, U);
+        if (U.best) {
+            const int r = uni(U.key >> 16), c = uni(U.key & 0xffff);
+            sb1 = se1 - r + 1;
+            sb2 = se2 - c + 1;
+        } else {
+            status |= 1;   // cannot happen when maxScore > 0; reported, never guessed
+        }
+    }
+    int nodes = 0;
+    int count = 0;
+    if (se1 - sb1 + 1 > 0 && se2 - sb2 + 1 > 0 && maxScore > 0) {
+        count = mm_count(S, s1, s2, go, ge, A.gap1, A.gap2, sb1, sb2, se1, se2, A.rmax, g, g2H,
+                         g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, stack, CW_STACK_CAP,
+                         A.status, &nodes);
+    }
+    if (lane == 0) {
+        A.out_count[p] = count;
+        if (A.out_info) {
+            int* info = A.out_info + 8 * p;
+            info[0] = maxScore; info[1] = se1; info[2] = se2; info[3] = sb1; info[4] = sb2;
+            info[5] = status; info[6] = nodes; info[7] = 0;
+        }
+        if (status) atomicOr(A.status, status);
+    }
+}
+
+__device__ void load_table(const DevArgs& A, int* tab)
+{
+    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) tab[k] = A.matrix[k];
+    __syncthreads();
+}
+
+} // namespace cwk
+
+using namespace cwk;
+
+extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS) cw_pairwise_plain(DevArgs A)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    int* tab = (int*)lds;
+    load_table(A, tab);
+    const int wave = threadIdx.x / WAVE;
+    uint8_t* wlds = lds + CW_TAB_BYTES + (size_t)wave * A.wave_lds;
+    const int64_t slot = (int64_t)blockIdx.x * (blockDim.x / WAVE) + wave;
+    const int64_t p = A.pair_base + slot;
+    if (p >= A.pair_end) return;
+    const int64_t q = A.order ? A.order[p] : p;
+    do_pair(A, q, (int)slot, tab, wlds);
+}
+
+extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS) cw_pairwise_persistent(DevArgs A)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    int* tab = (int*)lds;
+    load_table(A, tab);
+    const int wave = threadIdx.x / WAVE;
+    uint8_t* wlds = lds + CW_TAB_BYTES + (size_t)wave * A.wave_lds;
+    const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
+    for (;;) {
+        int item = 0;
+        if (lane_id() == 0) item = (int)atomicAdd(A.queue, 1u);
+        item = uni(__shfl(item, 0, WAVE));
+        if ((int64_t)item >= A.pair_end) break;
+        const int64_t q = A.order ? A.order[item] : item;
+        do_pair(A, q, slot, tab, wlds);
+    }
+}
+
+// ---- host-side launchers (used by cwhip_api.cpp) -------------------------
+hipError_t cw_launch_plain(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a)
+{
+    hipLaunchKernelGGL(cw_pairwise_plain, dim3(blocks), dim3(threads), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t cw_launch_persistent(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a)
+{
+    hipLaunchKernelGGL(cw_pairwise_persistent, dim3(blocks), dim3(threads), lds, st, a);
+    return hipGetLastError();
+}
+
+extern "C" __global__ void cw_reset_counter(unsigned int* q, int32_t* status)
+{
+    if (threadIdx.x == 0) { *q = 0u; *status = 0; }
+}
+
+hipError_t cw_launch_reset(hipStream_t st, unsigned int* q, int32_t* status)
+{
+    hipLaunchKernelGGL(cw_reset_counter, dim3(1), dim3(64), 0, st, q, status);
+    return hipGetLastError();
+}
+
+hipError_t cw_occupancy(int threads, size_t lds, int* plain_blocks, int* pers_blocks)
+{
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(plain_blocks, cw_pairwise_plain, threads, lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(pers_blocks, cw_pairwise_persistent, threads, lds);
+}

@@ -1,0 +1,133 @@
+/*
+ * cwhip.h -- C ABI of the MI355X (gfx950) implementation of ClustalW 2.0.11's
+ * full-DP pairwise distance stage.
+ *
+ * The ABI carries plain pointers and sizes only.  A context owns the device
+ * buffers (sequences resident in HBM, per-wave scratch, results) and is used
+ * from one host thread at a time.  Functions return 0 on success and a
+ * negative CWHIP_E* code on failure; cwhip_last_error() describes the last
+ * failure of a context.
+ *
+ * Reference interfaces replaced (paths in coldfunction/CUDA-clustalW under
+ * CPU/clustalw-2.0.11/src/):
+ *   cwhip_pairwise_range  <- FullPairwiseAlign::pairwiseAlign(Alignment*, DistMatrix*,
+ *                            int iStart, int iEnd, int jStart, int jEnd)
+ *                            (pairwise/FullPairwiseAlign.cpp:34-211,
+ *                             declared pairwise/PairwiseAlignBase.h:19-20)
+ *   cwhip_pair_params_default <- SubMatrix::getPairwiseMatrix + the gap
+ *                            parameters of FullPairwiseAlign.cpp:129-148
+ *                            (substitutionMatrix/SubMatrix.cpp:256-390, 689-829)
+ *   cwhip_distance        <- the float distance of FullPairwiseAlign.cpp:170-181
+ * The GPU variant of the reference exposes the same operation as the free
+ * function cudaFullPairwiseAlign(Alignment*, DistMatrix*, int,int,int,int)
+ * (GPU/cuda_clustalw_final/src/cuda/cudaFullPairwiseAlign.h:20).
+ */
+#ifndef CWHIP_H
+#define CWHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CWHIP_NUMRES 32
+
+enum {
+    CWHIP_OK = 0,
+    CWHIP_E_ARG = -1,       /* bad argument (null pointer, bad range, size) */
+    CWHIP_E_HIP = -2,       /* HIP runtime error */
+    CWHIP_E_NOSEQ = -3,     /* no sequences loaded */
+    CWHIP_E_TOOLONG = -4,   /* a sequence exceeds CWHIP_MAX_LEN */
+    CWHIP_E_KERNEL = -5,    /* a kernel reported an internal inconsistency */
+    CWHIP_E_NOMEM = -6      /* device allocation failed */
+};
+
+#define CWHIP_MAX_LEN 16000  /* longest sequence the kernels accept */
+
+/* kernel choice for cwhip_pairwise*; both give identical results */
+enum {
+    CWHIP_KERNEL_PLAIN = 0,   /* one wavefront per pair, launched over pair blocks */
+    CWHIP_KERNEL_BATCHED = 1  /* persistent kernel over a cost-sorted device work queue */
+};
+
+/* Scoring of one pairwise run.  matrix is NUMRES x NUMRES, row-major, indexed
+ * by residue code; gap_open_by_len[L] is the scaled gap-open penalty of a pair
+ * whose shorter gapped length is L (0 <= L <= max_len); gap_extend is the
+ * scaled extension penalty; gap_pos1/gap_pos2 are the two gap codes. */
+typedef struct {
+    int32_t matrix[CWHIP_NUMRES * CWHIP_NUMRES];
+    int32_t gap_extend;
+    int32_t gap_pos1;
+    int32_t gap_pos2;
+    int32_t max_len;
+    const int32_t *gap_open_by_len;
+} cwhip_pair_params;
+
+typedef struct cwhip_ctx cwhip_ctx;
+
+/* Create a context on HIP device `device`. */
+int cwhip_create(int device, cwhip_ctx **out);
+void cwhip_destroy(cwhip_ctx *ctx);
+const char *cwhip_last_error(const cwhip_ctx *ctx);
+
+/* Scoring set up the way the reference does for the pairwise stage
+ * (SubMatrix.cpp:256-390; FullPairwiseAlign.cpp:129-148): dna=0 -> Gonnet 250
+ * x10, gap open int(2*matAvg*(pw_gap_open + ln L)) in double, extend
+ * int(pw_gap_extend*10); dna=1 -> IUB x100 with transition weight, open
+ * int(2*pw_gap_open*100), extend int(pw_gap_extend*100).  gap_open_buf must
+ * hold max_len+1 ints; p->gap_open_by_len is pointed at it. */
+int cwhip_pair_params_default(int dna, float pw_gap_open, float pw_gap_extend,
+                              float transition_weight, int max_len,
+                              cwhip_pair_params *p, int32_t *gap_open_buf);
+
+/* Upload sequences: sequence s is codes[offsets[s] .. offsets[s]+lens[s]) (codes
+ * as Sequence::encodeSequence produces them, without the leading dummy).  The
+ * sequences stay resident on the device until the next call or destroy. */
+int cwhip_load_sequences(cwhip_ctx *ctx, const uint8_t *codes, const int64_t *offsets,
+                         const int32_t *lens, int nseq);
+
+/* Number of pairs the reference's loops visit for a range (FullPairwiseAlign.cpp:97,110). */
+int64_t cwhip_range_npairs(int nseq, int i_start, int i_end, int j_start, int j_end);
+
+/* The pairwise stage over the reference's range semantics: pairs (si, sj) with
+ * si in [max(0,i_start), min(N,i_end)), sj in [max(si+1, j_start+1), min(N,j_end)),
+ * in that visiting order.  out_pairs (2 ints per pair, may be NULL), out_count
+ * (identities; -1 when either sequence is empty, may be NULL) and out_dist
+ * (the float distance the reference stores; 1.0 for empty sequences) receive
+ * one entry per pair.  Returns CWHIP_E_ARG for the reference's fatal range
+ * check (i_start<0, i_end<i_start, j_start<0, j_end<j_start). */
+int cwhip_pairwise_range(cwhip_ctx *ctx, const cwhip_pair_params *p, int i_start, int i_end,
+                         int j_start, int j_end, int kernel, int32_t *out_pairs,
+                         int32_t *out_count, float *out_dist);
+
+/* Same for an explicit list of 0-based (si, sj) pairs. */
+int cwhip_pairwise_pairs(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pairs,
+                         int64_t npairs, int kernel, int32_t *out_count, float *out_dist);
+
+/* Distance of FullPairwiseAlign.cpp:170-181 from an identity count and the
+ * two gap-free lengths (IEEE single arithmetic, as the reference). */
+float cwhip_distance(int32_t count, int32_t len1, int32_t len2);
+
+/* ---- device-resident timing interface (inputs already in HBM) ---------- */
+
+/* Stage a pair list and the scoring on the device without running it. */
+int cwhip_stage_pairs(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pairs,
+                      int64_t npairs);
+/* Run the staged pairs `reps` times with `kernel`; *ms receives the device time of
+ * the last repetition measured with HIP events on the context's stream, and
+ * *kernel_ms the time of the main DP kernel alone. */
+int cwhip_run_staged(cwhip_ctx *ctx, int kernel, int reps, float *ms, float *kernel_ms);
+/* Copy the identity counts of the last run (staged order) to the host. */
+int cwhip_fetch_counts(cwhip_ctx *ctx, int32_t *out_count, int64_t npairs);
+/* Per-pair diagnostics of the last run, 8 ints per pair:
+ * [maxScore, se1, se2, sb1, sb2, status, Myers-Miller internal nodes, 0]. */
+int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
+
+/* Per-wave tuning knobs (0 = keep the built-in default): rows per lane limit,
+ * waves per workgroup, persistent workgroups per CU. */
+int cwhip_set_tuning(cwhip_ctx *ctx, int rmax, int waves_per_group, int groups_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
