@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark of the pairwise distance stage on one MI355X.
+
+A step is one full pass of the pairwise stage (forward pass, reverse pass,
+Myers-Miller recursion, identity count) over every pair of the workload, with
+the sequences already resident in HBM.  The default workload is 1000 synthetic
+protein sequences of length ~1523 (499,500 pairs, Gonnet 250, gap open 10.0,
+extension 0.1).  The reported value is the batched (persistent, cost-sorted
+work queue) kernel's GCUPS; the plain one-wave-per-pair kernel is timed beside
+it on the same input, and both must produce identical counts.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "cuda-clustalw_amd"))
+
+METRIC = "pairwise GCUPS (1000x1523), batched vs plain; serving p50/p99 latency"
+PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # CUs x SIMDs x lanes x clock (Tops/s)
+PEAK_HBM_GBS = 8000.0
+OPS_PER_CELL = 10                                 # int32 ops per Gotoh cell update
+
+
+def ref_cells(lens, pairs, info):
+    """Cell updates the reference CPU code performs (forward + reverse up to its
+    early exit + every Myers-Miller node), from the kernels' per-pair cells."""
+    n = lens[pairs[:, 0]].astype(np.int64)
+    m = lens[pairs[:, 1]].astype(np.int64)
+    ms, se1, se2, sb1, sb2 = (info[:, k].astype(np.int64) for k in range(5))
+    rev = np.where(ms > 0, (se1 - sb1) * se2 + (se2 - sb2 + 1), 0)
+    return n * m, rev, info[:, 7].astype(np.int64)
+
+
+def cpu_baseline(codes, offs, lens, pairs, seconds):
+    """The oracle (C restatement of the reference) on a bounded pair sample."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O  # checker/baseline only
+    sc = O.Scoring()
+    threads = min(16, os.cpu_count() or 1)
+    # calibrate with a small sample, then size the sample to ~`seconds`
+    stride = max(1, len(pairs) // 64)
+    probe = pairs[::stride][:64]
+    t = time.perf_counter()
+    O.pairwise_pairs(codes, offs, lens, probe, sc, nthreads=threads)
+    dt = max(time.perf_counter() - t, 1e-3)
+    per_pair = dt / len(probe)
+    want = int(max(64, min(len(pairs), seconds / per_pair)))
+    stride = max(1, len(pairs) // want)
+    sample = pairs[::stride]
+    t = time.perf_counter()
+    O.pairwise_pairs(codes, offs, lens, sample, sc, nthreads=threads)
+    dt = time.perf_counter() - t
+    cells = (lens[sample[:, 0]].astype(np.int64) * lens[sample[:, 1]]).sum()
+    return {"value": round(float(cells) / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": "every %dth pair of the workload (%d pairs, %.1f s, %d OpenMP threads)"
+                      % (stride, len(sample), dt, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="large1523")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the pairwise stage shards by pair lists "
+              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cwhip
+    from cwhip import seqgen
+
+    fam = seqgen.workload(args.workload)
+    codes, offs, lens = fam.packed()
+    nseq = len(lens)
+    iu = np.triu_indices(nseq, 1)
+    pairs = np.stack([iu[0], iu[1]], axis=1).astype(np.int32)   # reference visiting order
+    if args.max_pairs:
+        pairs = pairs[:args.max_pairs]
+    conv_cells = int((lens[pairs[:, 0]].astype(np.int64) * lens[pairs[:, 1]]).sum())
+
+    ctx = cwhip.Context(0)
+    params = cwhip.pair_params(max_len=int(lens.max()) + 1)
+    ctx.load_sequences(codes, offs, lens)
+    ctx.stage(params, pairs)
+
+    def timed(kernel, steps, warmup):
+        for _ in range(warmup):
+            ctx.run_staged(kernel=kernel)
+        tot, kern = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ms, kms = ctx.run_staged(kernel=kernel)
+            tot.append(ms)
+            kern.append(kms)
+        wall = time.perf_counter() - t0
+        return tot, kern, wall
+
+    tot_b, kern_b, wall_b = timed(cwhip.KERNEL_BATCHED, args.steps, args.warmup)
+    cnt_b = ctx.fetch_counts()
+    info = ctx.fetch_info()
+    fwd, rev, mm = ref_cells(lens, pairs, info)
+    upd_cells = int(fwd.sum() + rev.sum() + mm.sum())
+    ms_step = float(np.mean(tot_b))
+    kms = float(np.mean(kern_b))
+    gcups = conv_cells / (ms_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded tree-evolved protein family)",
+        "config": {"workload": args.workload, "sequences": nseq, "pairs": int(len(pairs)),
+                   "length_mean": round(float(lens.mean()), 1), "length_min": int(lens.min()),
+                   "length_max": int(lens.max()), "scoring": "Gonnet250 x10, pw gap 10.0/0.1",
+                   "conventional_cells": conv_cells, "reference_cell_updates": upd_cells},
+        "cells_per_s": round(upd_cells / (ms_step * 1e-3), 1),
+        "batched": {"gcups": round(gcups, 3), "ms": round(ms_step, 3), "kernel_ms": round(kms, 3),
+                    "wall_s": round(wall_b, 3)},
+    }
+    achieved = upd_cells * OPS_PER_CELL / (kms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT32_TOPS,
+                       "unit": "Tops/s (int32)", "frac": round(achieved / PEAK_INT32_TOPS, 4),
+                       "traffic": None,
+                       "note": "integer DP: %d int32 ops per reference cell update; "
+                               "the HBM roof (%.0f GB/s) is not the bound" % (OPS_PER_CELL, PEAK_HBM_GBS)}
+
+    if not args.no_plain:
+        tot_p, kern_p, wall_p = timed(cwhip.KERNEL_PLAIN, args.steps, min(args.warmup, 1))
+        cnt_p = ctx.fetch_counts()
+        ms_p = float(np.mean(tot_p))
+        out["plain"] = {"gcups": round(conv_cells / (ms_p * 1e-3) / 1e9, 3), "ms": round(ms_p, 3),
+                        "kernel_ms": round(float(np.mean(kern_p)), 3),
+                        "identical_to_batched": bool(np.array_equal(cnt_p, cnt_b))}
+        out["batched_over_plain"] = round(ms_p / ms_step, 3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(codes, offs, lens, pairs, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,7 +28,7 @@ struct DevArgs {
     int64_t scratch_stride;    // ints per wave slot (8 * rowcap)
     int32_t* scratch;          // per-wave DP boundary rows
     int32_t* out_count;        // per pair identity count (-1: empty sequence)
-    int32_t* out_info;         // optional per pair [maxScore,se1,se2,sb1,sb2,status,nodes,0]
+    int32_t* out_info;         // optional per pair [maxScore,se1,se2,sb1,sb2,status,nodes,mm cells]
     int32_t* status;           // OR of per-pair status bits (0 = consistent)
     unsigned int* queue;       // persistent kernel work counter
 };

@@ -275,10 +275,10 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
                         int gap1, int gap2, int sb1, int sb2, int se1, int se2, int rmax,
                         const Bufs& g, int* g2H, int* g2E, int* g3H, int* g3E,
                         int* smallF, int* smallR, int small_cap, int* stack, int stack_cap,
-                        int* status, int* nodes_out)
+                        int* status, int* nodes_out, int* cells_out)
 {
     const int lane = lane_id();
-    int count = 0, sp = 0, nodes = 0;
+    int count = 0, sp = 0, nodes = 0, cells = 0;
     if (lane == 0) {
         int* e = stack;
         e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
@@ -293,6 +293,7 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
         wave_fence();
         if (N <= 0 || M <= 0) continue;                 // only del()/add(): no identities
         if (M == 1) {                                     // :435-478
+            cells += N;
             const int tegN = te + ge * N;                 // N >= 1
             const int tbgN = tb + ge * N;
             int midh0 = -(tb + ge) - tegN;
@@ -315,6 +316,7 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
             continue;
         }
         ++nodes;
+        cells += M * N;
         const int midi = M / 2;
         const bool small = N <= small_cap;
         Track T{0, 0};
@@ -379,6 +381,7 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
         wave_fence();
     }
     *nodes_out = nodes;
+    *cells_out = cells;
     return count;
 }
 
@@ -398,8 +401,12 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, u
     const int lane = lane_id();
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
     const int n = uni(A.lens[si]), m = uni(A.lens[sj]);
-    if (n == 0 || m == 0) {
-        if (lane == 0) A.out_count[p] = -1;
+    if (n == 0 || m == 0) {                         // FullPairwiseAlign.cpp:113-118
+        if (lane == 0) {
+            A.out_count[p] = -1;
+            if (A.out_info)
+                for (int k = 0; k < 8; ++k) A.out_info[8 * p + k] = 0;
+        }
         return;
     }
     const int pad = CW_SEQ_PAD;
@@ -459,19 +466,19 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, u
             status |= 1;   // cannot happen when maxScore > 0; reported, never guessed
         }
     }
-    int nodes = 0;
+    int nodes = 0, mmcells = 0;
     int count = 0;
     if (se1 - sb1 + 1 > 0 && se2 - sb2 + 1 > 0 && maxScore > 0) {
         count = mm_count(S, s1, s2, go, ge, A.gap1, A.gap2, sb1, sb2, se1, se2, A.rmax, g, g2H,
                          g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, stack, CW_STACK_CAP,
-                         A.status, &nodes);
+                         A.status, &nodes, &mmcells);
     }
     if (lane == 0) {
         A.out_count[p] = count;
         if (A.out_info) {
             int* info = A.out_info + 8 * p;
             info[0] = maxScore; info[1] = se1; info[2] = se2; info[3] = sb1; info[4] = sb2;
-            info[5] = status; info[6] = nodes; info[7] = 0;
+            info[5] = status; info[6] = nodes; info[7] = mmcells;
         }
         if (status) atomicOr(A.status, status);
     }

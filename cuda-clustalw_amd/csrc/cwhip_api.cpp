@@ -383,7 +383,7 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
 
 int cwhip_fetch_counts(cwhip_ctx* ctx, int32_t* out_count, int64_t npairs)
 {
-    if (!ctx || !out_count || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
+    if (!ctx || (!out_count && npairs) || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
     HIPCHK(hipSetDevice(ctx->device));
     if (npairs) HIPCHK(hipMemcpy(out_count, ctx->d_count.p, npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
     return CWHIP_OK;
@@ -392,7 +392,7 @@ int cwhip_fetch_counts(cwhip_ctx* ctx, int32_t* out_count, int64_t npairs)
 // per-pair diagnostics of the last run: [maxScore,se1,se2,sb1,sb2,status,nodes,0]
 int cwhip_fetch_info(cwhip_ctx* ctx, int32_t* out_info, int64_t npairs)
 {
-    if (!ctx || !out_info || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
+    if (!ctx || (!out_info && npairs) || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
     HIPCHK(hipSetDevice(ctx->device));
     if (npairs) HIPCHK(hipMemcpy(out_info, ctx->d_info.p, 8 * npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
     return CWHIP_OK;
@@ -403,6 +403,7 @@ int cwhip_pairwise_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32
 {
     int rc = cwhip_stage_pairs(ctx, p, pairs, npairs);
     if (rc) return rc;
+    if (npairs == 0) return CWHIP_OK;
     rc = cwhip_run_staged(ctx, kernel, 1, nullptr, nullptr);
     if (rc) return rc;
     std::vector<int32_t> cnt(npairs);

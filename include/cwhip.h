@@ -120,7 +120,10 @@ int cwhip_run_staged(cwhip_ctx *ctx, int kernel, int reps, float *ms, float *ker
 /* Copy the identity counts of the last run (staged order) to the host. */
 int cwhip_fetch_counts(cwhip_ctx *ctx, int32_t *out_count, int64_t npairs);
 /* Per-pair diagnostics of the last run, 8 ints per pair:
- * [maxScore, se1, se2, sb1, sb2, status, Myers-Miller internal nodes, 0]. */
+ * [maxScore, se1, se2, sb1, sb2, status, Myers-Miller internal nodes,
+ *  Myers-Miller cell updates of the reference's diff()].  The reference's
+ * forward pass updates n*m cells and its reverse pass (se1-sb1)*se2 +
+ * (se2-sb2+1) cells (0 when maxScore is 0). */
 int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
 
 /* Per-wave tuning knobs (0 = keep the built-in default): rows per lane limit,
