@@ -61,15 +61,21 @@ __device__ __forceinline__ void wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
 
+// LDS address space: keeps sequence and matrix reads on ds_read_* rather
+// than flat loads.
+#define CW_LDS __attribute__((address_space(3)))
+typedef CW_LDS const uint8_t lcu8;
+typedef CW_LDS uint8_t lu8;
+
 struct Sweep {
     int nrows, ncols;
-    const uint8_t* rs; int rb, rd;   // residue of row r  : rs[rb + rd*r]
-    const uint8_t* cs; int cb, cd;   // residue of column c: cs[cb + cd*c]
+    lcu8* rs; int rb, rd;            // code of row r          : rs[rb + rd*r]
+    lcu8* cs; int cb, cd;            // 4 * code of column c   : cs[cb + cd*c]
     int th0, ths, teo;               // top row   : H[0][c] = th0 + c*ths, E = H + teo (c>=1)
     int lh0, lhs, lfo;               // left col  : H[r][0] = lh0 + r*lhs, F = H + lfo (r>=1)
     int go, ge, goe;                 // H[0][0] = 0 in every pass
     int thr;                         // MODE_REV threshold (maxScore)
-    const int* tab;                  // LDS substitution matrix [32][32]
+    lcu8* tab;                       // LDS substitution matrix, int32 [32][32], byte-addressed
 };
 
 struct Track {
@@ -79,13 +85,20 @@ struct Track {
 
 __device__ __forceinline__ int left_h(const Sweep& S, int r) { return r == 0 ? 0 : S.lh0 + r * S.lhs; }
 
+__device__ __forceinline__ int score_at(lcu8* tab, int byte_off)
+{
+    return *(CW_LDS const int*)(tab + byte_off);
+}
+
+template <bool V> struct Tag { static constexpr bool value = V; };
+
 // One stripe: rows row0+1 .. row0+P.  R rows per lane.  DIRECT: R==1 and P<64,
 // lanes >= P idle and lane P-1 writes its row straight to out (only ever the
 // single stripe of a sweep, so its top boundary is the formula row).
 template <int R, int MODE, bool DIRECT>
 __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int first,
-                                                 const int* inH, const int* inE, int* outH,
-                                                 int* outE, int want_out, Track& T)
+                                       const int* inH, const int* inE, int* outH,
+                                       int* outE, int want_out, Track& T)
 {
     const int lane = lane_id();
     const int ncols = S.ncols;
@@ -100,7 +113,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     for (int r = 0; r < R; ++r) {
         int row = firstrow + r - d;
         int rr = row < 1 ? 1 : (row > S.nrows ? S.nrows : row);
-        sb[r] = (int)S.rs[S.rb + S.rd * rr] * 32;
+        sb[r] = (int)S.rs[S.rb + S.rd * rr] * 128;
         hL[r] = left_h(S, row);
         f[r] = hL[r] + S.lfo;
         hLg[r] = hL[r] - S.goe;
@@ -108,7 +121,10 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         tj[r] = BIG;
     }
     int dprev = left_h(S, firstrow - 1);
-    const int ge = S.ge, goe = S.goe;
+    const int ge = S.ge, goe = S.goe, thr = S.thr;
+    lcu8* const tab = S.tab;
+    lcu8* const cs = S.cs;
+    const int cd = S.cd;
 
     // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
     auto chunk = [&](int ci, int& h, int& e) {
@@ -130,30 +146,35 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     if (nchunks > 1) chunk(1, nxtH, nxtE);
 
     int outHv = 0, outEv = 0, coutH = 0, coutE = 0;
-    int caddr = S.cb + S.cd * (1 - lane);
-    const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
+    int caddr = S.cb + cd * (1 - lane);
+    int b4 = cs[caddr];                                 // 4*code of this lane's column at t=1
 
-    for (int t = 1; t <= tend; ++t) {
+    auto step = [&](int t, auto masked) {
         if (((t - 1) & 63) == 0 && t > 1) {
-            int ci = (t - 1) >> 6;
+            const int ci = (t - 1) >> 6;
             cinH = nxtH;
             cinE = nxtE;
             if (ci + 1 < nchunks) chunk(ci + 1, nxtH, nxtE);
         }
-        const int inHv = shr1(cinH, outHv);
+        const int nH = shl1(cinH, cinH);                // chunk rotation for the next step
+        const int nE = shl1(cinE, cinE);
+        const int inHv = shr1(cinH, outHv);             // lane 0: chunk element; others: lane above
         const int inEv = shr1(cinE, outEv);
-        cinH = shl1(cinH, cinH);
-        cinE = shl1(cinE, cinE);
+        cinH = nH;
+        cinE = nE;
+        const int bcur = b4;
+        caddr += cd;
+        b4 = cs[caddr];                                 // prefetch next step's column code
         const int c = t - lane;
-        const int b = (int)S.cs[caddr];
-        caddr += S.cd;
-        if (lane_on && c >= 1 && c <= ncols) {
+        bool act = true;
+        if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= ncols;
+        if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
             int hupg = inHv - goe;
             dprev = inHv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int s = S.tab[sb[r] + b];
+                const int s = score_at(tab, sb[r] + bcur);
                 int e = max(eup - ge, hupg);
                 f[r] = max(f[r] - ge, hLg[r]);
                 int h = max(max(dg + s, e), f[r]);
@@ -170,7 +191,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 if (MODE == MODE_FWD) {
                     if (h > tv[r]) { tv[r] = h; tj[r] = c; }
                 } else if (MODE == MODE_REV) {
-                    if (h >= S.thr && tj[r] == BIG) tj[r] = c;
+                    tj[r] = min(tj[r], h >= thr ? c : BIG);
                 }
             }
             outHv = hup;
@@ -193,7 +214,22 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 }
             }
         }
+    };
+
+    if (DIRECT) {
+        const int tend = ncols + P - 1;
+        for (int t = 1; t <= tend; ++t) step(t, Tag<true>{});
+    } else {
+        // ramp-up (lanes still entering), steady state (every lane on a valid
+        // column: no exec masking), ramp-down
+        const int tend = ncols + 63;
+        const int up_end = ncols < 63 ? ncols + 63 : 63;
+        int t = 1;
+        for (; t <= up_end; ++t) step(t, Tag<true>{});
+        for (; t <= ncols; ++t) step(t, Tag<false>{});
+        for (; t <= tend; ++t) step(t, Tag<true>{});
     }
+
     // per-stripe reduction of the tracked cell
     if (MODE == MODE_FWD) {
         int v = 0, key = BIG;
@@ -219,6 +255,18 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     }
 }
 
+// Multi-row stripes are called out of line: each is register-allocated on its
+// own (no spills at <= 128 VGPRs) and the call costs nothing next to a stripe
+// of >= 128 rows.  Single-row stripes, which dominate the deep Myers-Miller
+// nodes, stay inline.
+template <int R, int MODE>
+__device__ __attribute__((noinline)) void stripe_call(const Sweep S, int row0, int P, int first,
+                                                      const int* inH, const int* inE, int* outH,
+                                                      int* outE, int want_out, Track* T)
+{
+    stripe<R, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, *T);
+}
+
 template <int MODE>
 __device__ void run_stripe(int R, bool direct, const Sweep& S, int row0, int P, int first,
                            const int* inH, const int* inE, int* outH, int* outE, int want_out,
@@ -227,13 +275,13 @@ __device__ void run_stripe(int R, bool direct, const Sweep& S, int row0, int P, 
     if (direct) { stripe<1, MODE, true>(S, row0, P, first, inH, inE, outH, outE, want_out, T); return; }
     switch (R) {
     case 1: stripe<1, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 2: stripe<2, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 3: stripe<3, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 4: stripe<4, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 5: stripe<5, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 6: stripe<6, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 7: stripe<7, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    default: stripe<8, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 2: stripe_call<2, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 3: stripe_call<3, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 4: stripe_call<4, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 5: stripe_call<5, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 6: stripe_call<6, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 7: stripe_call<7, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    default: stripe_call<8, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
     }
 }
 
@@ -271,7 +319,7 @@ struct Node { int A, B, M, N, tb, te; };
 
 // FullPairwiseAlign::diff (:418-612) restated as a depth-first walk; returns the
 // identity count summed over the M==1 leaves (what tracePath counts).
-__device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2, int go, int ge,
+__device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
                         int gap1, int gap2, int sb1, int sb2, int se1, int se2, int rmax,
                         const Bufs& g, int* g2H, int* g2E, int* g3H, int* g3E,
                         int* smallF, int* smallR, int small_cap, int* stack, int stack_cap,
@@ -303,14 +351,14 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
             int bv = -BIG, bj = BIG;
             for (int j = 1 + lane; j <= N; j += WAVE) {
                 const int k1 = N - j, k2 = j - 1;
-                const int v = base.tab[a * 32 + s2[B + j]] - (k1 <= 0 ? 0 : te + ge * k1) -
+                const int v = score_at(base.tab, a * 128 + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
                               (k2 <= 0 ? 0 : tb + ge * k2);
                 if (v > bv) { bv = v; bj = j; }
             }
             const int V = wave_max(bv);
             const int J = wave_min(bv == V ? bj : BIG);
             if (V > midh0) {
-                const int r2 = s2[B + J];
+                const int r2 = s2[B + J] >> 2;
                 if (a != gap1 && r2 != gap2 && a == r2) ++count;
             }
             continue;
@@ -385,18 +433,18 @@ __device__ int mm_count(const Sweep& base, const uint8_t* s1, const uint8_t* s2,
     return count;
 }
 
-__device__ void load_seq(uint8_t* dst, const uint8_t* src, int n, int pad)
+// dst[-pad .. n+pad]: residue k (1-based) at dst[k] as scale*code; padding = 0
+__device__ void load_seq(lu8* dst, const uint8_t* src, int n, int pad, int scale)
 {
     const int lane = lane_id();
-    // dst[-pad .. n+pad] : residue k (1-based) at dst[k]; padding = code 0
     for (int k = lane - pad; k <= n + pad; k += WAVE) {
-        uint8_t v = 0;
-        if (k >= 1 && k <= n) v = src[k - 1];
-        dst[k] = v;
+        int v = 0;
+        if (k >= 1 && k <= n) v = src[k - 1] * scale;
+        dst[k] = (uint8_t)v;
     }
 }
 
-__device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, uint8_t* wlds)
+__device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* wlds)
 {
     const int lane = lane_id();
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
@@ -410,13 +458,13 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, u
         return;
     }
     const int pad = CW_SEQ_PAD;
-    uint8_t* s1 = wlds + pad;
-    uint8_t* s2 = wlds + A.lcap + pad;
-    int* smallF = (int*)(wlds + 2 * A.lcap);
+    lu8* s1 = wlds + pad;                  // codes of seq1
+    lu8* s2 = wlds + A.lcap + pad;         // 4 * codes of seq2 (byte offsets into a matrix row)
+    int* smallF = (int*)(uint8_t*)(wlds + 2 * A.lcap);
     int* smallR = smallF + 2 * (CW_SMALL_COLS + 1);
     int* stack = smallR + 2 * (CW_SMALL_COLS + 1);
-    load_seq(s1, A.codes + A.offs[si], n, pad);
-    load_seq(s2, A.codes + A.offs[sj], m, pad);
+    load_seq(s1, A.codes + A.offs[si], n, pad, 1);
+    load_seq(s2, A.codes + A.offs[sj], m, pad, 4);
     wave_fence();
 
     int* scr = A.scratch + (int64_t)slot * A.scratch_stride;
@@ -484,7 +532,7 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, const int* tab, u
     }
 }
 
-__device__ void load_table(const DevArgs& A, int* tab)
+__device__ void load_table(const DevArgs& A, CW_LDS int* tab)
 {
     for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) tab[k] = A.matrix[k];
     __syncthreads();
@@ -494,13 +542,14 @@ __device__ void load_table(const DevArgs& A, int* tab)
 
 using namespace cwk;
 
-extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS) cw_pairwise_plain(DevArgs A)
+extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_plain(DevArgs A)
 {
-    extern __shared__ __align__(16) uint8_t lds[];
-    int* tab = (int*)lds;
-    load_table(A, tab);
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    lu8* lds = (lu8*)lds_raw;
+    load_table(A, (CW_LDS int*)lds);
     const int wave = threadIdx.x / WAVE;
-    uint8_t* wlds = lds + CW_TAB_BYTES + (size_t)wave * A.wave_lds;
+    lcu8* tab = lds;
+    lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int64_t slot = (int64_t)blockIdx.x * (blockDim.x / WAVE) + wave;
     const int64_t p = A.pair_base + slot;
     if (p >= A.pair_end) return;
@@ -508,13 +557,14 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS) cw_pairwise_plain(D
     do_pair(A, q, (int)slot, tab, wlds);
 }
 
-extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS) cw_pairwise_persistent(DevArgs A)
+extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_persistent(DevArgs A)
 {
-    extern __shared__ __align__(16) uint8_t lds[];
-    int* tab = (int*)lds;
-    load_table(A, tab);
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    lu8* lds = (lu8*)lds_raw;
+    load_table(A, (CW_LDS int*)lds);
     const int wave = threadIdx.x / WAVE;
-    uint8_t* wlds = lds + CW_TAB_BYTES + (size_t)wave * A.wave_lds;
+    lcu8* tab = lds;
+    lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
     for (;;) {
         int item = 0;
