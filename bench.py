@@ -66,6 +66,24 @@ def cpu_baseline(codes, offs, lens, pairs, seconds):
                       % (stride, len(sample), dt, threads)}
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes (the HIP runtime's blit kernel): the
+    reference dispatch whose FETCH_SIZE / WRITE_SIZE counters scale the
+    per-dispatch HBM traffic of the DP kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +94,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the pairwise stage shards by pair lists "
               "(see DESIGN.md); run with --gpus 1", file=sys.stderr)

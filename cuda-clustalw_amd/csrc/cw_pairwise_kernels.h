@@ -3,7 +3,7 @@
 #include <stdint.h>
 
 #define CW_SEQ_PAD 64          // code-0 padding on each side of a per-wave LDS sequence
-#define CW_SMALL_COLS 128      // Myers-Miller nodes with N <= this keep their rows in LDS
+#define CW_SMALL_COLS 256      // Myers-Miller nodes with N <= this run lanes-own-columns, rows in LDS
 #define CW_STACK_CAP 48        // Myers-Miller stack entries per wave (depth <= log2(L)+2)
 #define CW_TAB_BYTES 4096      // int32 [32][32] substitution matrix in LDS
 #define CW_MAX_THREADS 256     // threads per workgroup (4 waves)

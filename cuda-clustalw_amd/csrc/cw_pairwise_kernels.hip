@@ -317,6 +317,84 @@ __device__ void sweep(const Sweep& S, int rmax, const Bufs& g, int* finH, int* f
 
 struct Node { int A, B, M, N, tb, te; };
 
+// Final rows of both halves of a Myers-Miller node (:482-568) with lanes owning
+// columns: lanes 0..31 sweep the forward half (rows 1..midi from corner (A,B),
+// end penalty tb), lanes 32..63 the reverse half (rows M..midi+1 from corner
+// (A+M,B+N), end penalty te) at the same time, C columns per lane (N <= 32*C).
+// Lane l of a half works on its row t-l at step t; the (H,F) of its last
+// column moves to the next lane by DPP.  Writes HH/DD[j] to fH/fE[j] and
+// RR/SS[j] to rH/rE[N-j] (j = 1..N and 0..N-1), the layout of the row path.
+template <int C>
+__device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B, int M,
+                                               int N, int tb, int te, int go, int ge, int* fH,
+                                               int* fE, int* rH, int* rE)
+{
+    const int lane = lane_id();
+    const int grp = lane >> 5;                 // 0: forward half, 1: reverse half
+    const int L = lane & 31;
+    const int midi = M / 2;
+    const int rows = grp ? (M - midi) : midi;
+    const int t0 = grp ? te : tb;
+    const int goe = go + ge;
+    const int rb = grp ? (A + M + 1) : A;      // code of half-row i: s1[rb + rd*i]
+    const int rd = grp ? -1 : 1;
+    const int nl = (N + C - 1) / C;            // lanes in use per half
+    int b4[C], hup[C], hupg[C], e[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;          // half-local column
+        const int jc = jj <= N ? jj : N;
+        b4[c] = s2[grp ? (B + N + 1 - jc) : (B + jc)];
+        hup[c] = -t0 - jj * ge;                // top row H[0][jj]
+        e[c] = hup[c] - go;
+        hupg[c] = hup[c] - goe;
+    }
+    int dleft = (L == 0) ? 0 : (-t0 - (L * C) * ge);   // H[0][L*C]; corner is 0
+    int hl_out = 0, fl_out = 0;
+    int raddr = rb + rd * (1 - L);
+    const int T = (midi > M - midi ? midi : M - midi) + nl - 1;
+    for (int t = 1; t <= T; ++t) {
+        int hl = shr1(0, hl_out);
+        int fl = shr1(0, fl_out);
+        const int i = t - L;
+        if (L == 0) {                          // left column H[i][0], F[i][0]
+            hl = -t0 - i * ge;
+            fl = hl - go;
+        }
+        const int a128 = (int)s1[raddr] * 128;
+        raddr += rd;
+        if (i >= 1 && i <= rows && L < nl) {
+            int d = dleft;
+            int hc = hl, fc = fl;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int sc = score_at(tab, a128 + b4[c]);
+                const int en = max(e[c] - ge, hupg[c]);
+                const int fn = max(fc - ge, hc - goe);
+                const int h = max(max(d + sc, en), fn);
+                d = hup[c];
+                hup[c] = h;
+                hupg[c] = h - goe;
+                e[c] = en;
+                hc = h;
+                fc = fn;
+            }
+            dleft = hl;
+            hl_out = hc;
+            fl_out = fc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;
+        if (jj <= N) {
+            if (grp == 0) { fH[jj] = hup[c]; fE[jj] = e[c]; }
+            else { rH[jj] = hup[c]; rE[jj] = e[c]; }
+        }
+    }
+}
+
+
 // FullPairwiseAlign::diff (:418-612) restated as a depth-first walk; returns the
 // identity count summed over the M==1 leaves (what tracePath counts).
 __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
@@ -366,26 +444,33 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         ++nodes;
         cells += M * N;
         const int midi = M / 2;
-        const bool small = N <= small_cap;
-        Track T{0, 0};
-        Sweep F = base;
-        F.nrows = midi; F.ncols = N;
-        F.rs = s1; F.rb = A; F.rd = 1;
-        F.cs = s2; F.cb = B; F.cd = 1;
-        F.th0 = -tb; F.ths = -ge; F.teo = -go;
-        F.lh0 = -tb; F.lhs = -ge; F.lfo = -go;
-        int* fH = small ? smallF : g2H;
-        int* fE = small ? smallF + (small_cap + 1) : g2E;
-        sweep<MODE_GLB>(F, rmax, g, fH, fE, T);
-        Sweep Rv = base;
-        Rv.nrows = M - midi; Rv.ncols = N;
-        Rv.rs = s1; Rv.rb = A + M + 1; Rv.rd = -1;
-        Rv.cs = s2; Rv.cb = B + N + 1; Rv.cd = -1;
-        Rv.th0 = -te; Rv.ths = -ge; Rv.teo = -go;
-        Rv.lh0 = -te; Rv.lhs = -ge; Rv.lfo = -go;
-        int* rH = small ? smallR : g3H;
-        int* rE = small ? smallR + (small_cap + 1) : g3E;
-        sweep<MODE_GLB>(Rv, rmax, g, rH, rE, T);
+        int *fH, *fE, *rH, *rE;          // HH/DD[j] at [j]; RR/SS[j] at [N-j]
+        if (N <= 32 * 8) {
+            fH = smallF; fE = smallF + (small_cap + 1);
+            rH = smallR; rE = smallR + (small_cap + 1);
+            if (N <= 32) mm_halves_cols<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+            else if (N <= 64) mm_halves_cols<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+            else if (N <= 128) mm_halves_cols<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+            else mm_halves_cols<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+        } else {
+            Track T{0, 0};
+            Sweep F = base;
+            F.nrows = midi; F.ncols = N;
+            F.rs = s1; F.rb = A; F.rd = 1;
+            F.cs = s2; F.cb = B; F.cd = 1;
+            F.th0 = -tb; F.ths = -ge; F.teo = -go;
+            F.lh0 = -tb; F.lhs = -ge; F.lfo = -go;
+            fH = g2H; fE = g2E;
+            sweep<MODE_GLB>(F, rmax, g, fH, fE, T);
+            Sweep Rv = base;
+            Rv.nrows = M - midi; Rv.ncols = N;
+            Rv.rs = s1; Rv.rb = A + M + 1; Rv.rd = -1;
+            Rv.cs = s2; Rv.cb = B + N + 1; Rv.cd = -1;
+            Rv.th0 = -te; Rv.ths = -ge; Rv.teo = -go;
+            Rv.lh0 = -te; Rv.lhs = -ge; Rv.lfo = -go;
+            rH = g3H; rE = g3E;
+            sweep<MODE_GLB>(Rv, rmax, g, rH, rE, T);
+        }
         wave_fence();
         // midpoint (:570-593)
         const int HH0 = -tb - midi * ge;
