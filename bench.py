@@ -157,9 +157,14 @@ def main():
                     "wall_s": round(wall_b, 3)},
     }
     achieved = upd_cells * OPS_PER_CELL / (kms * 1e-3) / 1e12
+    traffic, tsrc = None, None
+    tfile = os.path.join(HERE, "profiles", "pmc_traffic_%s.json" % args.workload)
+    if os.path.exists(tfile) and not args.max_pairs:
+        t = json.load(open(tfile))
+        traffic, tsrc = int(t["traffic_bytes"]), t["summary"]
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT32_TOPS,
                        "unit": "Tops/s (int32)", "frac": round(achieved / PEAK_INT32_TOPS, 4),
-                       "traffic": None,
+                       "traffic": traffic, "traffic_source": tsrc,
                        "note": "integer DP: %d int32 ops per reference cell update; "
                                "the HBM roof (%.0f GB/s) is not the bound" % (OPS_PER_CELL, PEAK_HBM_GBS)}
 
