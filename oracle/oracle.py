@@ -52,6 +52,9 @@ def lib():
                                      C.c_int, C.c_void_p, C.POINTER(PairResult)]
         L.cwo_count_pairs.restype = C.c_int64
         L.cwo_count_pairs.argtypes = [C.c_int] * 5
+        L.cwo_nj_dnd.restype = C.c_void_p
+        L.cwo_nj_dnd.argtypes = [P(np.float64, flags="C"), C.c_int, C.POINTER(C.c_char_p)]
+        L.cwo_free.argtypes = [C.c_void_p]
         L.cwo_pairwise_list.argtypes = [P(np.uint8, flags="C"), P(np.int64, flags="C"),
                                         P(np.int32, flags="C"), P(np.int32, flags="C"),
                                         C.c_int64, P(np.int32, flags="C"), P(np.int32, flags="C"),
@@ -134,3 +137,24 @@ def range_pairs(nseq: int, i_start: int, i_end: int, j_start: int, j_end: int) -
         for sj in range(max(si + 1, j_start + 1), min(nseq, j_end)):
             out.append((si, sj))
     return np.array(out, dtype=np.int32).reshape(-1, 2)
+
+
+def nj_dnd(dist_full: np.ndarray, names) -> str:
+    """Guide-tree text (.dnd) of NJTree + ClusterTreeOutput for an (n+1)x(n+1)
+    1-based distance matrix (row/column 0 unused)."""
+    n = len(names)
+    d = np.ascontiguousarray(dist_full, dtype=np.float64).copy()
+    assert d.shape == (n + 1, n + 1)
+    arr = (C.c_char_p * (n + 1))(*([b""] + [nm.encode() for nm in names]))
+    p = lib().cwo_nj_dnd(d, n, arr)
+    txt = C.string_at(p).decode()
+    lib().cwo_free(p)
+    return txt
+
+
+def distance_matrix(n: int, pairs: np.ndarray, dist: np.ndarray) -> np.ndarray:
+    """(n+1)x(n+1) symmetric matrix from per-pair float distances (SetAt twice)."""
+    D = np.zeros((n + 1, n + 1), dtype=np.float64)
+    D[pairs[:, 0] + 1, pairs[:, 1] + 1] = dist.astype(np.float64)
+    D[pairs[:, 1] + 1, pairs[:, 0] + 1] = dist.astype(np.float64)
+    return D
