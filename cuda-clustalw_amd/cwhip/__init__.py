@@ -6,4 +6,4 @@ workloads and reads FASTA.
 """
 from . import seqgen  # noqa: F401
 from .lib import (KERNEL_BATCHED, KERNEL_PLAIN, Context, CwhipError,  # noqa: F401
-                  distance, load, pair_params)
+                  MsaParams, distance, load, msa_params, pair_params)

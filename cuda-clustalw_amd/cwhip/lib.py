@@ -21,7 +21,27 @@ EXPORTS = [
     "cwhip_load_sequences", "cwhip_range_npairs", "cwhip_pairwise_range",
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning",
+    "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
+    "cwhip_msa_stats",
 ]
+
+PRF_W = 35
+
+
+class PrfPair(C.Structure):
+    _fields_ = [("prf1", C.c_void_p), ("prf2", C.c_void_p), ("len1", C.c_int32),
+                ("len2", C.c_int32), ("displ", C.c_void_p), ("n_displ", C.c_int32),
+                ("score", C.c_int32)]
+
+
+class MsaParams(C.Structure):
+    _fields_ = [("dna", C.c_int32), ("gap_open", C.c_float), ("gap_extend", C.c_float),
+                ("gap_dist", C.c_int32), ("no_var_penalties", C.c_int32),
+                ("no_hyd_penalties", C.c_int32), ("no_pref_penalties", C.c_int32),
+                ("use_end_gaps", C.c_int32), ("end_gap_penalties", C.c_int32),
+                ("transition_weight", C.c_float), ("neg_matrix", C.c_int32),
+                ("no_weights", C.c_int32), ("divergence_cutoff", C.c_int32),
+                ("hyd_residues", C.c_char_p)]
 
 
 class PairParams(C.Structure):
@@ -68,6 +88,15 @@ def load() -> C.CDLL:
     L.cwhip_fetch_counts.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_fetch_info.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.cwhip_profile_dp.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.cwhip_msa_params_default.argtypes = [C.c_int, C.POINTER(MsaParams)]
+    L.cwhip_msa.argtypes = [vp, C.POINTER(MsaParams), C.c_int, P(np.uint8, flags="C"),
+                            P(np.int64, flags="C"), P(np.int32, flags="C"),
+                            P(np.float64, flags="C"), P(np.int32, flags="C"),
+                            P(np.int32, flags="C"), C.c_int, P(np.int32, flags="C"),
+                            C.POINTER(C.c_int32)]
+    L.cwhip_msa_rows.argtypes = [vp, P(np.uint8, flags="C"), C.c_int64]
+    L.cwhip_msa_stats.argtypes = [vp, P(np.float64, flags="C")]
     for name in EXPORTS:
         if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance"):
             getattr(L, name).restype = C.c_int
@@ -173,6 +202,71 @@ class Context:
         out = np.zeros(8 * max(n, 1), np.int32)
         self._chk(self._L.cwhip_fetch_info(self._h, out, n), "cwhip_fetch_info")
         return out[:8 * n].reshape(-1, 8)
+
+
+    # --- progressive alignment --------------------------------------------
+    def profile_dp(self, problems, end_gap_penalties=0):
+        """progDiff on a batch of (prf1, len1, prf2, len2) profile pairs:
+        [(score, edit script int32[])]."""
+        keep = []
+        arr = (PrfPair * max(len(problems), 1))()
+        for k, (p1, l1, p2, l2) in enumerate(problems):
+            a = np.ascontiguousarray(p1, dtype=np.int32).reshape(-1)
+            b = np.ascontiguousarray(p2, dtype=np.int32).reshape(-1)
+            if a.size != (l1 + 2) * PRF_W or b.size != (l2 + 2) * PRF_W:
+                raise ValueError("profile shape does not match its length")
+            d = np.zeros(l1 + l2 + 4, np.int32)
+            keep += [a, b, d]
+            arr[k].prf1 = a.ctypes.data
+            arr[k].prf2 = b.ctypes.data
+            arr[k].len1 = l1
+            arr[k].len2 = l2
+            arr[k].displ = d.ctypes.data
+        self._chk(self._L.cwhip_profile_dp(self._h, C.cast(arr, C.c_void_p), len(problems),
+                                           int(end_gap_penalties)), "cwhip_profile_dp")
+        out = []
+        for k in range(len(problems)):
+            d = keep[3 * k + 2]
+            out.append((int(arr[k].score), d[:arr[k].n_displ].copy()))
+        return out
+
+    def msa(self, seqs, sim, weights, steps, params=None, dna=False):
+        """multiSeqAlign: (aligned rows as uint8 arrays, 1-based output order, stats)."""
+        if params is None:
+            params = msa_params(dna)
+        n = len(seqs)
+        codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if n else np.zeros(0, np.uint8)
+        lens = np.array([len(s) for s in seqs], np.int32)
+        offs = np.zeros(n, np.int64)
+        if n > 1:
+            offs[1:] = np.cumsum(lens[:-1])
+        order = np.zeros(max(n, 1), np.int32)
+        alen = C.c_int32()
+        steps = np.ascontiguousarray(steps, np.int32).reshape(-1)
+        nsteps = steps.size // (n + 1) if n else 0
+        self._chk(self._L.cwhip_msa(self._h, C.byref(params), n, np.ascontiguousarray(codes),
+                                    offs, lens, np.ascontiguousarray(sim, np.float64).reshape(-1),
+                                    np.ascontiguousarray(weights, np.int32), steps, nsteps, order,
+                                    C.byref(alen)), "cwhip_msa")
+        L = alen.value
+        rows = np.zeros((n, max(L, 1)), np.uint8)
+        self._chk(self._L.cwhip_msa_rows(self._h, rows, rows.shape[1]), "cwhip_msa_rows")
+        st = np.zeros(5, np.float64)
+        self._chk(self._L.cwhip_msa_stats(self._h, st), "cwhip_msa_stats")
+        stats = dict(calls=int(st[0]), batches=int(st[1]), cells=st[2], host_ms=st[3],
+                     device_ms=st[4])
+        return [rows[i, :L].copy() for i in range(n)], order[:n].copy(), stats
+
+
+def msa_params(dna: bool = False, **kw) -> MsaParams:
+    """The reference's multiple-alignment defaults (UserParameters.cpp:33-175)."""
+    p = MsaParams()
+    rc = load().cwhip_msa_params_default(1 if dna else 0, C.byref(p))
+    if rc:
+        raise CwhipError("cwhip_msa_params_default failed: %d" % rc)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 def distance(count: int, len1: int, len2: int) -> float:

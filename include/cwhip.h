@@ -130,6 +130,77 @@ int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
  * waves per workgroup, persistent workgroups per CU. */
 int cwhip_set_tuning(cwhip_ctx *ctx, int rmax, int waves_per_group, int groups_per_cu);
 
+
+/* ------------------------------------------------------------------------ */
+/* Progressive (profile) alignment                                          */
+/* ------------------------------------------------------------------------ */
+
+#define CWHIP_PRF_W 35   /* ints per profile row: codes 0..31, GAPCOL 32, LENCOL 33, pad */
+
+/* One profile pair for cwhip_profile_dp.  prf1/prf2 hold len+2 rows of
+ * CWHIP_PRF_W ints laid out as MyersMillerProfileAlign's profile1/profile2
+ * (row 0 and rows 1..len; column 32 = GAPCOL, 33 = LENCOL).  displ receives
+ * the edit script displ[1..printPtr-1] of the reference (0 = aligned column,
+ * k > 0 = k columns of profile 2 against gaps, k < 0 = -k columns of profile 1
+ * against gaps); it needs room for len1 + len2 + 2 entries. */
+typedef struct {
+    const int32_t *prf1;
+    const int32_t *prf2;
+    int32_t len1, len2;
+    int32_t *displ;      /* out */
+    int32_t n_displ;     /* out: printPtr - 1 */
+    int32_t score;       /* out: the value progDiff returns (profileAlign returns score/100) */
+} cwhip_prf_pair;
+
+/* MyersMillerProfileAlign::progDiff + progTracepath's input for a batch of
+ * independent profile pairs (multipleAlign/MyersMillerProfileAlign.cpp:521-811,
+ * 1002-1262; the top call of :428-429 with go1/go2 = prf1 GAPCOL at rows 0 and
+ * len1).  end_gap_penalties is UserParameters::endGapPenalties. */
+int cwhip_profile_dp(cwhip_ctx *ctx, cwhip_prf_pair *pairs, int npairs, int end_gap_penalties);
+
+/* The UserParameters the progressive stage reads (general/UserParameters.cpp:
+ * 33-175, setDNAParams/setProtParams :501-523). */
+typedef struct {
+    int32_t dna;
+    float gap_open, gap_extend;       /* gapOpen / gapExtend of the multiple alignment */
+    int32_t gap_dist;                 /* gapDist */
+    int32_t no_var_penalties, no_hyd_penalties, no_pref_penalties;
+    int32_t use_end_gaps, end_gap_penalties;
+    float transition_weight;
+    int32_t neg_matrix;               /* useNegMatrix */
+    int32_t no_weights;
+    int32_t divergence_cutoff;
+    const char *hyd_residues;         /* hydResidues, e.g. "GPSNDQEKR" */
+} cwhip_msa_params;
+
+/* The reference defaults for protein (dna = 0) or DNA (dna = 1). */
+int cwhip_msa_params_default(int dna, cwhip_msa_params *p);
+
+/* MSA::multiSeqAlign with iStart = 0 (multipleAlign/MSA.cpp:32-347) driving
+ * MyersMillerProfileAlign::profileAlign (MyersMillerProfileAlign.cpp:36-505)
+ * for every step; the profile DPs run on the device, guide-tree steps on
+ * disjoint subtrees in one batch.  Inputs: the nseq sequences as codes
+ * (codes[offsets[s] .. + lens[s]]), the (nseq+1)^2 row-major similarity matrix
+ * of Tree::calcSimilarities (1-based, row/column 0 unused), the sequence
+ * weights of Tree::calcSeqWeights, and nsteps guide-tree steps of nseq+1 ints
+ * each (element 0 unused; 1 / 2 = group, 0 = not in the step) as
+ * Tree::createSets makes them.  out_order[k] (1-based) is the k-th sequence of
+ * the output order (outputOrder ALIGNED); *aln_len is the alignment length.
+ * The aligned rows stay in the context: fetch them with cwhip_msa_rows. */
+int cwhip_msa(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t *codes,
+              const int64_t *offsets, const int32_t *lens, const double *sim,
+              const int32_t *seq_weight, const int32_t *steps, int nsteps, int32_t *out_order,
+              int32_t *aln_len);
+
+/* Copies the rows of the last cwhip_msa (sequence s at out + s*stride,
+ * aln_len codes each; gaps as code 30 or 31). */
+int cwhip_msa_rows(cwhip_ctx *ctx, uint8_t *out, int64_t stride);
+
+/* Statistics of the last cwhip_msa: [0] profileAlign calls, [1] device batches,
+ * [2] Myers-Miller cells (forward + reverse halves), [3] host milliseconds,
+ * [4] device milliseconds. */
+int cwhip_msa_stats(cwhip_ctx *ctx, double *out5);
+
 #ifdef __cplusplus
 }
 #endif

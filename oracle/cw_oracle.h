@@ -78,6 +78,87 @@ void cwo_pairwise_list(const uint8_t *codes, const int64_t *offs, const int32_t 
                        int ge, int gap_pos1, int gap_pos2, int nthreads,
                        int32_t *out_count, float *out_dist, cwo_pair_result *out_res);
 
+
+/* ---------------------------------------------------------------------- */
+/* Progressive (profile) alignment: oracle/cw_msa.c and oracle/cw_guide.c.  */
+
+#define CWO_PRF_STRIDE 35   /* ints per profile row: codes 0..31, GAPCOL 32, LENCOL 33, pad */
+
+/* The UserParameters the progressive stage reads (UserParameters.cpp:33-175,
+ * set*Params :501-523); cwo_msa_params_default fills the reference defaults. */
+typedef struct {
+    int dna;
+    float gap_open, gap_extend;      /* gapOpen / gapExtend (multiple alignment) */
+    int gap_dist;                    /* gapDist */
+    int no_var_penalties, no_hyd_penalties, no_pref_penalties;
+    int use_end_gaps, end_gap_penalties;
+    float transition_weight;
+    int neg_matrix;                  /* useNegMatrix */
+    int no_weights;
+    int divergence_cutoff;
+    const char *hyd_residues;
+} cwo_msa_params;
+
+typedef struct {
+    int64_t fwd_cells, rev_cells, leaf_cells, nodes;
+} cwo_dp_stats;
+
+typedef struct {
+    int step;                        /* guide-tree step (>0) or -seq of a delayed one */
+    int score;                       /* progDiff's value (profileAlign returns score/100) */
+    int prf_len1, prf_len2, nseqs1, nseqs2, aln_len, n_displ;
+    double pcid;
+    int gapcoef, lencoef, mat_avg;
+    cwo_dp_stats stats;
+} cwo_step_info;
+
+void cwo_msa_params_default(int dna, cwo_msa_params *p);
+
+/* SubMatrix::getProfileAlignMatrix (SubMatrix.cpp:403-677) for the default
+ * matrix names (gonnet series / iub) and getMatrix(negFlag = neg_matrix). */
+int cwo_profile_matrix(int dna, double pcid, int min_len, float transition_weight,
+                       int neg_matrix, int distance_tree, int mat[CWO_NUMRES][CWO_NUMRES],
+                       float *scale, int *mat_avg);
+
+/* MyersMillerProfileAlign::progDiff on two profiles of (len+2) rows of
+ * CWO_PRF_STRIDE ints; the top call's go1/go2 are prf1's GAPCOL at rows 0 and
+ * len1 (MyersMillerProfileAlign.cpp:428-429).  displ receives the edit script
+ * (printPtr-1 entries, at most len1+len2+2); returns progDiff's score. */
+int cwo_prog_diff(const int *prf1, int len1, const int *prf2, int len2, int end_gap_penalties,
+                  int *displ, int *n_displ, cwo_dp_stats *stats);
+
+/* cwo_prog_diff that also records every progDiff call in depth-first order:
+ * 8 ints each (A, B, M, N, go1 | go2 << 1, type (0 = base case, 1, 2), midj,
+ * returned value); *ntrace gets the number of calls (records beyond
+ * trace_cap are dropped). */
+int cwo_prog_diff_trace(const int *prf1, int len1, const int *prf2, int len2,
+                        int end_gap_penalties, int *displ, int *n_displ, cwo_dp_stats *stats,
+                        int *trace, int trace_cap, int *ntrace);
+
+/* The two profiles profileAlign would build for one step (malloc'd; free
+ * with cwo_free).  Returns 0 when a group is empty, 1 otherwise. */
+int cwo_step_profiles(const cwo_msa_params *pm, int nseq, int *const *seqs, const int *lens,
+                      const double *sim, const int *weights, const int *group,
+                      const int *aligned, int **prf1, int *len1, int **prf2, int *len2);
+
+/* MSA::multiSeqAlign (MSA.cpp:32-347, iStart 0).  seqs[s] (malloc'd, lens[s]
+ * codes) are replaced by their aligned rows; sim is the (n+1)^2 similarity
+ * matrix, steps nsteps rows of n+1 ints (element 0 unused).  out_order gets
+ * the 1-based output order.  step_info (optional, room for nsteps + n) gets
+ * one record per profileAlign call.  Returns n, or -1. */
+int cwo_msa(const cwo_msa_params *pm, int nseq, int **seqs, int *lens, const double *sim,
+            int *seq_weight, const int *steps, int nsteps, int *out_order,
+            cwo_step_info *step_info, int *n_info);
+
+/* Tree::readTree + calcSeqWeights + calcSimilarities + createSets as
+ * TreeInterface::getWeightsAndStepsUseOldGuideTreeNJ calls them
+ * (TreeInterface.cpp:209-276), from the .dnd text.  names are the alignment's
+ * sequence names.  weights[n], sim[(n+1)^2]; *steps (malloc'd) gets nsteps
+ * rows of n+1 ints.  Returns 1 or a CWO_GUIDE_* error. */
+enum { CWO_GUIDE_TOO_FEW = -1, CWO_GUIDE_FORMAT = -2, CWO_GUIDE_MISMATCH = -3 };
+int cwo_guide_from_dnd(const char *dnd, int n, const char **names, int *weights,
+                       double *sim, int **steps, int *nsteps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,32 @@ class PairResult(C.Structure):
                 ("mm_nodes", C.c_int64)]
 
 
+class MsaParams(C.Structure):
+    _fields_ = [("dna", C.c_int), ("gap_open", C.c_float), ("gap_extend", C.c_float),
+                ("gap_dist", C.c_int), ("no_var_penalties", C.c_int),
+                ("no_hyd_penalties", C.c_int), ("no_pref_penalties", C.c_int),
+                ("use_end_gaps", C.c_int), ("end_gap_penalties", C.c_int),
+                ("transition_weight", C.c_float), ("neg_matrix", C.c_int),
+                ("no_weights", C.c_int), ("divergence_cutoff", C.c_int),
+                ("hyd_residues", C.c_char_p)]
+
+
+class DpStats(C.Structure):
+    _fields_ = [("fwd_cells", C.c_int64), ("rev_cells", C.c_int64),
+                ("leaf_cells", C.c_int64), ("nodes", C.c_int64)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("step", C.c_int), ("score", C.c_int), ("prf_len1", C.c_int),
+                ("prf_len2", C.c_int), ("nseqs1", C.c_int), ("nseqs2", C.c_int),
+                ("aln_len", C.c_int), ("n_displ", C.c_int), ("pcid", C.c_double),
+                ("gapcoef", C.c_int), ("lencoef", C.c_int), ("mat_avg", C.c_int),
+                ("stats", DpStats)]
+
+
+PRF_STRIDE = 35
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -55,6 +81,30 @@ def lib():
         L.cwo_nj_dnd.restype = C.c_void_p
         L.cwo_nj_dnd.argtypes = [P(np.float64, flags="C"), C.c_int, C.POINTER(C.c_char_p)]
         L.cwo_free.argtypes = [C.c_void_p]
+        L.cwo_msa_params_default.argtypes = [C.c_int, C.POINTER(MsaParams)]
+        L.cwo_guide_from_dnd.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p),
+                                         P(np.int32, flags="C"), P(np.float64, flags="C"),
+                                         C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
+        L.cwo_prog_diff.argtypes = [P(np.int32, flags="C"), C.c_int, P(np.int32, flags="C"),
+                                    C.c_int, C.c_int, P(np.int32, flags="C"),
+                                    C.POINTER(C.c_int), C.POINTER(DpStats)]
+        L.cwo_prog_diff_trace.argtypes = [P(np.int32, flags="C"), C.c_int, P(np.int32, flags="C"),
+                                          C.c_int, C.c_int, P(np.int32, flags="C"),
+                                          C.POINTER(C.c_int), C.POINTER(DpStats),
+                                          P(np.int32, flags="C"), C.c_int, C.POINTER(C.c_int)]
+        L.cwo_step_profiles.argtypes = [C.POINTER(MsaParams), C.c_int, C.c_void_p,
+                                        P(np.int32, flags="C"), P(np.float64, flags="C"),
+                                        P(np.int32, flags="C"), P(np.int32, flags="C"),
+                                        P(np.int32, flags="C"),
+                                        C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int),
+                                        C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
+        L.cwo_msa.argtypes = [C.POINTER(MsaParams), C.c_int, C.c_void_p,
+                              P(np.int32, flags="C"), P(np.float64, flags="C"),
+                              P(np.int32, flags="C"), P(np.int32, flags="C"), C.c_int,
+                              P(np.int32, flags="C"), C.c_void_p, C.POINTER(C.c_int)]
+        L.cwo_profile_matrix.argtypes = [C.c_int, C.c_double, C.c_int, C.c_float, C.c_int,
+                                         C.c_int, P(np.int32, flags="C"),
+                                         C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.cwo_pairwise_list.argtypes = [P(np.uint8, flags="C"), P(np.int64, flags="C"),
                                         P(np.int32, flags="C"), P(np.int32, flags="C"),
                                         C.c_int64, P(np.int32, flags="C"), P(np.int32, flags="C"),
@@ -158,3 +208,148 @@ def distance_matrix(n: int, pairs: np.ndarray, dist: np.ndarray) -> np.ndarray:
     D[pairs[:, 0] + 1, pairs[:, 1] + 1] = dist.astype(np.float64)
     D[pairs[:, 1] + 1, pairs[:, 0] + 1] = dist.astype(np.float64)
     return D
+
+
+# ---------------------------------------------------------------------------
+# Progressive alignment (cw_guide.c, cw_msa.c)
+
+def msa_params(dna: bool = False, **kw) -> MsaParams:
+    p = MsaParams()
+    lib().cwo_msa_params_default(1 if dna else 0, C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def guide_from_dnd(dnd: str, names):
+    """(weights int32[n], sim float64[(n+1),(n+1)], steps int32[nsteps, n+1]) from
+    the .dnd text as TreeInterface::getWeightsAndStepsUseOldGuideTreeNJ derives them."""
+    n = len(names)
+    w = np.zeros(max(n, 1), np.int32)
+    sim = np.zeros((n + 1) * (n + 1), np.float64)
+    arr = (C.c_char_p * max(n, 1))(*[nm.encode() for nm in names])
+    sp = C.POINTER(C.c_int)()
+    ns = C.c_int()
+    st = lib().cwo_guide_from_dnd(dnd.encode(), n, arr, w, sim, C.byref(sp), C.byref(ns))
+    if st != 1:
+        raise ValueError("guide tree not usable (code %d)" % st)
+    steps = np.ctypeslib.as_array(sp, shape=(ns.value * (n + 1),)).copy().reshape(ns.value, n + 1)
+    lib().cwo_free(C.cast(sp, C.c_void_p))
+    return w[:n].copy(), sim.reshape(n + 1, n + 1), steps
+
+
+class _SeqSet:
+    """malloc-free holder of the sequences as an int** the C side may replace."""
+
+    def __init__(self, seqs):
+        libc = C.CDLL(None)
+        self.malloc = libc.malloc
+        self.malloc.restype = C.c_void_p
+        self.malloc.argtypes = [C.c_size_t]
+        self.free = libc.free
+        self.free.argtypes = [C.c_void_p]
+        n = len(seqs)
+        self.n = n
+        self.ptrs = (C.c_void_p * max(n, 1))()
+        self.lens = np.zeros(max(n, 1), np.int32)
+        for i, s in enumerate(seqs):
+            a = np.ascontiguousarray(s, dtype=np.int32)
+            p = self.malloc(max(1, a.nbytes))
+            C.memmove(p, a.ctypes.data, a.nbytes)
+            self.ptrs[i] = p
+            self.lens[i] = len(a)
+
+    def rows(self):
+        out = []
+        for i in range(self.n):
+            L = int(self.lens[i])
+            buf = (C.c_int * max(L, 1)).from_address(self.ptrs[i])
+            out.append(np.ctypeslib.as_array(buf)[:L].copy())
+        return out
+
+    def close(self):
+        for i in range(self.n):
+            if self.ptrs[i]:
+                self.free(self.ptrs[i])
+                self.ptrs[i] = None
+
+
+def msa(seqs, sim, weights, steps, params: MsaParams):
+    """multiSeqAlign: (aligned rows as int32 arrays, 1-based output order, step infos)."""
+    n = len(seqs)
+    ss = _SeqSet(seqs)
+    try:
+        order = np.zeros(max(n, 1), np.int32)
+        info = (StepInfo * (len(steps) + n + 1))()
+        ninfo = C.c_int()
+        r = lib().cwo_msa(C.byref(params), n, C.cast(ss.ptrs, C.c_void_p), ss.lens,
+                          np.ascontiguousarray(sim, np.float64).reshape(-1),
+                          np.ascontiguousarray(weights, np.int32).copy(),
+                          np.ascontiguousarray(steps, np.int32).reshape(-1), len(steps),
+                          order, C.cast(info, C.c_void_p), C.byref(ninfo))
+        if r < 0:
+            raise RuntimeError("multiSeqAlign failed")
+        return ss.rows(), order[:n].copy(), list(info[:ninfo.value])
+    finally:
+        ss.close()
+
+
+def step_profiles(seqs, sim, weights, group, aligned, params: MsaParams):
+    """The two int32 profiles ((len+2) x 35) profileAlign builds for one step, or None."""
+    n = len(seqs)
+    ss = _SeqSet(seqs)
+    try:
+        p1, p2 = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        l1, l2 = C.c_int(), C.c_int()
+        r = lib().cwo_step_profiles(C.byref(params), n, C.cast(ss.ptrs, C.c_void_p), ss.lens,
+                                    np.ascontiguousarray(sim, np.float64).reshape(-1),
+                                    np.ascontiguousarray(weights, np.int32),
+                                    np.ascontiguousarray(group, np.int32),
+                                    np.ascontiguousarray(aligned, np.int32),
+                                    C.byref(p1), C.byref(l1), C.byref(p2), C.byref(l2))
+        if r == 0:
+            return None
+        a = np.ctypeslib.as_array(p1, shape=((l1.value + 2) * PRF_STRIDE,)).copy()
+        b = np.ctypeslib.as_array(p2, shape=((l2.value + 2) * PRF_STRIDE,)).copy()
+        lib().cwo_free(C.cast(p1, C.c_void_p))
+        lib().cwo_free(C.cast(p2, C.c_void_p))
+        return a.reshape(-1, PRF_STRIDE), l1.value, b.reshape(-1, PRF_STRIDE), l2.value
+    finally:
+        ss.close()
+
+
+def prog_diff(prf1, len1, prf2, len2, end_gap_penalties=0):
+    """(score, edit script int32[], DpStats) of progDiff on two profiles."""
+    a = np.ascontiguousarray(prf1, np.int32).reshape(-1)
+    b = np.ascontiguousarray(prf2, np.int32).reshape(-1)
+    displ = np.zeros(len1 + len2 + 4, np.int32)
+    nd = C.c_int()
+    st = DpStats()
+    score = lib().cwo_prog_diff(a, len1, b, len2, end_gap_penalties, displ, C.byref(nd),
+                                C.byref(st))
+    return score, displ[:nd.value].copy(), st
+
+
+def profile_matrix(dna: bool, pcid: float, min_len: int, neg_matrix=0, distance_tree=1):
+    m = np.zeros((NUMRES, NUMRES), np.int32)
+    sc = C.c_float()
+    avg = C.c_int()
+    r = lib().cwo_profile_matrix(1 if dna else 0, pcid, min_len, 0.5, neg_matrix, distance_tree,
+                                 m, C.byref(sc), C.byref(avg))
+    return r, m, sc.value, avg.value
+
+
+def prog_diff_trace(prf1, len1, prf2, len2, end_gap_penalties=0):
+    """progDiff's calls in depth-first order: int32[ncalls, 8] records
+    (A, B, M, N, go1 | go2 << 1, type, midj, value)."""
+    a = np.ascontiguousarray(prf1, np.int32).reshape(-1)
+    b = np.ascontiguousarray(prf2, np.int32).reshape(-1)
+    displ = np.zeros(len1 + len2 + 4, np.int32)
+    nd = C.c_int()
+    st = DpStats()
+    cap = 4 * (len1 + len2) + 16
+    tr = np.zeros(8 * cap, np.int32)
+    nt = C.c_int()
+    lib().cwo_prog_diff_trace(a, len1, b, len2, end_gap_penalties, displ, C.byref(nd),
+                              C.byref(st), tr, cap, C.byref(nt))
+    return tr[:8 * min(nt.value, cap)].reshape(-1, 8)

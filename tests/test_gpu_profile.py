@@ -1,0 +1,111 @@
+"""Profile-alignment parity on the GPU: the device Myers-Miller DP
+(cwhip_profile_dp) against the oracle's progDiff, bit for bit (score and edit
+script), and the whole progressive stage (cwhip_msa) against the reference's
+.aln fixtures and against the oracle on divergent synthetic sets."""
+import numpy as np
+import pytest
+
+import cwhip
+import oracle as O
+from cwhip import seqgen
+from fixtures import load, parse_aln, row_text, text
+from test_msa_oracle import random_profile_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cwhip.Context(0)
+    yield c
+    c.close()
+
+
+def _check_batch(ctx, probs, egp=0):
+    got = ctx.profile_dp([(p1, l1, p2, l2) for p1, l1, p2, l2 in probs], end_gap_penalties=egp)
+    for (p1, l1, p2, l2), (score, displ) in zip(probs, got):
+        s_ref, d_ref, _ = O.prog_diff(p1, l1, p2, l2, egp)
+        assert score == s_ref, (l1, l2)
+        np.testing.assert_array_equal(displ, d_ref)
+
+
+@pytest.mark.parametrize("setname", ["100_97", "100_498", "100_1002", "100_1523"])
+def test_profile_dp_matches_oracle(ctx, setname):
+    print("profile_dp", setname, flush=True)
+    _check_batch(ctx, random_profile_pairs(setname, 12, 7))
+
+
+def test_profile_dp_end_gap_penalties(ctx):
+    print("profile_dp egp", flush=True)
+    _check_batch(ctx, random_profile_pairs("100_498", 6, 9, egp=1), egp=1)
+
+
+def test_profile_dp_small_and_degenerate(ctx):
+    """Lengths 0..3 and very unequal lengths (M <= 1 leaves at the root,
+    single-row sweeps, empty profiles)."""
+    print("profile_dp small", flush=True)
+    names, codes, offs, lens, dna, dnd = load("100_97")
+    seqs = [codes[offs[i]:offs[i] + lens[i]].astype(np.int32) for i in range(len(lens))]
+    n = len(seqs)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    probs = []
+    for la, lb in [(1, 1), (1, 5), (5, 1), (2, 3), (3, 97), (97, 2), (0, 4), (4, 0), (1, 97)]:
+        s2 = [s.copy() for s in seqs]
+        s2[0] = s2[0][:la]
+        s2[1] = s2[1][:lb]
+        group = np.zeros(n + 1, np.int32)
+        group[1], group[2] = 1, 2
+        r = O.step_profiles(s2, sim, w, group, np.ones(n + 1, np.int32), O.msa_params(False))
+        if r is not None:
+            probs.append(r)
+    _check_batch(ctx, probs)
+
+
+SETS = ["100_97", "100_498", "1000_97", "100_1002", "100_1523", "1000_498"]
+
+
+@pytest.mark.parametrize("setname", SETS)
+def test_gpu_msa_reproduces_reference_aln(ctx, setname):
+    print("msa", setname, flush=True)
+    names, codes, offs, lens, dna, dnd = load(setname)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    seqs = [codes[offs[i]:offs[i] + lens[i]] for i in range(len(lens))]
+    rows, order, stats = ctx.msa(seqs, sim, w, steps, dna=dna)
+    ref_order, ref_rows = parse_aln(text(setname + ".aln"))
+    assert [names[i - 1] for i in order] == ref_order
+    for i, nm in enumerate(names):
+        assert row_text(rows[i]) == ref_rows[nm], nm
+    assert stats["calls"] == len(names) - 1
+
+
+def _divergent(seed, n1, n2, L):
+    a = seqgen.family(seed, n1, L, branch_mean=0.6, name_prefix="fa")
+    b = seqgen.family(seed + 1, n2, int(L * 0.8), branch_mean=1.5, name_prefix="fb")
+    return a.seqs + b.seqs, a.names + b.names
+
+
+@pytest.mark.parametrize("seed,n1,n2,L,dna", [(11, 12, 8, 120, False), (12, 20, 10, 200, False),
+                                               (13, 5, 25, 150, False), (14, 10, 10, 300, True)])
+def test_gpu_msa_matches_oracle_divergent(ctx, seed, n1, n2, L, dna):
+    """Sets with unrelated members: the delayed (divergent) pass of
+    multiSeqAlign, steps with an emptied group, and every matrix of the
+    gonnet series."""
+    print("msa divergent", seed, flush=True)
+    seqs, names = _divergent(seed, n1, n2, L)
+    if dna:
+        seqs = [np.array([(0, 2, 6, 18)[c % 4] for c in s], np.uint8) for s in seqs]
+    n = len(seqs)
+    codes = np.concatenate(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    pairs = O.range_pairs(n, 0, n, 0, n)
+    cnt, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring(dna))
+    dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    r_rows, r_order, info = O.msa([s.astype(np.int32) for s in seqs], sim, w, steps,
+                                  O.msa_params(dna))
+    rows, order, stats = ctx.msa(seqs, sim, w, steps, dna=dna)
+    np.testing.assert_array_equal(order, r_order)
+    for i in range(n):
+        np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
