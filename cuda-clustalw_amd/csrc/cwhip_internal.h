@@ -52,9 +52,9 @@ struct cwhip_ctx {
     DevBuf<PrfProb> p_probs;
     int prf_rmax = CWP_RMAX;
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
-    double prf_cells = 0.0;
+    double prf_cells = 0.0, prf_entries = 0.0;
     std::vector<std::vector<uint8_t>> msa_rows;   // rows of the last cwhip_msa
-    double msa_stats[5] = {0, 0, 0, 0, 0};
+    double msa_stats[6] = {0, 0, 0, 0, 0, 0};
 };
 
 static inline int fail(cwhip_ctx* c, int code, const std::string& msg)

@@ -220,6 +220,7 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
         jobs[k].score = out[4 * k + 0];
         jobs[k].cells = cells;
         ctx->prf_cells += cells;
+        ctx->prf_entries += (double)jobs[k].L1 * jobs[k].L2;
     }
     return CWHIP_OK;
 }
@@ -762,7 +763,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         return fail(ctx, CWHIP_E_ARG, "bad multiple alignment arguments");
     const auto t0 = std::chrono::steady_clock::now();
     ctx->prf_score_ms = ctx->prf_mm_ms = 0.f;
-    ctx->prf_cells = 0;
+    ctx->prf_cells = ctx->prf_entries = 0;
     Seqs& seqs = ctx->msa_rows;
     seqs.assign(nseq, {});
     for (int s = 0; s < nseq; ++s) {
@@ -903,6 +904,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
     ctx->msa_stats[2] = ctx->prf_cells;
     ctx->msa_stats[3] = host_ms;
     ctx->msa_stats[4] = ctx->prf_score_ms + ctx->prf_mm_ms;
+    ctx->msa_stats[5] = ctx->prf_entries;
     return CWHIP_OK;
 }
 
@@ -917,10 +919,10 @@ int cwhip_msa_rows(cwhip_ctx* ctx, uint8_t* out, int64_t stride)
     return CWHIP_OK;
 }
 
-int cwhip_msa_stats(cwhip_ctx* ctx, double* out5)
+int cwhip_msa_stats(cwhip_ctx* ctx, double* out6)
 {
-    if (!ctx || !out5) return CWHIP_E_ARG;
-    for (int k = 0; k < 5; ++k) out5[k] = ctx->msa_stats[k];
+    if (!ctx || !out6) return CWHIP_E_ARG;
+    for (int k = 0; k < 6; ++k) out6[k] = ctx->msa_stats[k];
     return CWHIP_OK;
 }
 

@@ -6,4 +6,5 @@ workloads and reads FASTA.
 """
 from . import seqgen  # noqa: F401
 from .lib import (KERNEL_BATCHED, KERNEL_PLAIN, Context, CwhipError,  # noqa: F401
-                  MsaParams, distance, load, msa_params, pair_params)
+                  MsaParams, clustal_text, distance, guide_from_tree, load,
+                  msa_params, nj_tree, pair_params)

@@ -22,7 +22,7 @@ EXPORTS = [
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
-    "cwhip_msa_stats",
+    "cwhip_msa_stats", "cwhip_nj_tree", "cwhip_guide_from_tree", "cwhip_clustal_text",
 ]
 
 PRF_W = 35
@@ -97,6 +97,14 @@ def load() -> C.CDLL:
                             C.POINTER(C.c_int32)]
     L.cwhip_msa_rows.argtypes = [vp, P(np.uint8, flags="C"), C.c_int64]
     L.cwhip_msa_stats.argtypes = [vp, P(np.float64, flags="C")]
+    L.cwhip_nj_tree.argtypes = [C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
+                                C.POINTER(C.c_int64)]
+    L.cwhip_guide_from_tree.argtypes = [C.c_int, C.c_char_p, vp, P(np.int32, flags="C"),
+                                        P(np.float64, flags="C"), P(np.int32, flags="C"),
+                                        C.POINTER(C.c_int32)]
+    L.cwhip_clustal_text.argtypes = [C.c_int, vp, P(np.uint8, flags="C"), C.c_int64,
+                                     P(np.int32, flags="C"), P(np.int32, flags="C"), C.c_int,
+                                     vp, C.c_int64, C.POINTER(C.c_int64)]
     for name in EXPORTS:
         if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance"):
             getattr(L, name).restype = C.c_int
@@ -251,11 +259,70 @@ class Context:
         L = alen.value
         rows = np.zeros((n, max(L, 1)), np.uint8)
         self._chk(self._L.cwhip_msa_rows(self._h, rows, rows.shape[1]), "cwhip_msa_rows")
-        st = np.zeros(5, np.float64)
+        st = np.zeros(6, np.float64)
         self._chk(self._L.cwhip_msa_stats(self._h, st), "cwhip_msa_stats")
-        stats = dict(calls=int(st[0]), batches=int(st[1]), cells=st[2], host_ms=st[3],
-                     device_ms=st[4])
+        stats = dict(calls=int(st[0]), batches=int(st[1]), cells=float(st[2]),
+                     host_ms=float(st[3]), device_ms=float(st[4]), prf_entries=float(st[5]))
         return [rows[i, :L].copy() for i in range(n)], order[:n].copy(), stats
+
+
+def _names(names):
+    arr = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    return arr
+
+
+def nj_tree(dist_full, names) -> str:
+    """.dnd text of the NJ guide tree of an (n+1)x(n+1) 1-based distance matrix."""
+    L = load()
+    n = len(names)
+    d = np.ascontiguousarray(dist_full, dtype=np.float64).reshape(-1)
+    arr = _names(names)
+    ln = C.c_int64()
+    L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), None, 0, C.byref(ln))
+    buf = C.create_string_buffer(ln.value + 1)
+    rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), ln.value + 1,
+                         C.byref(ln))
+    if rc:
+        raise CwhipError("cwhip_nj_tree failed: %d" % rc)
+    return buf.value.decode()
+
+
+def guide_from_tree(dnd: str, names):
+    """(weights int32[n], sim float64[n+1, n+1], steps int32[nsteps, n+1])."""
+    L = load()
+    n = len(names)
+    w = np.zeros(max(n, 1), np.int32)
+    sim = np.zeros((n + 1) * (n + 1), np.float64)
+    steps = np.zeros(max(n - 1, 1) * (n + 1), np.int32)
+    ns = C.c_int32()
+    arr = _names(names)
+    rc = L.cwhip_guide_from_tree(n, dnd.encode(), C.cast(arr, C.c_void_p), w, sim, steps,
+                                 C.byref(ns))
+    if rc:
+        raise CwhipError("cwhip_guide_from_tree failed: %d" % rc)
+    return w[:n].copy(), sim.reshape(n + 1, n + 1), steps[:ns.value * (n + 1)].reshape(-1, n + 1)
+
+
+def clustal_text(names, rows, order, dna=False) -> str:
+    """CLUSTAL (.aln) text of aligned rows (uint8 code arrays) in 1-based order."""
+    L = load()
+    n = len(names)
+    lens = np.array([len(r) for r in rows], np.int32)
+    stride = max(int(lens.max()) if n else 1, 1)
+    buf = np.zeros((n, stride), np.uint8)
+    for i, r in enumerate(rows):
+        buf[i, :len(r)] = r
+    order = np.ascontiguousarray(order, np.int32)
+    arr = _names(names)
+    ln = C.c_int64()
+    L.cwhip_clustal_text(n, C.cast(arr, C.c_void_p), buf, stride, lens, order, int(dna), None, 0,
+                         C.byref(ln))
+    out = C.create_string_buffer(ln.value + 1)
+    rc = L.cwhip_clustal_text(n, C.cast(arr, C.c_void_p), buf, stride, lens, order, int(dna),
+                              C.cast(out, C.c_void_p), ln.value + 1, C.byref(ln))
+    if rc:
+        raise CwhipError("cwhip_clustal_text failed: %d" % rc)
+    return out.value.decode()
 
 
 def msa_params(dna: bool = False, **kw) -> MsaParams:

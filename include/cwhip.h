@@ -197,9 +197,42 @@ int cwhip_msa(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t
 int cwhip_msa_rows(cwhip_ctx *ctx, uint8_t *out, int64_t stride);
 
 /* Statistics of the last cwhip_msa: [0] profileAlign calls, [1] device batches,
- * [2] Myers-Miller cells (forward + reverse halves), [3] host milliseconds,
- * [4] device milliseconds. */
-int cwhip_msa_stats(cwhip_ctx *ctx, double *out5);
+ * [2] Myers-Miller cell updates (forward + reverse halves of every node),
+ * [3] host milliseconds, [4] device milliseconds (kernels), [5] prfScore
+ * entries computed (sum of len1 * len2 over the profile pairs). */
+int cwhip_msa_stats(cwhip_ctx *ctx, double *out6);
+
+
+/* ------------------------------------------------------------------------ */
+/* Guide tree and output text (host code; no device needed)                 */
+/* ------------------------------------------------------------------------ */
+
+/* NJTree::generateTree + ClusterTreeOutput::printPhylipTree
+ * (tree/NJTree.cpp:161-751, tree/ClusterTreeOutput.cpp:45-220; two sequences:
+ * tree/UnRootedClusterTree.cpp:225-233): the .dnd text of the neighbour-joining
+ * guide tree of nseq >= 2 sequences.  dist is the (nseq+1)^2 row-major
+ * DistMatrix (1-based, as FullPairwiseAlign fills it); names[0..nseq-1].  The
+ * text (NUL-terminated) goes to out when cap > its length; *len is set
+ * either way, so a first call with out = NULL sizes the buffer. */
+int cwhip_nj_tree(int nseq, const double *dist, const char *const *names, char *out,
+                  int64_t cap, int64_t *len);
+
+/* Tree::readTree, calcSeqWeights, calcSimilarities and createSets as
+ * TreeInterface::getWeightsAndStepsUseOldGuideTreeNJ calls them
+ * (tree/TreeInterface.cpp:209-276, tree/Tree.cpp): from .dnd text to the
+ * sequence weights (nseq), the similarity matrix ((nseq+1)^2, row-major,
+ * 1-based) and the alignment steps (room for (nseq-1) x (nseq+1) ints; element
+ * 0 of each step unused).  *nsteps gets the number of steps. */
+int cwhip_guide_from_tree(int nseq, const char *dnd, const char *const *names, int32_t *weights,
+                          double *sim, int32_t *steps, int32_t *nsteps);
+
+/* AlignmentOutput::clustalOut (alignment/AlignmentOutput.cpp:1573-2116): the
+ * CLUSTAL text of an alignment.  rows: sequence s (0-based) at rows + s*stride,
+ * lens[s] codes; order: 1-based output order; dna selects the conservation
+ * line's rules.  Sizing as cwhip_nj_tree. */
+int cwhip_clustal_text(int nseq, const char *const *names, const uint8_t *rows, int64_t stride,
+                       const int32_t *lens, const int32_t *order, int dna, char *out, int64_t cap,
+                       int64_t *len);
 
 #ifdef __cplusplus
 }

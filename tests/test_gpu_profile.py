@@ -109,3 +109,17 @@ def test_gpu_msa_matches_oracle_divergent(ctx, seed, n1, n2, L, dna):
     np.testing.assert_array_equal(order, r_order)
     for i in range(n):
         np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
+
+
+@pytest.mark.parametrize("setname", SETS)
+def test_gpu_pipeline_reproduces_reference_dnd_and_aln(ctx, setname):
+    """FASTA -> device pairwise -> NJ -> guide -> device progressive alignment
+    -> CLUSTAL text, all on the library: the reference's .dnd and .aln files
+    byte for byte."""
+    from cwhip import pipeline
+    print("pipeline", setname, flush=True)
+    names, codes, offs, lens, dna, dnd = load(setname)
+    seqs = [codes[offs[i]:offs[i] + lens[i]] for i in range(len(lens))]
+    out_dnd, out_aln, rows, order = pipeline.align(ctx, names, seqs, dna)
+    assert out_dnd == dnd
+    assert out_aln == text(setname + ".aln")
