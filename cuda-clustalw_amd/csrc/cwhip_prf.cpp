@@ -728,6 +728,42 @@ int cwhip_profile_dp(cwhip_ctx* ctx, cwhip_prf_pair* pairs, int npairs, int end_
     return CWHIP_OK;
 }
 
+int cwhip_profile_align(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t* codes,
+                        const int64_t* offsets, const int32_t* lens, const double* sim,
+                        const int32_t* weights, int32_t* group, const int32_t* aligned,
+                        uint8_t* rows_out, int64_t stride, int32_t* info)
+{
+    if (!ctx || !p || nseq < 1 || !codes || !offsets || !lens || !sim || !weights || !group ||
+        !aligned || !info)
+        return fail(ctx, CWHIP_E_ARG, "bad profile alignment arguments");
+    Seqs seqs(nseq);
+    for (int s = 0; s < nseq; ++s) {
+        if (lens[s] < 0) return fail(ctx, CWHIP_E_ARG, "negative length");
+        seqs[s].assign(codes + offsets[s], codes + offsets[s] + lens[s]);
+    }
+    std::vector<int> g(group, group + nseq + 1), al(aligned, aligned + nseq + 1);
+    std::vector<int> w(weights, weights + nseq);
+    Step st;
+    for (int k = 0; k < 4; ++k) info[k] = 0;
+    const bool run = prepare_step(*p, seqs, sim, w, g, al, st);
+    for (int i = 0; i <= nseq; ++i) group[i] = g[i];          // profileAlign mutates group
+    if (!run) return CWHIP_OK;                                 // an empty group: score 0 (:96-99)
+    if (rows_out && stride < st.L1 + st.L2 + 2) return fail(ctx, CWHIP_E_ARG, "row stride too small");
+    std::vector<DpJob> jobs{st.job};
+    const int rc = run_dp_batch(ctx, jobs, p->end_gap_penalties ? 1 : 0);
+    if (rc != CWHIP_OK) return rc;
+    st.job = jobs[0];
+    apply_step(st, seqs);
+    const int alen = (int)seqs[st.members[0]].size();
+    if (rows_out)
+        for (int m : st.members) std::memcpy(rows_out + (int64_t)m * stride, seqs[m].data(), alen);
+    info[0] = st.job.score / 100;
+    info[1] = alen;
+    info[2] = st.L1;
+    info[3] = st.L2;
+    return CWHIP_OK;
+}
+
 int cwhip_msa_params_default(int dna, cwhip_msa_params* p)
 {
     if (!p) return CWHIP_E_ARG;

@@ -22,7 +22,7 @@ EXPORTS = [
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
-    "cwhip_msa_stats", "cwhip_nj_tree", "cwhip_guide_from_tree", "cwhip_clustal_text",
+    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_nj_tree", "cwhip_guide_from_tree", "cwhip_clustal_text",
 ]
 
 PRF_W = 35
@@ -96,6 +96,11 @@ def load() -> C.CDLL:
                             P(np.int32, flags="C"), C.c_int, P(np.int32, flags="C"),
                             C.POINTER(C.c_int32)]
     L.cwhip_msa_rows.argtypes = [vp, P(np.uint8, flags="C"), C.c_int64]
+    L.cwhip_profile_align.argtypes = [vp, C.POINTER(MsaParams), C.c_int, P(np.uint8, flags="C"),
+                                      P(np.int64, flags="C"), P(np.int32, flags="C"),
+                                      P(np.float64, flags="C"), P(np.int32, flags="C"),
+                                      P(np.int32, flags="C"), P(np.int32, flags="C"), vp,
+                                      C.c_int64, P(np.int32, flags="C")]
     L.cwhip_msa_stats.argtypes = [vp, P(np.float64, flags="C")]
     L.cwhip_nj_tree.argtypes = [C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
                                 C.POINTER(C.c_int64)]
@@ -323,6 +328,28 @@ def clustal_text(names, rows, order, dna=False) -> str:
     if rc:
         raise CwhipError("cwhip_clustal_text failed: %d" % rc)
     return out.value.decode()
+
+
+def profile_align(ctx, seqs, sim, weights, group, aligned, params):
+    """One profileAlign step: (info[4], group after the call, new rows by sequence
+    index for the members, None for the others)."""
+    n = len(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    if n > 1:
+        offs[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if n else np.zeros(0, np.uint8)
+    stride = int(lens.sum()) + 2
+    rows = np.zeros((n, stride), np.uint8)
+    g = np.ascontiguousarray(group, np.int32).copy()
+    info = np.zeros(4, np.int32)
+    ctx._chk(ctx._L.cwhip_profile_align(ctx._h, C.byref(params), n, codes, offs, lens,
+                                        np.ascontiguousarray(sim, np.float64).reshape(-1),
+                                        np.ascontiguousarray(weights, np.int32), g,
+                                        np.ascontiguousarray(aligned, np.int32),
+                                        rows.ctypes.data, stride, info), "cwhip_profile_align")
+    out = [rows[i, :info[1]].copy() if g[i + 1] in (1, 2) and info[1] > 0 else None for i in range(n)]
+    return info, g, out
 
 
 def msa_params(dna: bool = False, **kw) -> MsaParams:

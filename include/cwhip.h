@@ -192,6 +192,21 @@ int cwhip_msa(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t
               const int32_t *seq_weight, const int32_t *steps, int nsteps, int32_t *out_order,
               int32_t *aln_len);
 
+/* MyersMillerProfileAlign::profileAlign (multipleAlign/MyersMillerProfileAlign.cpp:
+ * 36-505) for one step: the two profiles of group (1-based, nseq+1 entries,
+ * 1 / 2 / 0) are built from the sequences (codes[offsets[s] .. + lens[s]])
+ * with the weights (nseq) and the similarity matrix ((nseq+1)^2, 1-based), the
+ * profile DP runs on the device, and every member's new row (info[1] codes)
+ * is written to rows_out + s*stride (stride >= info[2] + info[3] + 2).  As in
+ * the reference, group is modified (members with aligned[s] == 0 dropped, the
+ * groups swapped when group 2 is larger).  info: [0] profileAlign's return
+ * value (score / 100; 0 when a group is empty and nothing was aligned),
+ * [1] the new alignment length, [2] prfLength1, [3] prfLength2. */
+int cwhip_profile_align(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t *codes,
+                        const int64_t *offsets, const int32_t *lens, const double *sim,
+                        const int32_t *weights, int32_t *group, const int32_t *aligned,
+                        uint8_t *rows_out, int64_t stride, int32_t *info);
+
 /* Copies the rows of the last cwhip_msa (sequence s at out + s*stride,
  * aln_len codes each; gaps as code 30 or 31). */
 int cwhip_msa_rows(cwhip_ctx *ctx, uint8_t *out, int64_t stride);

@@ -22,3 +22,14 @@ def test_adapter_compiles_against_reference_headers(tmp_path):
                         "-I" + os.path.join(ROOT, "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC) or shutil.which("g++") is None,
+                    reason="reference headers not present")
+def test_profile_adapter_compiles_against_reference_headers(tmp_path):
+    src = os.path.join(ROOT, "cuda-clustalw_amd", "host", "HipMyersMillerProfileAlign.cpp")
+    r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-w",
+                        "-I" + os.path.join(REF_SRC, "multipleAlign"), "-I" + REF_SRC,
+                        "-I" + os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

@@ -123,3 +123,30 @@ def test_gpu_pipeline_reproduces_reference_dnd_and_aln(ctx, setname):
     out_dnd, out_aln, rows, order = pipeline.align(ctx, names, seqs, dna)
     assert out_dnd == dnd
     assert out_aln == text(setname + ".aln")
+
+
+@pytest.mark.parametrize("setname", ["100_97", "100_498", "100_1002"])
+def test_gpu_profile_align_step_by_step(ctx, setname):
+    """cwhip_profile_align called once per guide-tree step, as MSA.cpp:80-157
+    calls profileAlign (the fixtures have no divergent sequences, so the first
+    pass aligns everything): the final rows are the reference's."""
+    from cwhip.lib import msa_params, profile_align
+    print("profile_align steps", setname, flush=True)
+    names, codes, offs, lens, dna, dnd = load(setname)
+    n = len(names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    seqs = [codes[offs[i]:offs[i] + lens[i]].copy() for i in range(n)]
+    aligned = np.zeros(n + 1, np.int32)
+    p = msa_params(dna)
+    for s in steps:
+        for i in range(1, n + 1):
+            if s[i] != 0:
+                aligned[i] = 1
+        info, g, new = profile_align(ctx, seqs, sim, w, s, aligned, p)
+        assert info[0] >= 0
+        for i in range(n):
+            if new[i] is not None:
+                seqs[i] = new[i]
+    ref_order, ref_rows = parse_aln(text(setname + ".aln"))
+    for i, nm in enumerate(names):
+        assert row_text(seqs[i]) == ref_rows[nm], nm
