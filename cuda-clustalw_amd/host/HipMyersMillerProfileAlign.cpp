@@ -40,6 +40,15 @@ int HipMyersMillerProfileAlign::profileAlign(Alignment* alnPtr, DistMatrix* dist
 {
     const int numSeq = alnPtr->getNumSeqs();
     if (numSeq <= 0) return 0;
+    // the library carries the default series only: gonnet (matrixNum 3) and iub
+    // (DNAMatrixNum 1), SubMatrix.cpp:109-111
+    if ((!userParameters->getDNAFlag() && subMatrix->getMatrixNum() != 3) ||
+        (userParameters->getDNAFlag() && subMatrix->getDNAMatrixNum() != 1))
+    {
+        std::cerr << "The GPU profile alignment supports the default matrices only "
+                  << "(gonnet series, iub).\nNeed to terminate program.\n";
+        exit(1);
+    }
 
     cwhip_msa_params p;
     cwhip_msa_params_default(userParameters->getDNAFlag() ? 1 : 0, &p);
