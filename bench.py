@@ -124,6 +124,60 @@ def progressive_stage(cpu: bool):
     return out
 
 
+def serving_stage(rate: float, count: int, seed: int = 20260105):
+    """Poisson arrivals at `rate` requests/s of `count` serving requests (N ~ U[10,500],
+    L ~ lognormal(ln 300, 0.7) clipped [30, 2000]; 80% distances, 20% distances +
+    progressive alignment) into the library's request loop (4 slots, admission by
+    device footprint).  Latency is submission to completion; throughput counts
+    pairwise cells and Myers-Miller cell updates over the whole window.  Three
+    requests are then re-run alone on the direct path and compared."""
+    import cwhip
+    from cwhip import pipeline, seqgen
+    reqs = [seqgen.serving_request(seed, i) for i in range(count)]
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    arrive = np.cumsum(gaps) - gaps[0]
+    srv = cwhip.Server(0, slots=4, hbm_budget=128 << 30)
+    jobs = []
+    t0 = time.perf_counter()
+    for (kind, fam), ta in zip(reqs, arrive):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        jobs.append(srv.submit(kind, fam.seqs, fam.names))
+    for j in jobs:
+        srv.wait(j, 600)
+    times = np.array([srv.times(j) for j in jobs])
+    lat = times[:, 2] - times[:, 0]
+    window = times[:, 2].max() - times[:, 0].min()
+    cells = times[:, 3].sum()
+    st = srv.stats()
+    kinds = [k for k, _ in reqs]
+    # bit-exactness spot check against the direct path
+    same = True
+    ctx = cwhip.Context(0)
+    for i in (0, count // 2, count - 1):
+        kind, fam = reqs[i]
+        codes, offs, lens = fam.packed()
+        ctx.load_sequences(codes, offs, lens)
+        params = cwhip.pair_params(max_len=int(lens.max()))
+        _, _, dist = ctx.pairwise_range(params, 0, len(lens), 0, len(lens), kernel=cwhip.KERNEL_PLAIN)
+        same &= bool(np.array_equal(dist.view(np.uint32), srv.distances(jobs[i]).view(np.uint32)))
+        if kind == cwhip.JOB_ALIGNMENT:
+            dnd, aln, _, _ = pipeline.align(ctx, fam.names, fam.seqs, False)
+            same &= (dnd == srv.text(jobs[i], 0) and aln == srv.text(jobs[i], 1))
+    ctx.close()
+    srv.close()
+    return {"rate_per_s": rate, "requests": count, "alignment_jobs": int(sum(kinds)),
+            "distance_jobs": int(count - sum(kinds)),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "latency_mean_s": round(float(lat.mean()), 4),
+            "throughput_gcups": round(cells / window / 1e9, 2), "window_s": round(float(window), 3),
+            "max_concurrent_jobs": st["max_running"], "peak_footprint_bytes": st["peak_bytes"],
+            "slots": st["slots"], "spot_check_identical_to_direct_path": bool(same)}
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes (the HIP runtime's blit kernel): the
     reference dispatch whose FETCH_SIZE / WRITE_SIZE counters scale the
@@ -152,6 +206,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-progressive", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: requests per second")
+    ap.add_argument("--serve-count", type=int, default=80, help="serving: number of requests")
     ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (counter calibration)")
@@ -239,6 +296,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(codes, offs, lens, pairs, args.cpu_seconds)
     if not args.no_progressive:
         out["progressive"] = progressive_stage(not args.no_cpu)
+    if not args.no_serving:
+        out["serving"] = serving_stage(args.serve_rate, args.serve_count)
     print(json.dumps(out), flush=True)
 
 

@@ -285,7 +285,10 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     int64_t slots;
     int pers_blocks = 0;
     if (kernel == CWHIP_KERNEL_BATCHED) {
-        pers_blocks = ctx->num_cu * gpc;
+        // never more waves than pairs: a small job leaves the rest of the chip
+        // (and its scratch) to whatever else runs beside it
+        pers_blocks = (int)std::min<int64_t>((int64_t)ctx->num_cu * gpc,
+                                             (ctx->npairs + ctx->wpg - 1) / ctx->wpg);
         slots = (int64_t)pers_blocks * ctx->wpg;
     } else {
         slots = std::min<int64_t>(kPlainChunk, ctx->npairs);

@@ -22,7 +22,9 @@ EXPORTS = [
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
-    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_nj_tree", "cwhip_guide_from_tree", "cwhip_clustal_text",
+    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_nj_tree", "cwhip_server_create",
+    "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
+    "cwhip_server_text", "cwhip_server_times", "cwhip_server_release", "cwhip_server_stats", "cwhip_guide_from_tree", "cwhip_clustal_text",
 ]
 
 PRF_W = 35
@@ -110,8 +112,21 @@ def load() -> C.CDLL:
     L.cwhip_clustal_text.argtypes = [C.c_int, vp, P(np.uint8, flags="C"), C.c_int64,
                                      P(np.int32, flags="C"), P(np.int32, flags="C"), C.c_int,
                                      vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.cwhip_server_create.argtypes = [C.c_int, C.c_int, C.c_int64, C.POINTER(vp)]
+    L.cwhip_server_destroy.argtypes = [vp]
+    L.cwhip_server_destroy.restype = None
+    L.cwhip_server_submit.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(np.uint8, flags="C"),
+                                      P(np.int64, flags="C"), P(np.int32, flags="C"), vp,
+                                      C.POINTER(C.c_int64)]
+    L.cwhip_server_wait.argtypes = [vp, C.c_int64, C.c_double]
+    L.cwhip_server_distances.argtypes = [vp, C.c_int64, P(np.float32, flags="C")]
+    L.cwhip_server_text.argtypes = [vp, C.c_int64, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.cwhip_server_times.argtypes = [vp, C.c_int64, P(np.float64, flags="C")]
+    L.cwhip_server_release.argtypes = [vp, C.c_int64]
+    L.cwhip_server_stats.argtypes = [vp, P(np.float64, flags="C")]
     for name in EXPORTS:
-        if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance"):
+        if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance",
+                        "cwhip_server_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -350,6 +365,95 @@ def profile_align(ctx, seqs, sim, weights, group, aligned, params):
                                         rows.ctypes.data, stride, info), "cwhip_profile_align")
     out = [rows[i, :info[1]].copy() if g[i + 1] in (1, 2) and info[1] > 0 else None for i in range(n)]
     return info, g, out
+
+
+JOB_DISTANCES = 0
+JOB_ALIGNMENT = 1
+
+
+class Server:
+    """The library's request loop: jobs are admitted by device footprint into
+    `slots` concurrent device contexts (see include/cwhip.h)."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 64 << 30):
+        self._L = load()
+        h = C.c_void_p()
+        rc = self._L.cwhip_server_create(device, slots, hbm_budget, C.byref(h))
+        if rc:
+            raise CwhipError("cwhip_server_create failed: %d" % rc)
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cwhip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, kind, seqs, names=None, dna=False) -> int:
+        n = len(seqs)
+        lens = np.array([len(x) for x in seqs], np.int32)
+        offs = np.zeros(n, np.int64)
+        if n > 1:
+            offs[1:] = np.cumsum(lens[:-1])
+        codes = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint8) for x in seqs]))
+        arr = _names(names) if names is not None else None
+        job = C.c_int64()
+        rc = self._L.cwhip_server_submit(self._h, kind, int(dna), n, codes, offs, lens,
+                                         C.cast(arr, C.c_void_p) if arr is not None else None,
+                                         C.byref(job))
+        if rc:
+            raise CwhipError("cwhip_server_submit failed: %d" % rc)
+        self._n[job.value] = n
+        return job.value
+
+    def wait(self, job: int, timeout: float = -1.0) -> bool:
+        rc = self._L.cwhip_server_wait(self._h, job, timeout)
+        if rc == -7:
+            return False
+        if rc:
+            raise CwhipError("job %d failed: %d" % (job, rc))
+        return True
+
+    def distances(self, job: int) -> np.ndarray:
+        n = self._n[job]
+        d = np.zeros(max(n * (n - 1) // 2, 1), np.float32)
+        rc = self._L.cwhip_server_distances(self._h, job, d)
+        if rc:
+            raise CwhipError("cwhip_server_distances failed: %d" % rc)
+        return d[:n * (n - 1) // 2]
+
+    def text(self, job: int, which: int) -> str:
+        ln = C.c_int64()
+        self._L.cwhip_server_text(self._h, job, which, None, 0, C.byref(ln))
+        buf = C.create_string_buffer(ln.value + 1)
+        rc = self._L.cwhip_server_text(self._h, job, which, C.cast(buf, C.c_void_p), ln.value + 1,
+                                       C.byref(ln))
+        if rc:
+            raise CwhipError("cwhip_server_text failed: %d" % rc)
+        return buf.value.decode()
+
+    def times(self, job: int):
+        t = np.zeros(4, np.float64)
+        rc = self._L.cwhip_server_times(self._h, job, t)
+        if rc:
+            raise CwhipError("cwhip_server_times failed: %d" % rc)
+        return tuple(float(x) for x in t)
+
+    def release(self, job: int):
+        self._L.cwhip_server_release(self._h, job)
+        self._n.pop(job, None)
+
+    def stats(self) -> dict:
+        st = np.zeros(6, np.float64)
+        self._L.cwhip_server_stats(self._h, st)
+        return dict(done=int(st[0]), cells=float(st[1]), peak_bytes=int(st[2]),
+                    max_running=int(st[3]), budget=int(st[4]), slots=int(st[5]))
 
 
 def msa_params(dna: bool = False, **kw) -> MsaParams:

@@ -192,6 +192,19 @@ def lognormal_lengths(seed: int, n: int, mu: float, sigma: float, lo: int, hi: i
 
 # --- the named workloads -------------------------------------------------
 
+def serving_request(seed: int, index: int, n_range=(10, 500), len_cap: int = 2000):
+    """One request of the serving stream: (kind, Family); kind 1 (distances +
+    progressive alignment) with probability 0.2, else 0 (distances only).
+    N ~ Uniform[n_range], lengths lognormal(ln 300, 0.7) clipped to [30, len_cap],
+    the sequences evolved along a seeded tree and cut to those lengths."""
+    rng = Rng(seed * 1000003 + index)
+    n = int(rng.integers(n_range[0], n_range[1] + 1, 1)[0])
+    kind = 1 if float(rng.uniform(1)[0]) < 0.2 else 0
+    tl = lognormal_lengths(seed * 7919 + index, n, math.log(300.0), 0.7, 30, len_cap)
+    fam = family(seed * 31 + index, n, int(tl.max()), target_lens=tl, name_prefix="r%d_" % index)
+    return kind, fam
+
+
 def workload(name: str, seed: int | None = None) -> Family:
     """Named synthetic workloads.
 

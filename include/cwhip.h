@@ -39,7 +39,8 @@ enum {
     CWHIP_E_NOSEQ = -3,     /* no sequences loaded */
     CWHIP_E_TOOLONG = -4,   /* a sequence exceeds CWHIP_MAX_LEN */
     CWHIP_E_KERNEL = -5,    /* a kernel reported an internal inconsistency */
-    CWHIP_E_NOMEM = -6      /* device allocation failed */
+    CWHIP_E_NOMEM = -6,     /* device allocation failed, or a job larger than the HBM budget */
+    CWHIP_E_TIMEOUT = -7    /* cwhip_server_wait: the job did not finish in time */
 };
 
 #define CWHIP_MAX_LEN 16000  /* longest sequence the kernels accept */
@@ -248,6 +249,48 @@ int cwhip_guide_from_tree(int nseq, const char *dnd, const char *const *names, i
 int cwhip_clustal_text(int nseq, const char *const *names, const uint8_t *rows, int64_t stride,
                        const int32_t *lens, const int32_t *order, int dna, char *out, int64_t cap,
                        int64_t *len);
+
+
+/* ------------------------------------------------------------------------ */
+/* Serving: jobs of different sizes arriving over time                      */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference, which runs one alignment per process; the
+ * jobs run the entry points above (the reference operations they replace are
+ * listed there).  A server owns `slots` device contexts, each with its own
+ * stream and worker thread.  A submitted job is admitted, first fit in arrival
+ * order, as soon as a slot is free and its estimated device footprint fits in
+ * what hbm_budget bytes have left; admitted jobs run concurrently and their
+ * kernels overlap on the device.  Results do not depend on the mix. */
+
+typedef struct cwhip_server cwhip_server;
+
+enum {
+    CWHIP_JOB_DISTANCES = 0,  /* the DistMatrix of all pairs (FullPairwiseAlign over 0..n) */
+    CWHIP_JOB_ALIGNMENT = 1   /* distances + NJ tree (.dnd) + progressive alignment (.aln) */
+};
+
+int cwhip_server_create(int device, int slots, int64_t hbm_budget, cwhip_server **out);
+/* stops the workers after their running jobs; pending jobs are dropped */
+void cwhip_server_destroy(cwhip_server *srv);
+/* copies the inputs; names are needed for CWHIP_JOB_ALIGNMENT.  Returns
+ * CWHIP_E_NOMEM if the job alone exceeds the budget. */
+int cwhip_server_submit(cwhip_server *srv, int kind, int dna, int nseq, const uint8_t *codes,
+                        const int64_t *offsets, const int32_t *lens, const char *const *names,
+                        int64_t *job);
+/* waits up to timeout_s seconds (negative: no limit); CWHIP_OK when the job is done */
+int cwhip_server_wait(cwhip_server *srv, int64_t job, double timeout_s);
+/* nseq*(nseq-1)/2 float distances in the pair order of the range 0..nseq */
+int cwhip_server_distances(cwhip_server *srv, int64_t job, float *dist);
+/* which = 0: the .dnd text, 1: the .aln text (sizing as cwhip_nj_tree) */
+int cwhip_server_text(cwhip_server *srv, int64_t job, int which, char *out, int64_t cap,
+                      int64_t *len);
+/* [0] submitted, [1] started, [2] finished (seconds since the server was created),
+ * [3] conventional pairwise cells + Myers-Miller cell updates of the job */
+int cwhip_server_times(cwhip_server *srv, int64_t job, double *t4);
+int cwhip_server_release(cwhip_server *srv, int64_t job);
+/* [0] jobs finished, [1] cells, [2] peak admitted footprint (bytes), [3] most jobs
+ * running at once, [4] budget (bytes), [5] slots */
+int cwhip_server_stats(cwhip_server *srv, double *out6);
 
 #ifdef __cplusplus
 }

@@ -71,3 +71,12 @@ def test_no_gpu_context_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(cwhip.CwhipError):
         cwhip.Context(0)
+
+
+def test_server_needs_a_device():
+    """No CPU fallback: the request loop cannot start without a device."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(cwhip.CwhipError):
+        cwhip.Server(0, slots=2, hbm_budget=1 << 30)
