@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-progressive", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--stage", choices=["all", "progressive", "serving"], default="all",
+                    help="progressive / serving: run only that part and print its object")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: requests per second")
     ap.add_argument("--serve-count", type=int, default=80, help="serving: number of requests")
     ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
@@ -220,6 +222,12 @@ def main():
         print("bench.py measures one GPU only: the pairwise stage shards by pair lists "
               "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.stage == "progressive":
+        print(json.dumps({"progressive": progressive_stage(not args.no_cpu)}), flush=True)
+        return
+    if args.stage == "serving":
+        print(json.dumps({"serving": serving_stage(args.serve_rate, args.serve_count)}), flush=True)
+        return
 
     import cwhip
     from cwhip import seqgen

@@ -69,6 +69,10 @@ __device__ __forceinline__ void wave_fence() {
 }
 __device__ __forceinline__ int max3(int a, int b, int c) { return max(max(a, b), c); }
 
+// LDS address space: staged reads compile to ds_read_* rather than flat loads
+#define CWP_LDS __attribute__((address_space(3)))
+typedef CWP_LDS int lint;
+
 // One profile pair as the device sees it.
 struct Prob {
     int L1, L2, ldS;
@@ -110,6 +114,8 @@ struct PSweep {
     //   reverse: (go2 ? -open2(I(i), jedge) : 0) - (m2[jedge] & (pl1[ib] - pl1[I(i)]))
     int lt0, ltl;
     int go2, jedge;
+    lint* stage;       // this wave's LDS buffer (CWP_STAGE_INTS ints)
+    int jrec0;         // column of staged record 0 (B): staged record x is column jrec0 + x
 };
 
 template <int MODE>
@@ -128,7 +134,7 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 // R-1 rows behind a pass-through row).  The top boundary row comes from inH
 // (inE, or -inf on the first stripe); lane 63's rows leave through outH/outE.
 // DIRECT: R == 1, P < 64, lane P-1 writes its row straight to out.
-template <int R, int MODE, bool DIRECT>
+template <int R, int MODE, bool DIRECT, bool STAGED = false>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
                                         const int* inH, const int* inE, int* outH, int* outE)
 {
@@ -187,11 +193,24 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
     int nsv[R];
     auto prefetch = [&](int c) {
         const int J = colJ(c);
-        nrecF = p.col[J];
-        nrecE = (MODE == MODE_FWD) ? nrecF : p.col[J - 1];
-        const int* Sc = p.S + (int64_t)J * ldS;
+        if constexpr (STAGED) {
+            // single stripe, one row per lane: records at x = J - jrec0, scores
+            // column-major over (local column, local row)
+            const lint* lrec = W.stage;
+            const int x = J - W.jrec0;
+            nrecF = make_int4(lrec[4 * x], lrec[4 * x + 1], lrec[4 * x + 2], lrec[4 * x + 3]);
+            if (MODE == MODE_FWD) nrecE = nrecF;
+            else nrecE = make_int4(lrec[4 * x - 4], lrec[4 * x - 3], lrec[4 * x - 2], lrec[4 * x - 1]);
+            const int cc = c < 1 ? 1 : (c > ncols ? ncols : c);
+            const int rr = firstrow < 1 ? 1 : (firstrow > W.nrows ? W.nrows : firstrow);
+            nsv[0] = W.stage[4 * (ncols + 2) + (cc - 1) * W.nrows + (rr - 1)];
+        } else {
+            nrecF = p.col[J];
+            nrecE = (MODE == MODE_FWD) ? nrecF : p.col[J - 1];
+            const int* Sc = p.S + (int64_t)J * ldS;
 #pragma unroll
-        for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
+            for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
+        }
     };
     prefetch(1 - lane);
 
@@ -291,6 +310,7 @@ __device__ void run_pstripe(int R, bool direct, const PSweep& W, int row0, int P
                             const int* inH, const int* inE, int* outH, int* outE)
 {
     if (direct) { pstripe<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE); return; }
+    if (R == 0) { pstripe<1, MODE, true, true>(W, row0, P, first, inH, inE, outH, outE); return; }
     switch (R) {
     case 1: pstripe<1, MODE, false>(W, row0, P, first, inH, inE, outH, outE); break;
     case 2: pstripe_call<2, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
@@ -307,10 +327,49 @@ struct Bufs { int *h0, *e0, *h1, *e1; };
 
 // Whole half sweep in balanced stripes; the top boundary row must already be
 // in g.h0[1..ncols].  The final row goes to finH/finE[1..ncols].
+// Loads a single-stripe node's column records (columns jrec0 .. jrec0+N+1) and
+// its nrows x N score block into this wave's LDS buffer.
+template <int MODE>
+__device__ void stage_node(const PSweep& W)
+{
+    const int lane = lane_id();
+    const Prob& p = W.p;
+    const int N = W.ncols, P = W.nrows;
+    lint* st = W.stage;
+    for (int x = lane; x < N + 2; x += WAVE) {
+        const int4 r = p.col[W.jrec0 + x];
+        st[4 * x + 0] = r.x;
+        st[4 * x + 1] = r.y;
+        st[4 * x + 2] = r.z;
+        st[4 * x + 3] = r.w;
+    }
+    lint* lS = st + 4 * (N + 2);
+    const int per = WAVE / P;                 // columns per pass (P < 64)
+    const int cc = lane / P, rr = lane - cc * P;
+    if (cc < per) {
+        const int I = W.ib + W.id * (rr + 1);
+        for (int c0 = 0; c0 < N; c0 += per) {
+            const int c = c0 + cc;
+            if (c < N) {
+                const int J = W.jb + W.jd * (c + 1);
+                lS[c * P + rr] = p.S[(int64_t)J * p.ldS + I];
+            }
+        }
+    }
+    wave_fence();
+}
+
 template <int MODE>
 __device__ void psweep(const PSweep& W, int rmax, const Bufs& g, int* finH, int* finE)
 {
     const int nrows = W.nrows;
+    if (nrows < WAVE && nrows * W.ncols + 4 * (W.ncols + 2) <= CWP_STAGE_INTS) {
+        // one stripe, one row per lane, from LDS
+        stage_node<MODE>(W);
+        run_pstripe<MODE>(0, false, W, 0, nrows, 1, g.h0, g.e0, finH, finE);
+        wave_fence();
+        return;
+    }
     const int per = WAVE * rmax;
     const int nst = uni((nrows + per - 1) / per);
     const int base = nrows / nst, rem = nrows % nst;
@@ -355,7 +414,7 @@ __device__ void fill_top(const Prob& p, int A, int B, int M, int N, int* h0)
 // shuffle then reads an inactive lane) while others wait at its exit.  Kept
 // out of line for the same reason.
 __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is_root, int rmax, int* scr, int rowcap,
-                        int* out_score)
+                        int* out_score, lint* stage)
 {
     const int lane = lane_id();
     const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
@@ -401,6 +460,7 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
     F.ltl = -ext2(p, A + 1, B);
     F.go2 = 0; F.jedge = 0;
+    F.stage = stage; F.jrec0 = B;
     fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
     psweep<MODE_FWD>(F, rmax, g, fH, fE);
 
@@ -410,6 +470,7 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
     Rv.lt0 = 0; Rv.ltl = 0;
     Rv.go2 = go2; Rv.jedge = B + N;
+    Rv.stage = stage; Rv.jrec0 = B;
     fill_top<MODE_REV>(p, A, B, M, N, g.h0);
     psweep<MODE_REV>(Rv, rmax, g, rH, rK);
     wave_fence();
@@ -511,6 +572,7 @@ extern "C" __global__ void __launch_bounds__(256) cw_prf_score(PrfArgs a)
 extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 {
     __shared__ int s_lev_b, s_lev_e, s_ctr, s_err;
+    __shared__ int s_stage[CWP_WAVES * CWP_STAGE_INTS];
     const int prob = blockIdx.x;
     const PrfProb pp = a.probs[prob];
     const Prob p = make_prob(a, pp);
@@ -540,7 +602,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             const int idx = uni(__shfl(got, 0, WAVE));
             if (idx >= le) break;
             do_node(p, nodes + (int64_t)idx * CWP_NODE_INTS, idx == 0, a.rmax, scr, pp.rowcap,
-                    a.out + 4 * prob);
+                    a.out + 4 * prob, (lint*)(s_stage + wave * CWP_STAGE_INTS));
         }
         __syncthreads();
         if (wave == 0) {

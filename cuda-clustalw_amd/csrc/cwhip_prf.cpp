@@ -12,7 +12,9 @@
 // host: without a device cwhip_msa fails like every other entry point.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -502,11 +504,16 @@ void calc_profile_with_sub(ProfileSide& s, const std::vector<const uint8_t*>& ro
         }
         column_weights(rows, w, s.first, s.last, pos, wt);
         const float scale = (float)(numSeq - gaps[pos]) / (float)numSeq;
+        // the weighted sum over the codes present in the column (integer, so
+        // the order and the zero terms the reference also adds do not matter)
+        int present[NUMRES], np = 0;
+        for (int aa = 0; aa <= MAXAA; ++aa)
+            if (wt[aa]) present[np++] = aa;
+        if (wt[GAP1]) present[np++] = GAP1;
+        if (wt[GAP2]) present[np++] = GAP2;
         auto one = [&](int res) {
             int f = 0;
-            for (int aa = 0; aa <= MAXAA; ++aa) f += wt[aa] * mat[aa][res];
-            f += wt[GAP1] * mat[GAP1][res];
-            f += wt[GAP2] * mat[GAP2][res];
+            for (int k = 0; k < np; ++k) f += wt[present[k]] * mat[present[k]][res];
             return (int)(((float)f / (float)sum2) * scale);
         };
         for (int res = 0; res <= MAXAA; ++res) s.at(pos + 1, res) = one(res);
@@ -534,6 +541,26 @@ void calc_standard_profile(ProfileSide& s, const std::vector<const uint8_t*>& ro
         s.at(r + 1, GAP1) = (int)(10 * (float)wt[GAP1] / (float)sum2);
         s.at(r + 1, GAP2) = (int)(10 * (float)wt[GAP2] / (float)sum2);
     }
+}
+
+// fn(0..n-1) on up to 16 host threads (the caller's thread included)
+template <typename F>
+void parallel_for(size_t n, F fn)
+{
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    const size_t nt = std::min(hw, n);
+    if (nt <= 1) {
+        for (size_t k = 0; k < n; ++k) fn(k);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto body = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < n;) fn(k);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(body);
+    body();
+    for (auto& t : th) t.join();
 }
 
 // One profileAlign call (MyersMillerProfileAlign.cpp:36-505) split into the
@@ -851,25 +878,28 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         if (level[s] >= 0) by_level[level[s]].push_back(s);
     std::vector<int> all_aligned(nseq + 1, 1);
     for (int lv = 0; lv < nlev; ++lv) {
-        std::vector<Step> sts(by_level[lv].size());
+        const size_t nst = by_level[lv].size();
+        std::vector<Step> sts(nst);
+        std::vector<char> ok(nst, 0);
+        // the steps of a level touch disjoint sequences: build their profiles
+        // (and later apply their edit scripts) on several host threads
+        parallel_for(nst, [&](size_t k) {
+            std::vector<int> grp = groups[by_level[lv][k]];
+            ok[k] = prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k]) ? 1 : 0;
+        });
         std::vector<DpJob> jobs;
         std::vector<int> live;
-        for (size_t k = 0; k < by_level[lv].size(); ++k) {
-            std::vector<int> grp = groups[by_level[lv][k]];
-            if (prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k])) {
+        for (size_t k = 0; k < nst; ++k)
+            if (ok[k]) {
                 live.push_back((int)k);
                 jobs.push_back(sts[k].job);
             }
-        }
         const int rc = run_dp_batch(ctx, jobs, P.end_gap_penalties ? 1 : 0);
         if (rc != CWHIP_OK) return rc;
         ++batches;
-        for (size_t q = 0; q < live.size(); ++q) {
-            Step& st = sts[live[q]];
-            st.job = jobs[q];
-            apply_step(st, seqs);
-            ++calls;
-        }
+        for (size_t q = 0; q < live.size(); ++q) sts[live[q]].job = jobs[q];
+        parallel_for(live.size(), [&](size_t q) { apply_step(sts[live[q]], seqs); });
+        calls += (int)live.size();
     }
     // Second pass: the delayed (divergent) sequences one by one (:159-335).
     std::vector<int> treeWeight(seq_weight, seq_weight + nseq), group(nseq + 1, 0);
