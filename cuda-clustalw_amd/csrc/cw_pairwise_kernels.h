@@ -8,6 +8,8 @@
 #define CW_TAB_BYTES 4096      // int32 [32][32] substitution matrix in LDS
 #define CW_MAX_THREADS 256     // threads per workgroup (4 waves)
 #define CW_RMAX 8              // rows per lane the kernels are instantiated for
+#define CW_NCLASS 6            // tuning classes by the longer sequence: <=128 <=256 <=512 <=1024 <=2048 longer
+#define CW_NBINS 136           // work-queue bins: 4 per octave of the DP area n*m (n, m < 2^16)
 
 struct DevArgs {
     const uint8_t* codes;      // packed residue codes of all sequences
@@ -31,7 +33,28 @@ struct DevArgs {
     int32_t* out_info;         // optional per pair [maxScore,se1,se2,sb1,sb2,status,nodes,mm cells]
     int32_t* status;           // OR of per-pair status bits (0 = consistent)
     unsigned int* queue;       // persistent kernel work counter
+    int32_t* bins;             // batched queue: CW_NBINS counts, then CW_NBINS cursors
+    int32_t* order_out;        // batched queue: the binned work order the persistent kernel pops
+    int32_t rmax_cls[CW_NCLASS];   // stripe rows per lane by tuning class
+    int32_t colcap_cls[CW_NCLASS]; // Myers-Miller nodes with N <= this use the column path (<= CW_SMALL_COLS)
 };
+
+// Tuning class of a pair: by the longer sequence.
+static inline __host__ __device__ int cw_class_of(int n, int m)
+{
+    const int l = n > m ? n : m;
+    return l <= 128 ? 0 : l <= 256 ? 1 : l <= 512 ? 2 : l <= 1024 ? 3 : l <= 2048 ? 4 : 5;
+}
+
+// Queue bin of a pair: 4 bins per octave of n*m, larger area = larger bin.
+static inline __host__ __device__ int cw_bin_of(int n, int m)
+{
+    const uint64_t a = (uint64_t)n * (uint64_t)m;
+    if (a < 4) return (int)a;
+    const int lg = 63 - __builtin_clzll(a);
+    const int b = 4 * lg + (int)((a >> (lg - 2)) & 3);
+    return b < CW_NBINS ? b : CW_NBINS - 1;
+}
 
 static inline int cw_wave_lds_bytes(int lcap)
 {

@@ -400,8 +400,8 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
 __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
                         int gap1, int gap2, int sb1, int sb2, int se1, int se2, int rmax,
                         const Bufs& g, int* g2H, int* g2E, int* g3H, int* g3E,
-                        int* smallF, int* smallR, int small_cap, int* stack, int stack_cap,
-                        int* status, int* nodes_out, int* cells_out)
+                        int* smallF, int* smallR, int small_cap, int colcap, int* stack,
+                        int stack_cap, int* status, int* nodes_out, int* cells_out)
 {
     const int lane = lane_id();
     int count = 0, sp = 0, nodes = 0, cells = 0;
@@ -445,7 +445,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         cells += M * N;
         const int midi = M / 2;
         int *fH, *fE, *rH, *rE;          // HH/DD[j] at [j]; RR/SS[j] at [N-j]
-        if (N <= 32 * 8) {
+        if (N <= colcap) {
             fH = smallF; fE = smallF + (small_cap + 1);
             rH = smallR; rE = smallR + (small_cap + 1);
             if (N <= 32) mm_halves_cols<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
@@ -562,6 +562,9 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
 
     const int go = A.go_by_len[n < m ? n : m];
     const int ge = A.ge;
+    const int cls = cw_class_of(n, m);
+    const int rmax = uni(A.rmax_cls[cls]);
+    const int colcap = uni(A.colcap_cls[cls]);
     Sweep S;
     S.tab = tab;
     S.go = go; S.ge = ge; S.goe = go + ge;
@@ -575,7 +578,7 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
     F.th0 = 0; F.ths = 0; F.teo = -go;
     F.lh0 = 0; F.lhs = 0; F.lfo = -go;
     Track T{0, 0};
-    sweep<MODE_FWD>(F, A.rmax, g, nullptr, nullptr, T);
+    sweep<MODE_FWD>(F, rmax, g, nullptr, nullptr, T);
     const int maxScore = uni(T.best);
     int se1 = 0, se2 = 0, sb1 = 1, sb2 = 1, status = 0;
     if (maxScore > 0) {
@@ -590,7 +593,7 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
         Rv.lh0 = -1; Rv.lhs = 0; Rv.lfo = 0;
         Rv.thr = maxScore;
         Track U{0, 0};
-        sweep<MODE_REV>(Rv, A.rmax, g, nullptr, nullptr, U);
+        sweep<MODE_REV>(Rv, rmax, g, nullptr, nullptr, U);
         if (U.best) {
             const int r = uni(U.key >> 16), c = uni(U.key & 0xffff);
             sb1 = se1 - r + 1;
@@ -602,8 +605,8 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
     int nodes = 0, mmcells = 0;
     int count = 0;
     if (se1 - sb1 + 1 > 0 && se2 - sb2 + 1 > 0 && maxScore > 0) {
-        count = mm_count(S, s1, s2, go, ge, A.gap1, A.gap2, sb1, sb2, se1, se2, A.rmax, g, g2H,
-                         g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, stack, CW_STACK_CAP,
+        count = mm_count(S, s1, s2, go, ge, A.gap1, A.gap2, sb1, sb2, se1, se2, rmax, g, g2H,
+                         g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, colcap, stack, CW_STACK_CAP,
                          A.status, &nodes, &mmcells);
     }
     if (lane == 0) {
@@ -682,6 +685,98 @@ extern "C" __global__ void cw_reset_counter(unsigned int* q, int32_t* status)
 hipError_t cw_launch_reset(hipStream_t st, unsigned int* q, int32_t* status)
 {
     hipLaunchKernelGGL(cw_reset_counter, dim3(1), dim3(64), 0, st, q, status);
+    return hipGetLastError();
+}
+
+// ---- length-binned work queue of the batched kernel --------------------------
+// A counting sort of the pairs by cw_bin_of (4 bins per octave of n*m), largest
+// bins first, built on the device in three launches: count, scan, scatter.
+// Within a bin the order is whatever the atomics give; every pair's result is
+// independent of the order, so the output does not depend on it.
+
+constexpr int kBinItems = 1024;       // pairs per workgroup in count/scatter
+
+extern "C" __global__ void __launch_bounds__(256) cw_bin_reset(DevArgs A)
+{
+    for (int k = threadIdx.x; k < 2 * CW_NBINS; k += blockDim.x) A.bins[k] = 0;
+    if (threadIdx.x == 0) { *A.queue = 0u; *A.status = 0; }
+}
+
+__device__ __forceinline__ int pair_bin(const DevArgs& A, int64_t p)
+{
+    return cw_bin_of(A.lens[A.pairs[2 * p]], A.lens[A.pairs[2 * p + 1]]);
+}
+
+extern "C" __global__ void __launch_bounds__(256) cw_bin_count(DevArgs A)
+{
+    __shared__ int cnt[CW_NBINS];
+    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kBinItems;
+    for (int k = threadIdx.x; k < kBinItems; k += blockDim.x) {
+        const int64_t p = base + k;
+        if (p < A.pair_end) atomicAdd(&cnt[pair_bin(A, p)], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x)
+        if (cnt[k]) atomicAdd(&A.bins[k], cnt[k]);
+}
+
+// cursors: bin CW_NBINS-1 starts at 0, each smaller bin after the larger ones
+extern "C" __global__ void __launch_bounds__(64) cw_bin_scan(DevArgs A)
+{
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int b = CW_NBINS - 1; b >= 0; --b) {
+            A.bins[CW_NBINS + b] = run;
+            run += A.bins[b];
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256) cw_bin_scatter(DevArgs A)
+{
+    __shared__ int cnt[CW_NBINS], at[CW_NBINS];
+    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kBinItems;
+    constexpr int per = kBinItems / 256;
+    int rank[per], bin[per];
+#pragma unroll
+    for (int u = 0; u < per; ++u) {
+        const int64_t p = base + threadIdx.x + 256 * u;
+        bin[u] = -1;
+        if (p < A.pair_end) {
+            bin[u] = pair_bin(A, p);
+            rank[u] = atomicAdd(&cnt[bin[u]], 1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x)
+        at[k] = cnt[k] ? atomicAdd(&A.bins[CW_NBINS + k], cnt[k]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < per; ++u)
+        if (bin[u] >= 0) A.order_out[at[bin[u]] + rank[u]] = (int32_t)(base + threadIdx.x + 256 * u);
+}
+
+// The batched run's stages on one stream: reset, count, scan, scatter, then
+// the persistent kernel over the binned order (captured into one hipGraph by
+// the host).
+hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a)
+{
+    const int nb = (int)((a.pair_end + kBinItems - 1) / kBinItems);
+    hipLaunchKernelGGL(cw_bin_reset, dim3(1), dim3(256), 0, st, a);
+    if (nb > 0) {
+        hipLaunchKernelGGL(cw_bin_count, dim3(nb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(cw_bin_scan, dim3(1), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(cw_bin_scatter, dim3(nb), dim3(256), 0, st, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    DevArgs b = a;
+    b.order = a.order_out;
+    hipLaunchKernelGGL(cw_pairwise_persistent, dim3(blocks), dim3(threads), lds, st, b);
     return hipGetLastError();
 }
 

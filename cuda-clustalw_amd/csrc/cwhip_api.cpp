@@ -19,11 +19,39 @@
 hipError_t cw_launch_plain(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
 hipError_t cw_launch_persistent(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
 hipError_t cw_launch_reset(hipStream_t st, unsigned int* q, int32_t* status);
+hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
 hipError_t cw_occupancy(int threads, size_t lds, int* plain_blocks, int* pers_blocks);
+
+#include <map>
+#include <mutex>
 
 namespace {
 
 constexpr int kPlainChunk = 16384;   // pairs per plain-kernel launch (scratch slots)
+
+// Built-in batched shapes: every class at the plain kernel's stripe height and
+// the full column path.  cwhip_autotune replaces them per device.
+cwhip_tune_table default_table(int rmax)
+{
+    cwhip_tune_table t;
+    for (int c = 0; c < CWHIP_TUNE_CLASSES; ++c) {
+        t.rmax[c] = rmax;
+        t.colcap[c] = CW_SMALL_COLS;
+    }
+    return t;
+}
+
+// tables found by cwhip_autotune, by device: contexts created later start from them
+std::mutex g_tune_mu;
+std::map<int, cwhip_tune_table> g_tuned;
+
+bool table_ok(const cwhip_tune_table& t)
+{
+    for (int c = 0; c < CWHIP_TUNE_CLASSES; ++c)
+        if (t.rmax[c] < 1 || t.rmax[c] > CW_RMAX || t.colcap[c] < 32 || t.colcap[c] > CW_SMALL_COLS)
+            return false;
+    return true;
+}
 
 } // namespace
 
@@ -41,6 +69,12 @@ int cwhip_create(int device, cwhip_ctx** out)
     e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) { delete ctx; return CWHIP_E_HIP; }
     ctx->num_cu = prop.multiProcessorCount;
+    ctx->tune = default_table(ctx->rmax);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(device);
+        if (it != g_tuned.end()) ctx->tune = it->second;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipEventCreate(&ctx->ev2) != hipSuccess) {
@@ -59,6 +93,8 @@ void cwhip_destroy(cwhip_ctx* ctx)
     ctx->d_pairs.release(); ctx->d_order.release(); ctx->d_count.release();
     ctx->d_info.release(); ctx->d_matrix.release(); ctx->d_go.release();
     ctx->d_status.release(); ctx->d_scratch.release(); ctx->d_queue.release();
+    ctx->d_bins.release();
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -194,7 +230,10 @@ int cwhip_set_tuning(cwhip_ctx* ctx, int rmax, int waves_per_group, int groups_p
     if (rmax < 0 || rmax > CW_RMAX || waves_per_group < 0 || waves_per_group > CW_MAX_THREADS / 64 ||
         groups_per_cu < 0)
         return fail(ctx, CWHIP_E_ARG, "tuning value out of range");
-    if (rmax) ctx->rmax = rmax;
+    if (rmax) {
+        ctx->rmax = rmax;
+        for (int c = 0; c < CWHIP_TUNE_CLASSES; ++c) ctx->tune.rmax[c] = rmax;
+    }
     if (waves_per_group) ctx->wpg = waves_per_group;
     ctx->groups_per_cu = groups_per_cu;
     return CWHIP_OK;
@@ -217,14 +256,7 @@ int cwhip_stage_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t*
     ctx->gap1 = p->gap_pos1;
     ctx->gap2 = p->gap_pos2;
     ctx->ge = p->gap_extend;
-    // work order of the persistent kernel: longest-processing-time first, by
-    // the pair's DP area, so the queue drains evenly
-    std::vector<int32_t> order(npairs);
-    std::iota(order.begin(), order.end(), 0);
-    std::vector<int64_t> cost(npairs);
-    for (int64_t k = 0; k < npairs; ++k)
-        cost[k] = (int64_t)ctx->h_lens[pairs[2 * k]] * ctx->h_lens[pairs[2 * k + 1]];
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    // the batched kernel's work order is built on the device at run time
     HIPCHK(ctx->d_pairs.ensure(2 * npairs + 2));
     HIPCHK(ctx->d_order.ensure(npairs + 1));
     HIPCHK(ctx->d_count.ensure(npairs + 1));
@@ -233,10 +265,9 @@ int cwhip_stage_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t*
     HIPCHK(ctx->d_go.ensure(p->max_len + 1));
     HIPCHK(ctx->d_status.ensure(4));
     HIPCHK(ctx->d_queue.ensure(4));
-    if (npairs) {
+    HIPCHK(ctx->d_bins.ensure(2 * CW_NBINS));
+    if (npairs)
         HIPCHK(hipMemcpyAsync(ctx->d_pairs.p, pairs, 2 * npairs * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->d_order.p, order.data(), npairs * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    }
     HIPCHK(hipMemcpyAsync(ctx->d_matrix.p, p->matrix, 32 * 32 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_go.p, p->gap_open_by_len, (p->max_len + 1) * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -265,6 +296,64 @@ static void fill_args(cwhip_ctx* ctx, DevArgs& a)
     a.out_info = ctx->d_info.p;
     a.status = ctx->d_status.p;
     a.queue = ctx->d_queue.p;
+    a.bins = ctx->d_bins.p;
+    a.order_out = ctx->d_order.p;
+    for (int c = 0; c < CW_NCLASS; ++c) {
+        a.rmax_cls[c] = ctx->rmax;
+        a.colcap_cls[c] = CW_SMALL_COLS;
+    }
+}
+
+// The batched stages as one graph launch; the graph is captured again whenever
+// the launch it was captured for changes (buffers, sizes, shapes).
+static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int threads, size_t lds)
+{
+    if (!ctx->use_graph) {
+        HIPCHK(cw_launch_batched(blocks, threads, lds, ctx->stream, a));
+        return CWHIP_OK;
+    }
+    std::vector<uint8_t> key(sizeof(DevArgs) + 3 * sizeof(int64_t));
+    std::memcpy(key.data(), &a, sizeof(DevArgs));
+    const int64_t dims[3] = {blocks, threads, (int64_t)lds};
+    std::memcpy(key.data() + sizeof(DevArgs), dims, sizeof dims);
+    if (!ctx->gexec || key != ctx->gkey) {
+        if (ctx->gexec) {
+            HIPCHK(hipGraphExecDestroy(ctx->gexec));
+            ctx->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 = cw_launch_batched(blocks, threads, lds, ctx->stream, a);
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hipfail(ctx, e1 != hipSuccess ? e1 : e2, "capturing the batched stages");
+        }
+        const hipError_t e3 = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) {
+            ctx->gexec = nullptr;
+            return hipfail(ctx, e3, "hipGraphInstantiate");
+        }
+        ctx->gkey = key;
+    }
+    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    return CWHIP_OK;
+}
+
+// Persistent workgroups of a batched launch: as many as fit, never more waves
+// than pairs (a small job leaves the rest of the chip to whatever runs beside it).
+static int batched_shape(cwhip_ctx* ctx, const DevArgs& a, int64_t npairs, int* blocks, int* threads, size_t* lds)
+{
+    *threads = 64 * ctx->wpg;
+    *lds = CW_TAB_BYTES + (size_t)ctx->wpg * a.wave_lds;
+    if (*lds > 160 * 1024) return fail(ctx, CWHIP_E_TOOLONG, "per-workgroup LDS exceeds 160 KiB");
+    int occ_plain = 0, occ_pers = 0;
+    HIPCHK(cw_occupancy(*threads, *lds, &occ_plain, &occ_pers));
+    if (occ_pers < 1 || occ_plain < 1) return fail(ctx, CWHIP_E_TOOLONG, "kernel does not fit on a CU");
+    const int gpc = ctx->groups_per_cu ? std::min(ctx->groups_per_cu, occ_pers) : occ_pers;
+    *blocks = (int)std::min<int64_t>((int64_t)ctx->num_cu * gpc, (npairs + ctx->wpg - 1) / ctx->wpg);
+    return CWHIP_OK;
 }
 
 int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* kernel_ms)
@@ -275,22 +364,22 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     if (ctx->npairs == 0) { if (ms) *ms = 0; if (kernel_ms) *kernel_ms = 0; return CWHIP_OK; }
     DevArgs a;
     fill_args(ctx, a);
-    const int threads = 64 * ctx->wpg;
-    const size_t lds = CW_TAB_BYTES + (size_t)ctx->wpg * a.wave_lds;
-    if (lds > 160 * 1024) return fail(ctx, CWHIP_E_TOOLONG, "per-workgroup LDS exceeds 160 KiB");
-    int occ_plain = 0, occ_pers = 0;
-    HIPCHK(cw_occupancy(threads, lds, &occ_plain, &occ_pers));
-    if (occ_pers < 1 || occ_plain < 1) return fail(ctx, CWHIP_E_TOOLONG, "kernel does not fit on a CU");
-    const int gpc = ctx->groups_per_cu ? std::min(ctx->groups_per_cu, occ_pers) : occ_pers;
+    int threads = 64 * ctx->wpg, pers_blocks = 0;
+    size_t lds = CW_TAB_BYTES + (size_t)ctx->wpg * a.wave_lds;
     int64_t slots;
-    int pers_blocks = 0;
     if (kernel == CWHIP_KERNEL_BATCHED) {
-        // never more waves than pairs: a small job leaves the rest of the chip
-        // (and its scratch) to whatever else runs beside it
-        pers_blocks = (int)std::min<int64_t>((int64_t)ctx->num_cu * gpc,
-                                             (ctx->npairs + ctx->wpg - 1) / ctx->wpg);
+        for (int c = 0; c < CW_NCLASS; ++c) {
+            a.rmax_cls[c] = ctx->tune.rmax[c];
+            a.colcap_cls[c] = ctx->tune.colcap[c];
+        }
+        const int rc = batched_shape(ctx, a, ctx->npairs, &pers_blocks, &threads, &lds);
+        if (rc) return rc;
         slots = (int64_t)pers_blocks * ctx->wpg;
     } else {
+        if (lds > 160 * 1024) return fail(ctx, CWHIP_E_TOOLONG, "per-workgroup LDS exceeds 160 KiB");
+        int occ_plain = 0, occ_pers = 0;
+        HIPCHK(cw_occupancy(threads, lds, &occ_plain, &occ_pers));
+        if (occ_plain < 1) return fail(ctx, CWHIP_E_TOOLONG, "kernel does not fit on a CU");
         slots = std::min<int64_t>(kPlainChunk, ctx->npairs);
         slots = (slots + ctx->wpg - 1) / ctx->wpg * ctx->wpg;
     }
@@ -298,13 +387,15 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     a.scratch = ctx->d_scratch.p;
     for (int rep = 0; rep < reps; ++rep) {
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(cw_launch_reset(ctx->stream, ctx->d_queue.p, ctx->d_status.p));
-        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         if (kernel == CWHIP_KERNEL_BATCHED) {
-            a.order = ctx->d_order.p;
+            // queue reset, binning and the persistent kernel form one graph
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
             a.pair_base = 0;
-            HIPCHK(cw_launch_persistent(pers_blocks, threads, lds, ctx->stream, a));
+            const int rc = launch_batched(ctx, a, pers_blocks, threads, lds);
+            if (rc) return rc;
         } else {
+            HIPCHK(cw_launch_reset(ctx->stream, ctx->d_queue.p, ctx->d_status.p));
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
             a.order = nullptr;
             for (int64_t b0 = 0; b0 < ctx->npairs; b0 += slots) {
                 a.pair_base = b0;
@@ -324,6 +415,126 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     int32_t st = 0;
     HIPCHK(hipMemcpy(&st, ctx->d_status.p, sizeof st, hipMemcpyDeviceToHost));
     if (st) return fail(ctx, CWHIP_E_KERNEL, "kernel status " + std::to_string(st));
+    return CWHIP_OK;
+}
+
+int cwhip_get_tune_table(cwhip_ctx* ctx, cwhip_tune_table* t)
+{
+    if (!ctx || !t) return fail(ctx, CWHIP_E_ARG, "null argument");
+    *t = ctx->tune;
+    return CWHIP_OK;
+}
+
+int cwhip_set_tune_table(cwhip_ctx* ctx, const cwhip_tune_table* t)
+{
+    if (!ctx || !t) return fail(ctx, CWHIP_E_ARG, "null argument");
+    if (!table_ok(*t)) return fail(ctx, CWHIP_E_ARG, "tuning table value out of range");
+    ctx->tune = *t;
+    return CWHIP_OK;
+}
+
+int cwhip_set_graph(cwhip_ctx* ctx, int enable)
+{
+    if (!ctx) return CWHIP_E_ARG;
+    ctx->use_graph = enable != 0;
+    return CWHIP_OK;
+}
+
+// Candidate shapes of one class, without the ones that cannot differ for
+// sequences no longer than lmax (a single stripe; every node on the column path).
+static std::vector<std::pair<int, int>> tune_candidates(int lmax)
+{
+    static const int R[] = {2, 3, 4, 6, 8};
+    static const int CC[] = {64, 128, 256};
+    const int rcap = std::max(1, (lmax + 63) / 64);
+    std::vector<std::pair<int, int>> out;
+    for (int r : R) {
+        const int re = std::min(r, rcap);
+        for (int cc : CC) {
+            const int ce = std::min(cc, std::max(32, lmax <= 32 ? 32 : lmax <= 64 ? 64 : lmax <= 128 ? 128 : 256));
+            if (std::find(out.begin(), out.end(), std::make_pair(re, ce)) == out.end()) out.emplace_back(re, ce);
+        }
+    }
+    return out;
+}
+
+int cwhip_autotune(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t* pairs, int64_t npairs,
+                   cwhip_tune_table* out, float* tune_ms)
+{
+    if (!ctx || !p || (npairs > 0 && !pairs) || npairs < 0) return fail(ctx, CWHIP_E_ARG, "bad autotune arguments");
+    if (ctx->nseq <= 0) return fail(ctx, CWHIP_E_NOSEQ, "no sequences loaded");
+    for (int64_t k = 0; k < 2 * npairs; ++k)
+        if (pairs[k] < 0 || pairs[k] >= ctx->nseq) return fail(ctx, CWHIP_E_ARG, "pair index out of range");
+    HIPCHK(hipSetDevice(ctx->device));
+    const std::vector<int32_t> restage = ctx->h_pairs;
+    const int64_t restage_n = ctx->npairs;
+    // members of every class, and the longest sequence in each
+    std::vector<std::vector<int64_t>> members(CW_NCLASS);
+    std::vector<int> lmax(CW_NCLASS, 0);
+    for (int64_t k = 0; k < npairs; ++k) {
+        const int n = ctx->h_lens[pairs[2 * k]], m = ctx->h_lens[pairs[2 * k + 1]];
+        if (n == 0 || m == 0) continue;
+        const int c = cw_class_of(n, m);
+        members[c].push_back(k);
+        lmax[c] = std::max(lmax[c], std::max(n, m));
+    }
+    cwhip_tune_table best = ctx->tune;
+    const cwhip_tune_table saved = ctx->tune;
+    float spent = 0.f;
+    DevArgs a0;
+    fill_args(ctx, a0);
+    int blocks = 0, threads = 0;
+    size_t lds = 0;
+    int rc = batched_shape(ctx, a0, INT64_MAX / 2, &blocks, &threads, &lds);
+    if (rc) return rc;
+    const int64_t waves = (int64_t)blocks * ctx->wpg;
+    for (int c = 0; c < CW_NCLASS; ++c) {
+        const auto& mem = members[c];
+        if (mem.empty()) continue;
+        // a strided sample that fills every wave twice
+        const int64_t want = std::min<int64_t>((int64_t)mem.size(), 2 * waves);
+        std::vector<int32_t> sp(2 * want);
+        for (int64_t i = 0; i < want; ++i) {
+            const int64_t k = mem[(size_t)(i * (int64_t)mem.size() / want)];
+            sp[2 * i] = pairs[2 * k];
+            sp[2 * i + 1] = pairs[2 * k + 1];
+        }
+        rc = cwhip_stage_pairs(ctx, p, sp.data(), want);
+        if (rc) { ctx->tune = saved; return rc; }
+        float best_ms = 1e30f;
+        for (const auto& cand : tune_candidates(lmax[c])) {
+            for (int k = 0; k < CW_NCLASS; ++k) {
+                ctx->tune.rmax[k] = cand.first;
+                ctx->tune.colcap[k] = cand.second;
+            }
+            float t = 0.f, tmin = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                rc = cwhip_run_staged(ctx, CWHIP_KERNEL_BATCHED, 1, &t, nullptr);
+                if (rc) { ctx->tune = saved; return rc; }
+                spent += t;
+                tmin = std::min(tmin, t);
+            }
+            if (tmin < best_ms) {
+                best_ms = tmin;
+                best.rmax[c] = cand.first;
+                best.colcap[c] = cand.second;
+            }
+        }
+    }
+    ctx->tune = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[ctx->device] = best;
+    }
+    if (out) *out = best;
+    if (tune_ms) *tune_ms = spent;
+    if (restage_n > 0) {
+        rc = cwhip_stage_pairs(ctx, p, restage.data(), restage_n);
+        if (rc) return rc;
+    } else {
+        ctx->npairs = 0;
+        ctx->h_pairs.clear();
+    }
     return CWHIP_OK;
 }
 

@@ -47,6 +47,12 @@ struct cwhip_ctx {
     int gap1 = 30, gap2 = 31, ge = 1;
     // tuning
     int rmax = CW_RMAX, wpg = CW_MAX_THREADS / 64, groups_per_cu = 0;
+    cwhip_tune_table tune;                        // batched kernel shapes by class
+    // batched stages captured as a graph; gkey holds the launch it was captured for
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gkey;
+    DevBuf<int32_t> d_bins;
     // profile alignment
     DevBuf<int32_t> p_pool, p_out, p_tiles;
     DevBuf<PrfProb> p_probs;

@@ -6,6 +6,6 @@ workloads and reads FASTA.
 """
 from . import seqgen  # noqa: F401
 from .lib import (KERNEL_BATCHED, KERNEL_PLAIN, Context, CwhipError,  # noqa: F401
-                  MsaParams, clustal_text, distance, guide_from_tree, load,
+                  MsaParams, TuneTable, clustal_text, distance, guide_from_tree, load,
                   JOB_ALIGNMENT, JOB_DISTANCES, Server, msa_params, nj_tree,
                   pair_params)

@@ -20,7 +20,8 @@ EXPORTS = [
     "cwhip_create", "cwhip_destroy", "cwhip_last_error", "cwhip_pair_params_default",
     "cwhip_load_sequences", "cwhip_range_npairs", "cwhip_pairwise_range",
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
-    "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning",
+    "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning", "cwhip_get_tune_table",
+    "cwhip_set_tune_table", "cwhip_autotune", "cwhip_set_graph",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
     "cwhip_msa_stats", "cwhip_profile_align", "cwhip_nj_tree", "cwhip_server_create",
     "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
@@ -50,6 +51,17 @@ class PairParams(C.Structure):
     _fields_ = [("matrix", C.c_int32 * 1024), ("gap_extend", C.c_int32),
                 ("gap_pos1", C.c_int32), ("gap_pos2", C.c_int32), ("max_len", C.c_int32),
                 ("gap_open_by_len", C.POINTER(C.c_int32))]
+
+
+TUNE_CLASSES = 6
+TUNE_CLASS_EDGES = (128, 256, 512, 1024, 2048)   # tuning class = first edge >= the longer sequence
+
+
+class TuneTable(C.Structure):
+    _fields_ = [("rmax", C.c_int32 * TUNE_CLASSES), ("colcap", C.c_int32 * TUNE_CLASSES)]
+
+    def as_dict(self):
+        return {"rmax": list(self.rmax), "colcap": list(self.colcap)}
 
 
 class CwhipError(RuntimeError):
@@ -90,6 +102,11 @@ def load() -> C.CDLL:
     L.cwhip_fetch_counts.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_fetch_info.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.cwhip_get_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
+    L.cwhip_set_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
+    L.cwhip_autotune.argtypes = [vp, C.POINTER(PairParams), P(np.int32, flags="C"), C.c_int64,
+                                 C.POINTER(TuneTable), C.POINTER(C.c_float)]
+    L.cwhip_set_graph.argtypes = [vp, C.c_int]
     L.cwhip_profile_dp.argtypes = [vp, vp, C.c_int, C.c_int]
     L.cwhip_msa_params_default.argtypes = [C.c_int, C.POINTER(MsaParams)]
     L.cwhip_msa.argtypes = [vp, C.POINTER(MsaParams), C.c_int, P(np.uint8, flags="C"),
@@ -185,6 +202,30 @@ class Context:
     def set_tuning(self, rmax=0, waves_per_group=0, groups_per_cu=0):
         self._chk(self._L.cwhip_set_tuning(self._h, rmax, waves_per_group, groups_per_cu),
                   "cwhip_set_tuning")
+
+    def tune_table(self):
+        t = TuneTable()
+        self._chk(self._L.cwhip_get_tune_table(self._h, C.byref(t)), "cwhip_get_tune_table")
+        return t
+
+    def set_tune_table(self, rmax, colcap):
+        t = TuneTable()
+        for c in range(TUNE_CLASSES):
+            t.rmax[c] = rmax[c] if hasattr(rmax, "__len__") else rmax
+            t.colcap[c] = colcap[c] if hasattr(colcap, "__len__") else colcap
+        self._chk(self._L.cwhip_set_tune_table(self._h, C.byref(t)), "cwhip_set_tune_table")
+
+    def autotune(self, params, pairs):
+        """Time the batched kernel's candidate shapes per tuning class on a
+        sample of `pairs`; returns (TuneTable, device ms spent)."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        t, ms = TuneTable(), C.c_float()
+        self._chk(self._L.cwhip_autotune(self._h, C.byref(params), pairs, len(pairs), C.byref(t),
+                                         C.byref(ms)), "cwhip_autotune")
+        return t, ms.value
+
+    def set_graph(self, enable):
+        self._chk(self._L.cwhip_set_graph(self._h, 1 if enable else 0), "cwhip_set_graph")
 
     def pairwise_pairs(self, params, pairs, kernel=KERNEL_BATCHED):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)

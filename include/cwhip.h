@@ -48,7 +48,8 @@ enum {
 /* kernel choice for cwhip_pairwise*; both give identical results */
 enum {
     CWHIP_KERNEL_PLAIN = 0,   /* one wavefront per pair, launched over pair blocks */
-    CWHIP_KERNEL_BATCHED = 1  /* persistent kernel over a cost-sorted device work queue */
+    CWHIP_KERNEL_BATCHED = 1  /* persistent kernel over a length-binned device work queue,
+                                 shapes by tuning class, stages replayed as a hipGraph */
 };
 
 /* Scoring of one pairwise run.  matrix is NUMRES x NUMRES, row-major, indexed
@@ -130,6 +131,31 @@ int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
 /* Per-wave tuning knobs (0 = keep the built-in default): rows per lane limit,
  * waves per workgroup, persistent workgroups per CU. */
 int cwhip_set_tuning(cwhip_ctx *ctx, int rmax, int waves_per_group, int groups_per_cu);
+
+/* Shapes of the batched kernel by tuning class (the longer sequence of a pair:
+ * <=128, <=256, <=512, <=1024, <=2048, longer): stripe height in rows per lane
+ * (a stripe is 64*rmax rows) and the widest Myers-Miller node that runs on the
+ * column path (lanes own 1, 2, 4 or 8 columns; 32..256).  The plain kernel
+ * always uses the built-in defaults.  No counterpart in the reference (its
+ * pairwise loop has no tunable shape); results do not depend on the table. */
+#define CWHIP_TUNE_CLASSES 6
+typedef struct {
+    int32_t rmax[CWHIP_TUNE_CLASSES];
+    int32_t colcap[CWHIP_TUNE_CLASSES];
+} cwhip_tune_table;
+int cwhip_get_tune_table(cwhip_ctx *ctx, cwhip_tune_table *t);
+int cwhip_set_tune_table(cwhip_ctx *ctx, const cwhip_tune_table *t);
+/* Autotuner: for every class present in `pairs` (the caller's loaded
+ * sequences), times the batched kernel on a strided sample of that class's
+ * pairs under each candidate shape and keeps the fastest.  The table goes to
+ * *out (may be NULL), becomes this context's table and the default of contexts
+ * created later on the same device.  *tune_ms (may be NULL) receives the device
+ * time spent.  A pair list staged before the call is staged again afterwards. */
+int cwhip_autotune(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pairs,
+                   int64_t npairs, cwhip_tune_table *out, float *tune_ms);
+/* Batched runs replay the captured stages (queue reset, binning, persistent
+ * kernel) as one hipGraph; 0 launches them one by one instead. */
+int cwhip_set_graph(cwhip_ctx *ctx, int enable);
 
 
 /* ------------------------------------------------------------------------ */
