@@ -10,7 +10,7 @@
 #define CWP_RMAX 8            // rows per lane the sweep is instantiated for
 #define CWP_NODE_INTS 8       // node record: A B M N flags type midj child
 #define CWP_SCR_ROWS 8        // per-wave scratch rows: 4 ping-pong + 4 final
-#define CWP_STAGE_INTS 8192   // per-wave LDS: a small node's column records and score block
+#define CWP_LOCAL_INTS 8192   // per-wave LDS for a subtree solved locally (records, profile rows, scores)
 
 // Per profile pair; offsets are in int32 units into PrfArgs::pool.
 struct PrfProb {

@@ -69,7 +69,7 @@ __device__ __forceinline__ void wave_fence() {
 }
 __device__ __forceinline__ int max3(int a, int b, int c) { return max(max(a, b), c); }
 
-// LDS address space: staged reads compile to ds_read_* rather than flat loads
+// LDS address space: reads through these compile to ds_read_* rather than flat loads
 #define CWP_LDS __attribute__((address_space(3)))
 typedef CWP_LDS int lint;
 
@@ -114,8 +114,6 @@ struct PSweep {
     //   reverse: (go2 ? -open2(I(i), jedge) : 0) - (m2[jedge] & (pl1[ib] - pl1[I(i)]))
     int lt0, ltl;
     int go2, jedge;
-    lint* stage;       // this wave's LDS buffer (CWP_STAGE_INTS ints)
-    int jrec0;         // column of staged record 0 (B): staged record x is column jrec0 + x
 };
 
 template <int MODE>
@@ -134,7 +132,7 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 // R-1 rows behind a pass-through row).  The top boundary row comes from inH
 // (inE, or -inf on the first stripe); lane 63's rows leave through outH/outE.
 // DIRECT: R == 1, P < 64, lane P-1 writes its row straight to out.
-template <int R, int MODE, bool DIRECT, bool STAGED = false>
+template <int R, int MODE, bool DIRECT>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
                                         const int* inH, const int* inE, int* outH, int* outE)
 {
@@ -193,24 +191,11 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
     int nsv[R];
     auto prefetch = [&](int c) {
         const int J = colJ(c);
-        if constexpr (STAGED) {
-            // single stripe, one row per lane: records at x = J - jrec0, scores
-            // column-major over (local column, local row)
-            const lint* lrec = W.stage;
-            const int x = J - W.jrec0;
-            nrecF = make_int4(lrec[4 * x], lrec[4 * x + 1], lrec[4 * x + 2], lrec[4 * x + 3]);
-            if (MODE == MODE_FWD) nrecE = nrecF;
-            else nrecE = make_int4(lrec[4 * x - 4], lrec[4 * x - 3], lrec[4 * x - 2], lrec[4 * x - 1]);
-            const int cc = c < 1 ? 1 : (c > ncols ? ncols : c);
-            const int rr = firstrow < 1 ? 1 : (firstrow > W.nrows ? W.nrows : firstrow);
-            nsv[0] = W.stage[4 * (ncols + 2) + (cc - 1) * W.nrows + (rr - 1)];
-        } else {
-            nrecF = p.col[J];
-            nrecE = (MODE == MODE_FWD) ? nrecF : p.col[J - 1];
-            const int* Sc = p.S + (int64_t)J * ldS;
+        nrecF = p.col[J];
+        nrecE = (MODE == MODE_FWD) ? nrecF : p.col[J - 1];
+        const int* Sc = p.S + (int64_t)J * ldS;
 #pragma unroll
-            for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
-        }
+        for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
     };
     prefetch(1 - lane);
 
@@ -310,7 +295,6 @@ __device__ void run_pstripe(int R, bool direct, const PSweep& W, int row0, int P
                             const int* inH, const int* inE, int* outH, int* outE)
 {
     if (direct) { pstripe<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE); return; }
-    if (R == 0) { pstripe<1, MODE, true, true>(W, row0, P, first, inH, inE, outH, outE); return; }
     switch (R) {
     case 1: pstripe<1, MODE, false>(W, row0, P, first, inH, inE, outH, outE); break;
     case 2: pstripe_call<2, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
@@ -327,49 +311,10 @@ struct Bufs { int *h0, *e0, *h1, *e1; };
 
 // Whole half sweep in balanced stripes; the top boundary row must already be
 // in g.h0[1..ncols].  The final row goes to finH/finE[1..ncols].
-// Loads a single-stripe node's column records (columns jrec0 .. jrec0+N+1) and
-// its nrows x N score block into this wave's LDS buffer.
-template <int MODE>
-__device__ void stage_node(const PSweep& W)
-{
-    const int lane = lane_id();
-    const Prob& p = W.p;
-    const int N = W.ncols, P = W.nrows;
-    lint* st = W.stage;
-    for (int x = lane; x < N + 2; x += WAVE) {
-        const int4 r = p.col[W.jrec0 + x];
-        st[4 * x + 0] = r.x;
-        st[4 * x + 1] = r.y;
-        st[4 * x + 2] = r.z;
-        st[4 * x + 3] = r.w;
-    }
-    lint* lS = st + 4 * (N + 2);
-    const int per = WAVE / P;                 // columns per pass (P < 64)
-    const int cc = lane / P, rr = lane - cc * P;
-    if (cc < per) {
-        const int I = W.ib + W.id * (rr + 1);
-        for (int c0 = 0; c0 < N; c0 += per) {
-            const int c = c0 + cc;
-            if (c < N) {
-                const int J = W.jb + W.jd * (c + 1);
-                lS[c * P + rr] = p.S[(int64_t)J * p.ldS + I];
-            }
-        }
-    }
-    wave_fence();
-}
-
 template <int MODE>
 __device__ void psweep(const PSweep& W, int rmax, const Bufs& g, int* finH, int* finE)
 {
     const int nrows = W.nrows;
-    if (nrows < WAVE && nrows * W.ncols + 4 * (W.ncols + 2) <= CWP_STAGE_INTS) {
-        // one stripe, one row per lane, from LDS
-        stage_node<MODE>(W);
-        run_pstripe<MODE>(0, false, W, 0, nrows, 1, g.h0, g.e0, finH, finE);
-        wave_fence();
-        return;
-    }
     const int per = WAVE * rmax;
     const int nst = uni((nrows + per - 1) / per);
     const int base = nrows / nst, rem = nrows % nst;
@@ -414,7 +359,7 @@ __device__ void fill_top(const Prob& p, int A, int B, int M, int N, int* h0)
 // shuffle then reads an inactive lane) while others wait at its exit.  Kept
 // out of line for the same reason.
 __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is_root, int rmax, int* scr, int rowcap,
-                        int* out_score, lint* stage)
+                        int* out_score)
 {
     const int lane = lane_id();
     const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
@@ -423,9 +368,9 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     if (N <= 0 || M <= 1) {                                        // :540-600
         int midj = 0, midh;
         if (N <= 0) {
-            midh = -gap_pen1(p, A, B, M);
+            midh = is_root ? -gap_pen1(p, A, B, M) : 0;      // only the root's score is kept
         } else if (M <= 0) {
-            midh = -gap_pen2(p, A, B, N);
+            midh = is_root ? -gap_pen2(p, A, B, N) : 0;
         } else {
             const int midh0 = go1 == 0 ? -gap_pen1(p, A + 1, B + 1, N)
                                        : -gap_pen2(p, A + 1, B, 1) - gap_pen1(p, A + 1, B + 1, N);
@@ -460,7 +405,6 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
     F.ltl = -ext2(p, A + 1, B);
     F.go2 = 0; F.jedge = 0;
-    F.stage = stage; F.jrec0 = B;
     fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
     psweep<MODE_FWD>(F, rmax, g, fH, fE);
 
@@ -470,7 +414,6 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
     Rv.lt0 = 0; Rv.ltl = 0;
     Rv.go2 = go2; Rv.jedge = B + N;
-    Rv.stage = stage; Rv.jrec0 = B;
     fill_top<MODE_REV>(p, A, B, M, N, g.h0);
     psweep<MODE_REV>(Rv, rmax, g, rH, rK);
     wave_fence();
@@ -503,6 +446,591 @@ __device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is
     nd[5] = type;
     nd[6] = midj;
     if (is_root) *out_score = midh;
+}
+
+// The two children of a split node (:796-809): type 1 splits at (midi, midj),
+// type 2 through the gap of rows midi..midi+1 (the del(2) between them is the
+// host's).  Records: A B M N flags type midj child.
+__device__ __forceinline__ void make_children(int* c0, int* c1, int A, int B, int M, int N, int fl,
+                                              int type, int midj)
+{
+    const int go1 = fl & 1, go2 = (fl >> 1) & 1;
+    const int midi = M / 2;
+    if (type == 1) {
+        c0[0] = A; c0[1] = B; c0[2] = midi; c0[3] = midj; c0[4] = go1 | 2;
+        c1[0] = A + midi; c1[1] = B + midj; c1[2] = M - midi; c1[3] = N - midj;
+        c1[4] = 1 | (go2 << 1);
+    } else {
+        c0[0] = A; c0[1] = B; c0[2] = midi - 1; c0[3] = midj; c0[4] = go1;
+        c1[0] = A + midi + 1; c1[1] = B + midj; c1[2] = M - midi - 1; c1[3] = N - midj;
+        c1[4] = go2 << 1;
+    }
+    c0[5] = c1[5] = 0;
+    c0[6] = c1[6] = 0;
+    c0[7] = c1[7] = -1;
+}
+
+// ---- subtrees solved in LDS ---------------------------------------------------
+// A node with at most 2*64 rows and 256 columns whose data fits the wave's
+// CWP_LOCAL_INTS is solved with all its descendants by one wave, depth first,
+// without returning to the level loop.  Its profile rows, column records,
+// prefix sums and score block are copied to LDS once; every node below it
+// reads only rows A0..A0+M0+1 and columns B0..B0+N0+1 (the leaves' N<=0 / M<=0
+// scores, which read further, are needed for the root only).  Records of the
+// subtree stay in LDS and go to the node table in one pass at the end; child
+// slots come from the workgroup's slot counter.
+//
+// A local node runs both half sweeps at once with lanes owning columns: lanes
+// 0..31 the forward half (rows 1..midi from the corner (A,B)), lanes 32..63
+// the reverse half (rows 1..M-midi from (A+M+1,B+N+1)), C = 1, 2, 4 or 8
+// columns per lane.  At step t lane L works on row t-L of its half; the H and
+// F (forward) or G (reverse) of its last column move to lane L+1 by DPP.  The
+// top rows and left columns are the reference's boundary values (:636-693),
+// computed in registers.
+
+constexpr int kLocalRec = 9;       // A B M N flags type midj child slot
+constexpr int kLocalStack = 64;    // depth-first stack (record indices)
+constexpr int kLocalMaxRows = 2 * WAVE;
+constexpr int kLocalMaxCols = 32 * 8;
+constexpr int kLocalFlag = 4;      // flags bit: subtree solved locally (level loop skips it)
+
+// A profile pair's data for rows A0..A0+M0+1 and columns B0..B0+N0+1 in LDS;
+// accessors take global indices.
+struct LProb {
+    int L1, L2, A0, B0, M0;
+    lint* rw;          // row records, 4 ints each
+    lint* cl;          // column records, 4 ints each
+    lint* p1;          // pl1
+    lint* p2;          // pl2
+    lint* sc;          // scores, sc[(J-B0-1)*M0 + I-A0-1]
+    __device__ int4 row(int I) const {
+        const lint* r = rw + 4 * (I - A0);
+        return make_int4(r[0], r[1], r[2], r[3]);
+    }
+    __device__ int4 col(int J) const {
+        const lint* c = cl + 4 * (J - B0);
+        return make_int4(c[0], c[1], c[2], c[3]);
+    }
+    __device__ int pl1(int x) const { return p1[x - A0]; }
+    __device__ int pl2(int x) const { return p2[x - B0]; }
+    __device__ int S(int I, int J) const { return sc[(J - B0 - 1) * M0 + (I - A0 - 1)]; }
+};
+
+// gapPenalty1/2 (:1166-1189, :1239-1262) on the LDS copy
+__device__ __forceinline__ int lgap_pen1(const LProb& q, int i, int j, int k) {
+    if (k <= 0) return 0;
+    const int4 r = q.row(i);
+    if (r.z == 0) return 0;
+    const int e = j + k < q.L2 ? j + k : q.L2;
+    return r.x + q.col(j).x + (j < q.L2 ? q.pl2(e) - q.pl2(j) : 0);
+}
+__device__ __forceinline__ int lgap_pen2(const LProb& q, int i, int j, int k) {
+    if (k <= 0) return 0;
+    const int4 c = q.col(j);
+    if (c.z == 0) return 0;
+    const int e = i + k < q.L1 ? i + k : q.L1;
+    return q.row(i).x + c.x + (i < q.L1 ? q.pl1(e) - q.pl1(i) : 0);
+}
+
+__device__ __forceinline__ int local_ints(int M0, int N0)
+{
+    return 4 * (M0 + 2) + 4 * (N0 + 2) + (M0 + 2) + (N0 + 2) + M0 * N0 + 4 * (N0 + 2) +
+           (kLocalRec + 1) * (2 * M0 + 4) + kLocalStack + 2;
+}
+
+__device__ __forceinline__ bool local_fits(int M, int N)
+{
+    return M >= 2 && M <= kLocalMaxRows && N >= 1 && N <= kLocalMaxCols &&
+           local_ints(M, N) <= CWP_LOCAL_INTS;
+}
+
+// The final rows of both halves of node (A,B,M,N): fH/fE[j] = HH/DD of row
+// midi (j = 1..N), rH/rK[c] = RR and K of reverse row M-midi at reversed
+// column c (j = N+1-c).
+template <int C>
+__device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int go2, lint* fH,
+                        lint* fE, lint* rH, lint* rK)
+{
+    const int lane = lane_id();
+    const int grp = lane >> 5;                 // 0: forward half, 1: reverse half
+    const int L = lane & 31;
+    const int midi = M / 2;
+    const int rows = grp ? (M - midi) : midi;
+    const int nl = (N + C - 1) / C;            // lanes in use per half
+    // forward top row H(0,j) = t0 + j*tl; left column H(i,0) = lt0 + i*ltl
+    const int4 ra = q.row(A), rb1 = q.row(A + 1);
+    const int4 cb = q.col(B), cb1 = q.col(B + 1);
+    const int t0 = -(ra.z & (ra.x + cb1.x)), tl = -(ra.z & cb1.y);
+    const int lt0 = go1 ? -(cb.z & (rb1.x + cb.x)) : 0, ltl = -(cb.z & rb1.y);
+    // reverse top row: -(m1 & (g1 + g2[J])) - (m1 & (pl2[B+N+1] - pl2[J])) at row A+M;
+    // left column: (go2 ? -open2(I, B+N) : 0) - (m2[B+N] & (pl1[A+M+1] - pl1[I]))
+    const int4 rm = q.row(A + M);
+    const int4 ce = q.col(B + N);
+    const int p2e = q.pl2(B + N + 1), p1e = q.pl1(A + M + 1);
+    int4 cF[C], cE[C];
+    int hup[C], vst[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;
+        const int jc = jj <= N ? jj : N;
+        const int J = grp ? (B + N + 1 - jc) : (B + jc);
+        cF[c] = q.col(J);
+        cE[c] = grp ? q.col(J - 1) : cF[c];
+        hup[c] = grp ? (-(rm.z & (rm.x + cF[c].x)) - (rm.z & (p2e - q.pl2(J)))) : (t0 + jj * tl);
+        vst[c] = NEG;
+    }
+    // left boundary H of half-row i (lane L == 0 only)
+    auto left = [&](int i) -> int {
+        if (i <= 0) return 0;
+        if (!grp) return lt0 + i * ltl;
+        const int I = A + M + 1 - i;
+        const int g = go2 ? -(ce.z & (q.row(I).x + ce.x)) : 0;
+        return g - (ce.z & (p1e - q.pl1(I)));
+    };
+    int dleft;
+    if (L == 0) {
+        dleft = 0;
+    } else {
+        const int jl = L * C;                  // H(0, jl) of this lane's left neighbour column
+        const int Jl = grp ? (B + N + 1 - (jl <= N ? jl : N)) : 0;
+        dleft = grp ? (-(rm.z & (rm.x + q.col(Jl).x)) - (rm.z & (p2e - q.pl2(Jl)))) : (t0 + jl * tl);
+    }
+    // row data of the lane's row, one step ahead
+    auto rowI = [&](int i) {
+        const int ic = i < 1 ? 1 : (i > rows ? rows : i);
+        return grp ? (A + M + 1 - ic) : (A + ic);
+    };
+    int4 nrf, nre;
+    int ns[C];
+    auto fetch = [&](int i) {
+        const int I = rowI(i);
+        nre = q.row(I);
+        nrf = grp ? q.row(I - 1) : nre;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int jj = L * C + c + 1;
+            const int jc = jj <= N ? jj : N;
+            const int J = grp ? (B + N + 1 - jc) : (B + jc);
+            ns[c] = q.S(I, J);
+        }
+    };
+    fetch(1 - L);
+    int hl_out = 0, fl_out = NEG;
+    const int T = (midi > M - midi ? midi : M - midi) + nl - 1;
+    for (int t = 1; t <= T; ++t) {
+        int hl = shr1(0, hl_out);
+        int fl = shr1(NEG, fl_out);
+        const int i = t - L;
+        const int4 rf = nrf, re = nre;
+        int s[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) s[c] = ns[c];
+        fetch(i + 1);
+        if (L == 0) {
+            hl = left(i);
+            fl = NEG;
+            dleft = left(i - 1);
+        }
+        if (i >= 1 && i <= rows && L < nl) {
+            int d = dleft, hc = hl, fc = fl;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                int h;
+                if (!grp) {
+                    const int fn = max(hc - ((rf.x + cF[c].w) & rf.z), fc - (cF[c].y & rf.z));
+                    const int en = max(hup[c] - ((re.w + cE[c].x) & cE[c].z), vst[c] - (re.y & cE[c].z));
+                    h = max3(d + s[c], fn, en);
+                    vst[c] = en;
+                    fc = fn;
+                } else {
+                    const int g = max(hc, fc) - (cF[c].y & rf.z);
+                    const int k = max(hup[c], vst[c]) - (re.y & cE[c].z);
+                    h = max3(d + s[c], g - ((rf.x + cF[c].x) & rf.z), k - ((re.x + cE[c].x) & cE[c].z));
+                    vst[c] = k;
+                    fc = g;
+                }
+                d = hup[c];
+                hup[c] = h;
+                hc = h;
+            }
+            dleft = hl;
+            hl_out = hc;
+            fl_out = fc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;
+        if (jj <= N) {
+            if (grp == 0) { fH[jj] = hup[c]; fE[jj] = vst[c]; }
+            else { rH[jj] = hup[c]; rK[jj] = vst[c]; }
+        }
+    }
+}
+
+// One node of a local subtree (progDiff on LDS data): leaves as in do_node,
+// internal nodes by lhalves and the midpoint of :756-776.  Writes type and
+// midj to the record; *midh_out gets the score (used for the root only).
+__device__ __attribute__((noinline)) void local_node(const LProb& q, int* rd, int is_root, lint* rows4,
+                                                     int rstride, int* out_score)
+{
+    const int lane = lane_id();
+    const int A = uni(rd[0]), B = uni(rd[1]), M = uni(rd[2]), N = uni(rd[3]);
+    const int flags = uni(rd[4]);
+    const int go1 = flags & 1, go2 = (flags >> 1) & 1;
+    if (N <= 0 || M <= 1) {                                        // :540-600
+        int midj = 0, midh = 0;
+        if (N <= 0 || M <= 0) {
+            // only the root's score is kept; a local root has M >= 2 and N >= 1
+        } else {
+            const int midh0 = go1 == 0 ? -lgap_pen1(q, A + 1, B + 1, N)
+                                       : -lgap_pen2(q, A + 1, B, 1) - lgap_pen1(q, A + 1, B + 1, N);
+            int bv = -BIG, bj = BIG;
+            for (int j = 1 + lane; j <= N; j += WAVE) {
+                const int v = -lgap_pen1(q, A, B + 1, j - 1) + q.S(A + 1, B + j) -
+                              lgap_pen1(q, A + 1, B + j + 1, N - j);
+                if (v > bv) { bv = v; bj = j; }
+            }
+            const int V = wave_max(bv);
+            const int J = wave_min(bv == V ? bj : BIG);
+            if (V > midh0) { midh = V; midj = J; }
+            else midh = midh0;
+        }
+        rd[5] = 0;
+        rd[6] = midj;
+        rd[7] = -1;
+        if (is_root) *out_score = midh;
+        return;
+    }
+    const int midi = M / 2;
+    lint* fH = rows4;
+    lint* fE = rows4 + rstride;
+    lint* rH = rows4 + 2 * rstride;
+    lint* rK = rows4 + 3 * rstride;
+    if (N <= 32) lhalves<1>(q, A, B, M, N, go1, go2, fH, fE, rH, rK);
+    else if (N <= 64) lhalves<2>(q, A, B, M, N, go1, go2, fH, fE, rH, rK);
+    else if (N <= 128) lhalves<4>(q, A, B, M, N, go1, go2, fH, fE, rH, rK);
+    else lhalves<8>(q, A, B, M, N, go1, go2, fH, fE, rH, rK);
+    wave_fence();
+    // HH[0] = left column of forward row midi; RR[N] = left column of reverse row M-midi
+    const int4 rb1 = q.row(A + 1);
+    const int4 cb = q.col(B), ce = q.col(B + N);
+    const int HH0 = (go1 ? -(cb.z & (rb1.x + cb.x)) : 0) + midi * (-(cb.z & rb1.y));
+    const int Ie = A + midi + 1;                 // I of reverse row M-midi
+    const int RRN = (go2 ? -(ce.z & (q.row(Ie).x + ce.x)) : 0) - (ce.z & (q.pl1(A + M + 1) - q.pl1(Ie)));
+    const int4 rmid = q.row(A + midi + 1);
+    int vmax = -BIG, jfirst = BIG, jc = -1, wmax = -BIG, jw = -1;
+    for (int j = lane; j <= N; j += WAVE) {
+        const int HHj = (j == 0) ? HH0 : fH[j];
+        const int DDj = (j == 0) ? HH0 : fE[j];
+        const int4 cj = q.col(B + j);
+        const int gS = cj.z & (rmid.x + cj.x);
+        int RRj, SSj;
+        if (j == N) { RRj = RRN; SSj = RRN; }
+        else { RRj = rH[N - j]; SSj = rK[N - j] - gS; }
+        const int v = HHj + RRj;
+        const int w = DDj + SSj + gS;
+        if (v > vmax) { vmax = v; jfirst = j; jc = -1; }
+        if (v == vmax && HHj != DDj && RRj == SSj) jc = j;
+        if (w >= wmax) { wmax = w; jw = j; }
+    }
+    const int Mv = wave_max(vmax);
+    const int J0 = wave_min(vmax == Mv ? jfirst : BIG);
+    const int JC = wave_max(vmax == Mv ? jc : -1);
+    const int Wv = wave_max(wmax);
+    const int JW = wave_max(wmax == Wv ? jw : -1);
+    int type, midj, midh;
+    if (Wv > Mv) { type = 2; midj = JW; midh = Wv; }
+    else { type = 1; midj = J0 > JC ? J0 : JC; midh = Mv; }
+    rd[5] = type;
+    rd[6] = midj;
+    if (is_root) *out_score = midh;
+}
+
+// ---- tiny nodes, one lane each -----------------------------------------------
+// Leaves and nodes of at most kTinyRows x kTinyCols are not worth a wave: the
+// wave keeps them in an LDS queue and every lane runs one of them as scalar
+// code, the columns unrolled in registers (column j of the forward half and of
+// the reverse half at the same index), the midpoint as the reference's own
+// sequential scan over j = 0..N.  A split node appends its two children to
+// the queue.
+
+constexpr int kTinyRows = 16;
+constexpr int kTinyCols = 16;
+
+__device__ __forceinline__ bool is_tiny(int M, int N)
+{
+    return M <= 1 || N <= 0 || (M <= kTinyRows && N <= kTinyCols);
+}
+
+// One node by one lane.  ctr[0]: records used, ctr[1]: queue tail.
+__device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int* lq, int* ctr,
+                                          int rec_cap, int* s_alloc, int cap, int* s_err,
+                                          int* out_score)
+{
+    int* rd = recs + kLocalRec * ri;
+    const int A = rd[0], B = rd[1], M = rd[2], N = rd[3], fl = rd[4];
+    const int go1 = fl & 1, go2 = (fl >> 1) & 1;
+    const bool root = rd[8] == 0;
+    if (N <= 0 || M <= 1) {                                        // :540-600
+        int midj = 0, midh = 0;
+        if (N > 0 && M > 0) {
+            const int midh0 = go1 == 0 ? -lgap_pen1(q, A + 1, B + 1, N)
+                                       : -lgap_pen2(q, A + 1, B, 1) - lgap_pen1(q, A + 1, B + 1, N);
+            int bv = -BIG, bj = 0;
+            for (int j = 1; j <= N; ++j) {
+                const int v = -lgap_pen1(q, A, B + 1, j - 1) + q.S(A + 1, B + j) -
+                              lgap_pen1(q, A + 1, B + j + 1, N - j);
+                if (v > bv) { bv = v; bj = j; }
+            }
+            if (bv > midh0) { midh = bv; midj = bj; }
+            else midh = midh0;
+        }
+        rd[5] = 0;
+        rd[6] = midj;
+        rd[7] = -1;
+        if (root) *out_score = midh;
+    } else {
+        constexpr int NT = kTinyCols;
+        const int midi = M / 2;
+        int cx[NT + 1], cy[NT + 1], cz[NT + 1];          // g2, l2, m2 of column B+k
+#pragma unroll
+        for (int k = 0; k <= NT; ++k) {
+            const int4 c = q.col(B + (k <= N ? k : N));
+            cx[k] = c.x;
+            cy[k] = c.y;
+            cz[k] = c.z;
+        }
+        // forward half (:636-680): rows 1..midi from corner (A,B)
+        const int4 ra = q.row(A), rb1 = q.row(A + 1);
+        const int t0 = -(ra.z & (ra.x + cx[1])), tl = -(ra.z & cy[1]);
+        const int lt0 = go1 ? -(cz[0] & (rb1.x + cx[0])) : 0, ltl = -(cz[0] & rb1.y);
+        int H[NT + 1], E[NT + 1];
+        H[0] = 0;
+        E[0] = NEG;
+#pragma unroll
+        for (int k = 1; k <= NT; ++k) {
+            H[k] = t0 + k * tl;
+            E[k] = NEG;
+        }
+        for (int i = 1; i <= midi; ++i) {
+            const int4 r = q.row(A + i);
+            int d = H[0], hc = lt0 + i * ltl, f = NEG;
+            H[0] = hc;
+#pragma unroll
+            for (int k = 1; k <= NT; ++k) {
+                if (k <= N) {
+                    const int sv = q.S(A + i, B + k);
+                    const int fn = max(hc - ((r.x + cx[k] + cy[k]) & r.z), f - (cy[k] & r.z));
+                    const int en = max(H[k] - ((r.w + cx[k]) & cz[k]), E[k] - (r.y & cz[k]));
+                    const int h = max3(d + sv, fn, en);
+                    d = H[k];
+                    H[k] = h;
+                    E[k] = en;
+                    hc = h;
+                    f = fn;
+                }
+            }
+        }
+        // reverse half (:684-747): rows M..midi+1 from corner (A+M+1,B+N+1);
+        // RH/RK[k] hold column B+k
+        const int4 rm = q.row(A + M);
+        const int4 ce = q.col(B + N);
+        const int p1e = q.pl1(A + M + 1), p2e = q.pl2(B + N + 1);
+        auto rleft = [&](int r) -> int {
+            if (r <= 0) return 0;
+            const int I = A + M + 1 - r;
+            const int g = go2 ? -(ce.z & (q.row(I).x + ce.x)) : 0;
+            return g - (ce.z & (p1e - q.pl1(I)));
+        };
+        int RH[NT + 2], RK[NT + 2];
+        RH[0] = RK[0] = RH[NT + 1] = RK[NT + 1] = 0;
+#pragma unroll
+        for (int k = 1; k <= NT; ++k) {
+            const int kk = k <= N ? k : N;
+            RH[k] = -(rm.z & (rm.x + cx[k])) - (rm.z & (p2e - q.pl2(B + kk)));
+            RK[k] = NEG;
+        }
+        const int R = M - midi;
+        for (int r = 1; r <= R; ++r) {
+            const int I = A + M + 1 - r;
+            const int4 rf = q.row(I - 1), re = q.row(I);
+            int hc = rleft(r), g = NEG, d = rleft(r - 1);
+#pragma unroll
+            for (int k = NT; k >= 1; --k) {
+                if (k <= N) {
+                    const int sv = q.S(I, B + k);
+                    const int gn = max(hc, g) - (cy[k] & rf.z);
+                    const int kn = max(RH[k], RK[k]) - (re.y & cz[k - 1]);
+                    const int h = max3(d + sv, gn - ((rf.x + cx[k]) & rf.z),
+                                       kn - ((re.x + cx[k - 1]) & cz[k - 1]));
+                    d = RH[k];
+                    RH[k] = h;
+                    RK[k] = kn;
+                    hc = h;
+                    g = gn;
+                }
+            }
+        }
+        // midpoint (:756-776) in the reference's scan order
+        const int HH0 = H[0];
+        const int RRN = rleft(R);
+        const int gx = q.row(A + midi + 1).x;
+        int vmax = -BIG, jfirst = 0, jc = -1, wmax = -BIG, jw = -1;
+#pragma unroll
+        for (int j = 0; j <= NT; ++j) {
+            if (j <= N) {
+                const int HHj = (j == 0) ? HH0 : H[j];
+                const int DDj = (j == 0) ? HH0 : E[j];
+                const int gS = cz[j] & (gx + cx[j]);
+                int RRj, SSj;
+                if (j == N) { RRj = RRN; SSj = RRN; }
+                else { RRj = RH[j + 1]; SSj = RK[j + 1] - gS; }
+                const int v = HHj + RRj;
+                const int w = DDj + SSj + gS;
+                if (v > vmax) { vmax = v; jfirst = j; jc = -1; }
+                if (v == vmax && HHj != DDj && RRj == SSj) jc = j;
+                if (w >= wmax) { wmax = w; jw = j; }
+            }
+        }
+        int type, midj, midh;
+        if (wmax > vmax) { type = 2; midj = jw; midh = wmax; }
+        else { type = 1; midj = jfirst > jc ? jfirst : jc; midh = vmax; }
+        rd[5] = type;
+        rd[6] = midj;
+        rd[7] = -1;
+        if (root) *out_score = midh;
+        const int slot = atomicAdd(s_alloc, 2);
+        const int nr = atomicAdd(&ctr[0], 2);
+        const int qp = atomicAdd(&ctr[1], 2);
+        if (slot + 2 > cap || nr + 2 > rec_cap || qp + 2 > rec_cap) {
+            *s_err = 1;
+        } else {
+            int* c0 = recs + kLocalRec * nr;
+            int* c1 = c0 + kLocalRec;
+            make_children(c0, c1, A, B, M, N, fl, type, midj);
+            c0[8] = slot;
+            c1[8] = slot + 1;
+            rd[7] = slot;
+            lq[qp] = nr;
+            lq[qp + 1] = nr + 1;
+        }
+    }
+}
+
+__device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* nodes, int root, int* base,
+                                                        int* s_alloc, int cap, int* s_err, int* out_score)
+{
+    const int lane = lane_id();
+    const int* rnd = nodes + (int64_t)root * CWP_NODE_INTS;
+    const int A0 = uni(rnd[0]), B0 = uni(rnd[1]), M0 = uni(rnd[2]), N0 = uni(rnd[3]);
+    const int fl0 = uni(rnd[4]);
+    lint* lb = (lint*)base;
+    lint* lrow = lb;                                 // rows A0 .. A0+M0+1, 4 ints each
+    lint* lcol = lrow + 4 * (M0 + 2);                // columns B0 .. B0+N0+1
+    lint* lpl1 = lcol + 4 * (N0 + 2);
+    lint* lpl2 = lpl1 + (M0 + 2);
+    lint* lS = lpl2 + (N0 + 2);                      // lS[(J-B0-1)*M0 + I-A0-1]
+    lint* lout = lS + M0 * N0;                       // final rows of a node: 4 x (N0 + 2)
+    int* recs = base + (4 * (M0 + 2) + 4 * (N0 + 2) + (M0 + 2) + (N0 + 2) + M0 * N0 + 4 * (N0 + 2));
+    const int rec_cap = 2 * M0 + 4;
+    int* stk = recs + kLocalRec * rec_cap;
+    int* lq = stk + kLocalStack;                     // tiny-node queue (record indices)
+    int* ctr = lq + rec_cap;                         // records used, queue tail
+
+    // copy in, 16 loads in flight per lane before the LDS stores
+    for (int x = lane; x < M0 + 2; x += WAVE) {
+        const int4 r = p.row[A0 + x];
+        lrow[4 * x] = r.x; lrow[4 * x + 1] = r.y; lrow[4 * x + 2] = r.z; lrow[4 * x + 3] = r.w;
+        lpl1[x] = p.pl1[A0 + x];
+    }
+    for (int x = lane; x < N0 + 2; x += WAVE) {
+        const int4 c = p.col[B0 + x];
+        lcol[4 * x] = c.x; lcol[4 * x + 1] = c.y; lcol[4 * x + 2] = c.z; lcol[4 * x + 3] = c.w;
+        lpl2[x] = p.pl2[B0 + x];
+    }
+    const int tot = M0 * N0;
+    for (int k0 = 0; k0 < tot; k0 += 16 * WAVE) {
+        int v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = k0 + u * WAVE + lane;
+            const int c = k / M0, r = k - c * M0;
+            v[u] = k < tot ? p.S[(int64_t)(B0 + 1 + c) * p.ldS + A0 + 1 + r] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = k0 + u * WAVE + lane;
+            if (k < tot) lS[k] = v[u];
+        }
+    }
+    LProb q;
+    q.L1 = p.L1; q.L2 = p.L2; q.A0 = A0; q.B0 = B0; q.M0 = M0;
+    q.rw = lrow; q.cl = lcol; q.p1 = lpl1; q.p2 = lpl2; q.sc = lS;
+    lint* rows4 = lout;                              // 4 rows of stride N0 + 2
+
+    recs[0] = A0; recs[1] = B0; recs[2] = M0; recs[3] = N0; recs[4] = fl0;
+    recs[5] = 0; recs[6] = 0; recs[7] = -1; recs[8] = root;
+    stk[0] = 0;
+    wave_fence();
+    int sp = 1, nrec = 1, nq = 0;
+    while (sp > 0) {
+        sp -= 1;
+        const int ri = uni(stk[sp]);
+        int* rd = recs + kLocalRec * ri;
+        if (is_tiny(uni(rd[2]), uni(rd[3]))) {
+            lq[nq] = ri;                             // same value from every lane
+            nq += 1;
+            wave_fence();
+            continue;
+        }
+        local_node(q, rd, uni(rd[8]) == 0, rows4, N0 + 2, out_score);
+        wave_fence();
+        const int type = uni(rd[5]);
+        if (type != 0) {
+            const int got = atomicAdd(s_alloc, lane == 0 ? 2 : 0);
+            const int slot = uni(__shfl(got, 0, WAVE));
+            if (slot + 2 > cap || nrec + 2 > rec_cap || sp + 2 > kLocalStack) {
+                *s_err = 1;                          // same value from every lane
+                break;
+            }
+            int* c0 = recs + kLocalRec * nrec;
+            int* c1 = c0 + kLocalRec;
+            make_children(c0, c1, uni(rd[0]), uni(rd[1]), uni(rd[2]), uni(rd[3]), uni(rd[4]), type,
+                          uni(rd[6]));
+            c0[8] = slot;
+            c1[8] = slot + 1;
+            rd[7] = slot;
+            stk[sp] = nrec + 1;
+            stk[sp + 1] = nrec;
+            sp += 2;
+            nrec += 2;
+            wave_fence();
+        }
+    }
+    // the tiny nodes, one per lane, until the queue is empty
+    ctr[0] = nrec;
+    ctr[1] = nq;
+    wave_fence();
+    int head = 0;
+    for (;;) {
+        const int tail = uni(ctr[1]);
+        if (head >= tail) break;
+        const int take = tail - head < WAVE ? tail - head : WAVE;
+        if (lane < take)
+            tiny_node(q, recs, lq[head + lane], lq, ctr, rec_cap, s_alloc, cap, s_err, out_score);
+        head += take;
+        wave_fence();
+    }
+    nrec = uni(ctr[0]);
+    // records out to the node table; the root keeps its slot, flagged as done
+    for (int k = lane; k < nrec * CWP_NODE_INTS; k += WAVE) {
+        const int r = k >> 3, f = k & 7;
+        const int* rc = recs + kLocalRec * r;
+        const int v = (f == 4 && r == 0) ? (rc[4] | kLocalFlag) : rc[f];
+        nodes[(int64_t)rc[8] * CWP_NODE_INTS + f] = v;
+    }
+    wave_fence();
 }
 
 } // namespace cwp
@@ -571,8 +1099,8 @@ extern "C" __global__ void __launch_bounds__(256) cw_prf_score(PrfArgs a)
 // Myers-Miller recursion of one profile pair per workgroup, level by level.
 extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 {
-    __shared__ int s_lev_b, s_lev_e, s_ctr, s_err;
-    __shared__ int s_stage[CWP_WAVES * CWP_STAGE_INTS];
+    __shared__ int s_lev_b, s_lev_e, s_ctr, s_err, s_alloc;
+    __shared__ __align__(16) int s_local[CWP_WAVES * CWP_LOCAL_INTS];
     const int prob = blockIdx.x;
     const PrfProb pp = a.probs[prob];
     const Prob p = make_prob(a, pp);
@@ -587,6 +1115,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         nd[4] = pp.flags & 3; nd[5] = 0; nd[6] = 0; nd[7] = -1;
         s_lev_b = 0;
         s_lev_e = 1;
+        s_alloc = 1;
         s_err = 0;
     }
     __syncthreads();
@@ -601,38 +1130,29 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             const int got = atomicAdd(&s_ctr, lane == 0 ? 1 : 0);
             const int idx = uni(__shfl(got, 0, WAVE));
             if (idx >= le) break;
-            do_node(p, nodes + (int64_t)idx * CWP_NODE_INTS, idx == 0, a.rmax, scr, pp.rowcap,
-                    a.out + 4 * prob, (lint*)(s_stage + wave * CWP_STAGE_INTS));
+            int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
+            if (local_fits(uni(nd[2]), uni(nd[3])))
+                local_subtree(p, nodes, idx, s_local + wave * CWP_LOCAL_INTS, &s_alloc, cap, &s_err,
+                              a.out + 4 * prob);
+            else
+                do_node(p, nd, idx == 0, a.rmax, scr, pp.rowcap, a.out + 4 * prob);
         }
         __syncthreads();
         if (wave == 0) {
-            int next = le;
+            const int first = s_alloc;
+            int next = first;
             for (int base = lb; base < le; base += WAVE) {
                 const int idx = base + lane;
                 int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
-                const int type = idx < le ? nd[5] : 0;
+                const int type = (idx < le && !(nd[4] & kLocalFlag)) ? nd[5] : 0;
                 const uint64_t bal = __ballot(type != 0);
                 const int before = __popcll(bal & ((1ull << lane) - 1ull));
                 const int slot = next + 2 * before;
                 if (type != 0) {
                     if (slot + 2 <= cap) {
-                        const int A = nd[0], B = nd[1], M = nd[2], N = nd[3], fl = nd[4];
-                        const int go1 = fl & 1, go2 = (fl >> 1) & 1, midj = nd[6];
-                        const int midi = M / 2;
                         int* c0 = nodes + (int64_t)slot * CWP_NODE_INTS;
-                        int* c1 = c0 + CWP_NODE_INTS;
-                        if (type == 1) {
-                            c0[0] = A; c0[1] = B; c0[2] = midi; c0[3] = midj; c0[4] = go1 | 2;
-                            c1[0] = A + midi; c1[1] = B + midj; c1[2] = M - midi; c1[3] = N - midj;
-                            c1[4] = 1 | (go2 << 1);
-                        } else {
-                            c0[0] = A; c0[1] = B; c0[2] = midi - 1; c0[3] = midj; c0[4] = go1;
-                            c1[0] = A + midi + 1; c1[1] = B + midj; c1[2] = M - midi - 1; c1[3] = N - midj;
-                            c1[4] = go2 << 1;
-                        }
-                        c0[5] = c1[5] = 0;
-                        c0[6] = c1[6] = 0;
-                        c0[7] = c1[7] = -1;
+                        make_children(c0, c0 + CWP_NODE_INTS, nd[0], nd[1], nd[2], nd[3], nd[4], type,
+                                      nd[6]);
                         nd[7] = slot;
                     } else {
                         s_err = 1;
@@ -640,14 +1160,15 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
                 }
                 next += 2 * __popcll(bal);
             }
-            s_lev_b = le;                           // same value from every lane
+            s_lev_b = first;                        // same value from every lane
             s_lev_e = next < cap ? next : cap;
+            s_alloc = next < cap ? next : cap;
         }
         __syncthreads();
         ++levels;
     }
     if (threadIdx.x == 0) {
-        a.out[4 * prob + 1] = s_lev_e;
+        a.out[4 * prob + 1] = s_alloc;
         a.out[4 * prob + 2] = s_err;
         a.out[4 * prob + 3] = levels;
     }
