@@ -25,6 +25,7 @@ struct PrfProb {
     int64_t S;                // score matrix S[J * ldS + I] = prfScore(I, J)
     int64_t nodes;            // node records
     int64_t scratch;          // CWP_WAVES x CWP_SCR_ROWS x rowcap
+    int64_t lvl;              // final rows of a level's half sweeps: 4 x (L2 + 2 + node_cap)
 };
 
 struct PrfArgs {

@@ -352,75 +352,112 @@ __device__ void fill_top(const Prob& p, int A, int B, int M, int N, int* h0)
     wave_fence();
 }
 
-// One node of the recursion (one call of progDiff) by one wave.  Every lane
-// runs every statement here and writes the (identical) node fields: the node
-// loop of cw_prf_mm must contain no lane-divergent branch, or the compiler's
-// structurization can leave some lanes cycling through the loop header (whose
-// shuffle then reads an inactive lane) while others wait at its exit.  Kept
-// out of line for the same reason.
-__device__ __attribute__((noinline)) void do_node(const Prob& p, int* nd, int is_root, int rmax, int* scr, int rowcap,
-                        int* out_score)
+// A node of the recursion (one call of progDiff) outside the local subtrees is
+// done in two phases by whole waves: its two half sweeps as independent tasks
+// (so two waves work on one node), then, after a workgroup barrier, its
+// midpoint.  The half sweeps leave their final rows in the level buffers at
+// offset B + k (k = the node's position in its level): the nodes of a level
+// cover disjoint column ranges [B+1, B+N] in order, so these slices do not
+// overlap.  Every lane runs every statement of these functions and writes the
+// (identical) node fields: the task loops of cw_prf_mm must contain no
+// lane-divergent branch, or the compiler's structurization can leave some lanes
+// cycling through the loop header (whose shuffle then reads an inactive lane)
+// while others wait at its exit.  Kept out of line for the same reason.
+
+struct LevelBufs {
+    int* fH;   // forward final row H, [B+k+j], j = 1..N
+    int* fE;   // forward final row E
+    int* rH;   // reverse final row H, [B+k+c], reversed column c = 1..N
+    int* rK;   // reverse final row K
+};
+
+__device__ __forceinline__ LevelBufs level_bufs(int* lvl, int lcap, int B, int k)
+{
+    return LevelBufs{lvl + B + k, lvl + lcap + B + k, lvl + 2 * lcap + B + k, lvl + 3 * lcap + B + k};
+}
+
+// Leaves (:540-600): N <= 0, M <= 0, or M == 1.
+__device__ __attribute__((noinline)) void do_leaf(const Prob& p, int* nd, int is_root, int* out_score)
+{
+    const int lane = lane_id();
+    const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
+    const int go1 = uni(nd[4]) & 1;
+    int midj = 0, midh;
+    if (N <= 0) {
+        midh = is_root ? -gap_pen1(p, A, B, M) : 0;      // only the root's score is kept
+    } else if (M <= 0) {
+        midh = is_root ? -gap_pen2(p, A, B, N) : 0;
+    } else {
+        const int midh0 = go1 == 0 ? -gap_pen1(p, A + 1, B + 1, N)
+                                   : -gap_pen2(p, A + 1, B, 1) - gap_pen1(p, A + 1, B + 1, N);
+        int bv = -BIG, bj = BIG;
+        for (int j = 1 + lane; j <= N; j += WAVE) {
+            const int v = -gap_pen1(p, A, B + 1, j - 1) + p.S[(int64_t)(B + j) * p.ldS + A + 1] -
+                          gap_pen1(p, A + 1, B + j + 1, N - j);
+            if (v > bv) { bv = v; bj = j; }
+        }
+        const int V = wave_max(bv);
+        const int J = wave_min(bv == V ? bj : BIG);
+        if (V > midh0) { midh = V; midj = J; }
+        else midh = midh0;
+    }
+    nd[5] = 0;
+    nd[6] = midj;
+    nd[7] = -1;
+    if (is_root) *out_score = midh;
+}
+
+// Forward (half 0, rows 1..midi) or reverse (half 1, rows M..midi+1) sweep of
+// node nd (M >= 2, N >= 1) into the level buffers.
+__device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, int half, int rmax,
+                                                  int* scr, int rowcap, LevelBufs lv)
+{
+    const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
+    const int flags = uni(nd[4]);
+    const int go1 = flags & 1, go2 = (flags >> 1) & 1;
+    const int midi = M / 2;
+    const Bufs g{scr, scr + rowcap, scr + 2 * rowcap, scr + 3 * rowcap};
+    if (half == 0) {
+        PSweep F;
+        F.p = p;
+        F.nrows = midi; F.ncols = N;
+        F.ib = A; F.id = 1; F.jb = B; F.jd = 1;
+        F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
+        F.ltl = -ext2(p, A + 1, B);
+        F.go2 = 0; F.jedge = 0;
+        fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
+        psweep<MODE_FWD>(F, rmax, g, lv.fH, lv.fE);
+    } else {
+        PSweep Rv;
+        Rv.p = p;
+        Rv.nrows = M - midi; Rv.ncols = N;
+        Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
+        Rv.lt0 = 0; Rv.ltl = 0;
+        Rv.go2 = go2; Rv.jedge = B + N;
+        fill_top<MODE_REV>(p, A, B, M, N, g.h0);
+        psweep<MODE_REV>(Rv, rmax, g, lv.rH, lv.rK);
+    }
+    wave_fence();
+}
+
+// The midpoint of node nd (:756-776) from its half sweeps' final rows.
+__device__ __attribute__((noinline)) void do_mid(const Prob& p, int* nd, int is_root, LevelBufs lv,
+                                                 int* out_score)
 {
     const int lane = lane_id();
     const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
     const int flags = uni(nd[4]);
     const int go1 = flags & 1, go2 = (flags >> 1) & 1;
-    if (N <= 0 || M <= 1) {                                        // :540-600
-        int midj = 0, midh;
-        if (N <= 0) {
-            midh = is_root ? -gap_pen1(p, A, B, M) : 0;      // only the root's score is kept
-        } else if (M <= 0) {
-            midh = is_root ? -gap_pen2(p, A, B, N) : 0;
-        } else {
-            const int midh0 = go1 == 0 ? -gap_pen1(p, A + 1, B + 1, N)
-                                       : -gap_pen2(p, A + 1, B, 1) - gap_pen1(p, A + 1, B + 1, N);
-            int bv = -BIG, bj = BIG;
-            for (int j = 1 + lane; j <= N; j += WAVE) {
-                const int v = -gap_pen1(p, A, B + 1, j - 1) + p.S[(int64_t)(B + j) * p.ldS + A + 1] -
-                              gap_pen1(p, A + 1, B + j + 1, N - j);
-                if (v > bv) { bv = v; bj = j; }
-            }
-            const int V = wave_max(bv);
-            const int J = wave_min(bv == V ? bj : BIG);
-            if (V > midh0) { midh = V; midj = J; }
-            else midh = midh0;
-        }
-        nd[5] = 0;
-        nd[6] = midj;
-        nd[7] = -1;
-        if (is_root) *out_score = midh;
-        return;
-    }
     const int midi = M / 2;
-    const Bufs g{scr, scr + rowcap, scr + 2 * rowcap, scr + 3 * rowcap};
-    int* fH = scr + 4 * rowcap;
-    int* fE = scr + 5 * rowcap;
-    int* rH = scr + 6 * rowcap;
-    int* rK = scr + 7 * rowcap;
-
-    PSweep F;
-    F.p = p;
-    F.nrows = midi; F.ncols = N;
-    F.ib = A; F.id = 1; F.jb = B; F.jd = 1;
-    F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
-    F.ltl = -ext2(p, A + 1, B);
-    F.go2 = 0; F.jedge = 0;
-    fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
-    psweep<MODE_FWD>(F, rmax, g, fH, fE);
-
-    PSweep Rv;
-    Rv.p = p;
-    Rv.nrows = M - midi; Rv.ncols = N;
-    Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
-    Rv.lt0 = 0; Rv.ltl = 0;
-    Rv.go2 = go2; Rv.jedge = B + N;
-    fill_top<MODE_REV>(p, A, B, M, N, g.h0);
-    psweep<MODE_REV>(Rv, rmax, g, rH, rK);
-    wave_fence();
-
-    // midpoint (:756-776)
-    const int HH0 = left_h<MODE_FWD>(F, midi);
-    const int RRN = left_h<MODE_REV>(Rv, M - midi);
+    // HH[0]: left column of forward row midi; RR[N]: left column of reverse row M-midi
+    const int HH0 = (go1 ? -open2(p, A + 1, B) : 0) - midi * ext2(p, A + 1, B);
+    const int Ie = A + midi + 1;
+    const int4 ce = p.col[B + N];
+    const int RRN = (go2 ? -(ce.z & (p.row[Ie].x + ce.x)) : 0) - (ce.z & (p.pl1[A + M + 1] - p.pl1[Ie]));
+    const int* fH = lv.fH;
+    const int* fE = lv.fE;
+    const int* rH = lv.rH;
+    const int* rK = lv.rK;
     int vmax = -BIG, jfirst = BIG, jc = -1, wmax = -BIG, jw = -1;
     for (int j = lane; j <= N; j += WAVE) {
         const int HHj = (j == 0) ? HH0 : fH[j];
@@ -668,7 +705,7 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
     }
 }
 
-// One node of a local subtree (progDiff on LDS data): leaves as in do_node,
+// One node of a local subtree (progDiff on LDS data): leaves as in do_leaf,
 // internal nodes by lhalves and the midpoint of :756-776.  Writes type and
 // midj to the record; *midh_out gets the score (used for the root only).
 __device__ __attribute__((noinline)) void local_node(const LProb& q, int* rd, int is_root, lint* rows4,
@@ -1119,10 +1156,35 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         s_err = 0;
     }
     __syncthreads();
+    int* lvl = a.pool + pp.lvl;
+    const int lcap = pp.L2 + 2 + cap;
     int levels = 0;
     for (;;) {
         const int lb = s_lev_b, le = s_lev_e;
         if (lb >= le) break;
+        __syncthreads();
+        if (threadIdx.x == 0) s_ctr = 0;
+        __syncthreads();
+        // tasks 2k, 2k+1: the two halves of node lb+k (a local subtree or a leaf
+        // is one task)
+        const int ntask = 2 * (le - lb);
+        for (;;) {
+            const int got = atomicAdd(&s_ctr, lane == 0 ? 1 : 0);
+            const int t = uni(__shfl(got, 0, WAVE));
+            if (t >= ntask) break;
+            const int idx = lb + (t >> 1), half = t & 1;
+            int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
+            const int M = uni(nd[2]), N = uni(nd[3]);
+            if (local_fits(M, N)) {
+                if (half == 0)
+                    local_subtree(p, nodes, idx, s_local + wave * CWP_LOCAL_INTS, &s_alloc, cap, &s_err,
+                                  a.out + 4 * prob);
+            } else if (N <= 0 || M <= 1) {
+                if (half == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
+            } else {
+                do_half(p, nd, half, a.rmax, scr, pp.rowcap, level_bufs(lvl, lcap, uni(nd[1]), idx - lb));
+            }
+        }
         __syncthreads();
         if (threadIdx.x == 0) s_ctr = lb;
         __syncthreads();
@@ -1131,11 +1193,9 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             const int idx = uni(__shfl(got, 0, WAVE));
             if (idx >= le) break;
             int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
-            if (local_fits(uni(nd[2]), uni(nd[3])))
-                local_subtree(p, nodes, idx, s_local + wave * CWP_LOCAL_INTS, &s_alloc, cap, &s_err,
-                              a.out + 4 * prob);
-            else
-                do_node(p, nd, idx == 0, a.rmax, scr, pp.rowcap, a.out + 4 * prob);
+            const int M = uni(nd[2]), N = uni(nd[3]);
+            if (!local_fits(M, N) && N >= 1 && M >= 2)
+                do_mid(p, nd, idx == 0, level_bufs(lvl, lcap, uni(nd[1]), idx - lb), a.out + 4 * prob);
         }
         __syncthreads();
         if (wave == 0) {

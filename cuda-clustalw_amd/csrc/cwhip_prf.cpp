@@ -148,9 +148,10 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
         pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(L2 + 2) * pp.ldS);
         pp.nodes = work_sz; work_sz = up4(work_sz + (int64_t)pp.node_cap * CWP_NODE_INTS);
         pp.scratch = work_sz; work_sz += (int64_t)CWP_WAVES * CWP_SCR_ROWS * pp.rowcap;
+        pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)L2 + 2 + pp.node_cap));
         tile_base[k + 1] = tile_base[k] + ((L1 + 63) / 64) * ((L2 + 63) / 64);
     }
-    for (auto& pp : probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; }
+    for (auto& pp : probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
     // host images of the inputs: raw profiles, {gap, len, mask, gap+len}
     // records and LENCOL prefix sums (the mask is 0 at the profile ends when
     // end-gap penalties are off, the rule of openPenalty1/2 and extPenalty1/2)

@@ -62,7 +62,7 @@ int64_t footprint(const Job& j, int num_cu)
     int64_t b = total + 12 * n + 48 * pairs + waves * 8 * (lmax + 66) * 4 + (1 << 20);
     if (j.kind == CWHIP_JOB_ALIGNMENT) {
         const int64_t lp = lmax + lmax / 4 + 2;
-        b += (n / 2 + 1) * (lp * lp * 4 + 64 * lp * 4 + 4 * 8 * lp * 4);
+        b += (n / 2 + 1) * (lp * lp * 4 + 112 * lp * 4 + 4 * 8 * lp * 4);
         b += (n + 1) * (n + 1) * 16;
     }
     return b;
