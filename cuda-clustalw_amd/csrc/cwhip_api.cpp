@@ -94,6 +94,8 @@ void cwhip_destroy(cwhip_ctx* ctx)
     ctx->d_info.release(); ctx->d_matrix.release(); ctx->d_go.release();
     ctx->d_status.release(); ctx->d_scratch.release(); ctx->d_queue.release();
     ctx->d_bins.release();
+    ctx->p_pool.release(); ctx->p_out.release(); ctx->p_tiles.release(); ctx->p_probs.release();
+    ctx->h_prf_in.release(); ctx->h_prf_nodes.release();
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

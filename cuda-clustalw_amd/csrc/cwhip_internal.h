@@ -25,6 +25,22 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Page-locked host staging (hipHostMalloc), so copies to and from the device
+// are DMA transfers that do not block the calling thread.
+template <typename T>
+struct PinBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        if (p) { (void)hipHostFree(p); p = nullptr; n = 0; }
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+};
+
 struct cwhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -56,6 +72,7 @@ struct cwhip_ctx {
     // profile alignment
     DevBuf<int32_t> p_pool, p_out, p_tiles;
     DevBuf<PrfProb> p_probs;
+    PinBuf<int32_t> h_prf_in, h_prf_nodes;      // staging of a profile batch's inputs and node tables
     int prf_rmax = CWP_RMAX;
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;

@@ -121,6 +121,30 @@ bool emit_script(const int32_t* nodes, int nnodes, Emitter& em, double* cells)
     return true;
 }
 
+// fn(0..n-1) on up to 16 host threads (the caller's thread included)
+template <typename F>
+void parallel_for(size_t n, F fn)
+{
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    const size_t nt = std::min(hw, n);
+    if (nt <= 1) {
+        for (size_t k = 0; k < n; ++k) fn(k);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto body = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < n;) fn(k);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(body);
+    body();
+    for (auto& t : th) t.join();
+}
+
+// One device batch of profile pairs: inputs packed (on host threads) into
+// page-locked staging and uploaded in one copy, the score and Myers-Miller
+// kernels, the node tables (contiguous in the pool) fetched in one copy and
+// turned into edit scripts on host threads.
 int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
 {
     const int n = (int)jobs.size();
@@ -145,18 +169,28 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
         pp.col = in_sz; in_sz += 4 * (int64_t)(L2 + 2);
         pp.pl1 = in_sz; in_sz = up4(in_sz + L1 + 2);
         pp.pl2 = in_sz; in_sz = up4(in_sz + L2 + 2);
-        pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(L2 + 2) * pp.ldS);
-        pp.nodes = work_sz; work_sz = up4(work_sz + (int64_t)pp.node_cap * CWP_NODE_INTS);
-        pp.scratch = work_sz; work_sz += (int64_t)CWP_WAVES * CWP_SCR_ROWS * pp.rowcap;
-        pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)L2 + 2 + pp.node_cap));
         tile_base[k + 1] = tile_base[k] + ((L1 + 63) / 64) * ((L2 + 63) / 64);
+    }
+    // work area: node tables of all pairs first (one copy back), then per pair
+    // the score matrix, per-wave scratch rows and level buffers
+    for (int k = 0; k < n; ++k) {
+        probs[k].nodes = work_sz;
+        work_sz = up4(work_sz + (int64_t)probs[k].node_cap * CWP_NODE_INTS);
+    }
+    const int64_t nodes_sz = work_sz;
+    for (int k = 0; k < n; ++k) {
+        PrfProb& pp = probs[k];
+        pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(pp.L2 + 2) * pp.ldS);
+        pp.scratch = work_sz; work_sz += (int64_t)CWP_WAVES * CWP_SCR_ROWS * pp.rowcap;
+        pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)pp.L2 + 2 + pp.node_cap));
     }
     for (auto& pp : probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
     // host images of the inputs: raw profiles, {gap, len, mask, gap+len}
     // records and LENCOL prefix sums (the mask is 0 at the profile ends when
     // end-gap penalties are off, the rule of openPenalty1/2 and extPenalty1/2)
-    std::vector<int32_t> h_in(in_sz, 0);
-    for (int k = 0; k < n; ++k) {
+    HIPCHK(ctx->h_prf_in.ensure(in_sz + 4));
+    int32_t* h_in = ctx->h_prf_in.p;
+    parallel_for(n, [&](size_t k) {
         const PrfProb& pp = probs[k];
         const DpJob& j = jobs[k];
         std::memcpy(&h_in[pp.prf1], j.prf1, sizeof(int32_t) * (size_t)(j.L1 + 2) * W);
@@ -178,12 +212,12 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
                 acc += l;
             }
         }
-    }
+    });
     HIPCHK(ctx->p_pool.ensure(in_sz + work_sz + 4));
     HIPCHK(ctx->p_probs.ensure(n));
     HIPCHK(ctx->p_out.ensure(4 * (size_t)n));
     HIPCHK(ctx->p_tiles.ensure(n + 1));
-    HIPCHK(hipMemcpyAsync(ctx->p_pool.p, h_in.data(), in_sz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->p_pool.p, h_in, in_sz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->p_probs.p, probs.data(), n * sizeof(PrfProb), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->p_tiles.p, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     PrfArgs a;
@@ -198,31 +232,38 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(cwp_launch_mm(n, ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
-    std::vector<int32_t> out(4 * (size_t)n);
-    HIPCHK(hipMemcpyAsync(out.data(), ctx->p_out.p, out.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx->h_prf_nodes.ensure(nodes_sz + 4 * (size_t)n + 4));
+    int32_t* h_nodes = ctx->h_prf_nodes.p;
+    int32_t* out = h_nodes + nodes_sz;
+    HIPCHK(hipMemcpyAsync(out, ctx->p_out.p, 4 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_nodes, ctx->p_pool.p + in_sz, nodes_sz * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms1 = 0, ms2 = 0;
     HIPCHK(hipEventElapsedTime(&ms1, ctx->ev0, ctx->ev1));
     HIPCHK(hipEventElapsedTime(&ms2, ctx->ev1, ctx->ev2));
     ctx->prf_score_ms += ms1;
     ctx->prf_mm_ms += ms2;
-    std::vector<int32_t> nodes;
     for (int k = 0; k < n; ++k) {
         const int nn = out[4 * k + 1];
         if (out[4 * k + 2] != 0 || nn < 1 || nn > probs[k].node_cap)
             return fail(ctx, CWHIP_E_KERNEL, "profile DP node table overflow");
-        nodes.resize((size_t)nn * CWP_NODE_INTS);
-        HIPCHK(hipMemcpyAsync(nodes.data(), ctx->p_pool.p + probs[k].nodes,
-                              nodes.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<char> bad(n, 0);
+    std::vector<double> cells(n, 0.0);
+    parallel_for(n, [&](size_t k) {
         Emitter em{jobs[k].displ};
-        double cells = 0;
-        if (!emit_script(nodes.data(), nn, em, &cells))
-            return fail(ctx, CWHIP_E_KERNEL, "profile DP node table is malformed");
+        if (!emit_script(h_nodes + (probs[k].nodes - in_sz), out[4 * k + 1], em, &cells[k])) {
+            bad[k] = 1;
+            return;
+        }
         jobs[k].n_displ = em.printPtr - 1;
         jobs[k].score = out[4 * k + 0];
-        jobs[k].cells = cells;
-        ctx->prf_cells += cells;
+        jobs[k].cells = cells[k];
+    });
+    for (int k = 0; k < n; ++k) {
+        if (bad[k]) return fail(ctx, CWHIP_E_KERNEL, "profile DP node table is malformed");
+        ctx->prf_cells += cells[k];
         ctx->prf_entries += (double)jobs[k].L1 * jobs[k].L2;
     }
     return CWHIP_OK;
@@ -310,11 +351,45 @@ struct ProfileSide {
     int32_t& at(int i, int j) { return prf[(size_t)i * W + j]; }
 };
 
+// Per column of rows[first..last): the weighted residue counts and the plain
+// counts of every code, gathered in one row-major pass (on host threads over
+// column blocks when `par` and the group is large).  Integer sums, so the
+// order of the additions does not matter.
+struct ColHist {
+    std::vector<int> wsum, cnt;   // [col * NUMRES + code]
+};
+
+void column_hist(const std::vector<const uint8_t*>& rows, const std::vector<int>& w, int first,
+                 int last, int L, bool par, ColHist& h)
+{
+    h.wsum.assign((size_t)L * NUMRES, 0);
+    h.cnt.assign((size_t)L * NUMRES, 0);
+    constexpr int blk = 256;
+    const size_t nb = (size_t)(L + blk - 1) / blk;
+    auto body = [&](size_t b) {
+        const int c0 = (int)b * blk, c1 = std::min(L, c0 + blk);
+        for (int k = first; k < last; ++k) {
+            const uint8_t* r = rows[k];
+            const int wk = w[k];
+            for (int col = c0; col < c1; ++col) {
+                const int c = r[col];
+                if (c < NUMRES) {
+                    h.wsum[(size_t)col * NUMRES + c] += wk;
+                    h.cnt[(size_t)col * NUMRES + c] += 1;
+                }
+            }
+        }
+    };
+    if (par && (int64_t)(last - first) * L > (1 << 18)) parallel_for(nb, body);
+    else
+        for (size_t b = 0; b < nb; ++b) body(b);
+}
+
 // ProfileBase::calcGapCoeff (ProfileBase.cpp:52-298) and its helpers
 // (:310-549).  `rows` are the step's padded rows; `gaps` is the per-column
 // count the caller shares with calcProfileWithSub.
 void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector<const uint8_t*>& rows,
-                    std::vector<int>& gaps, int gapCoef, int lenCoef)
+                    const ColHist& hist, std::vector<int>& gaps, int gapCoef, int lenCoef)
 {
     const int L = s.L, first = s.first, last = s.last, numSeq = last - first;
     int pcid = 0;
@@ -381,10 +456,7 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
         resW.assign(L + 2, 0);
         for (int i = 0; i < L; ++i) {
             int w = 0;
-            for (int k = first; k < last; ++k) {
-                const int c = rows[k][i];
-                if (!is_gap_code(c)) w += pen[c];
-            }
+            for (int c = 0; c <= MAXAA; ++c) w += hist.cnt[(size_t)i * NUMRES + c] * pen[c];
             resW[i] = w / numSeq;
         }
     }
@@ -476,34 +548,21 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
     }
 }
 
-// Weighted residue counts of one column over rows[first..last).
-inline void column_weights(const std::vector<const uint8_t*>& rows, const std::vector<int>& w,
-                           int first, int last, int col, int wt[NUMRES])
-{
-    for (int c = 0; c < NUMRES; ++c) wt[c] = 0;
-    for (int k = first; k < last; ++k) {
-        const int c = rows[k][col];
-        if (c < NUMRES) wt[c] += w[k];
-    }
-}
-
 // ProfileWithSub::calcProfileWithSub (ProfileWithSub.cpp:42-147)
-void calc_profile_with_sub(ProfileSide& s, const std::vector<const uint8_t*>& rows,
-                           const std::vector<int>& gaps, const int mat[NUMRES][NUMRES],
-                           const std::vector<int>& w)
+void calc_profile_with_sub(ProfileSide& s, const ColHist& hist, const std::vector<int>& gaps,
+                           const int mat[NUMRES][NUMRES], const std::vector<int>& w, bool par)
 {
     const int numSeq = s.last - s.first;
     int sum2 = 0;
     for (int k = s.first; k < s.last; ++k) sum2 += w[k];
-    int wt[NUMRES];
-    for (int pos = 0; pos < s.L; ++pos) {
+    auto column = [&](int pos) {
         if (gaps[pos] == numSeq) {
             for (int res = 0; res <= MAXAA; ++res) s.at(pos + 1, res) = mat[res][GAP1];
             s.at(pos + 1, GAP1) = mat[GAP1][GAP1];
             s.at(pos + 1, GAP2) = mat[GAP2][GAP1];
-            continue;
+            return;
         }
-        column_weights(rows, w, s.first, s.last, pos, wt);
+        const int* wt = &hist.wsum[(size_t)pos * NUMRES];
         const float scale = (float)(numSeq - gaps[pos]) / (float)numSeq;
         // the weighted sum over the codes present in the column (integer, so
         // the order and the zero terms the reference also adds do not matter)
@@ -520,16 +579,22 @@ void calc_profile_with_sub(ProfileSide& s, const std::vector<const uint8_t*>& ro
         for (int res = 0; res <= MAXAA; ++res) s.at(pos + 1, res) = one(res);
         s.at(pos + 1, GAP1) = one(GAP1);
         s.at(pos + 1, GAP2) = one(GAP2);
-    }
+    };
+    constexpr int blk = 128;
+    const size_t nb = (size_t)(s.L + blk - 1) / blk;
+    auto body = [&](size_t b) {
+        for (int pos = (int)b * blk; pos < std::min(s.L, (int)(b + 1) * blk); ++pos) column(pos);
+    };
+    if (par && (int64_t)numSeq * s.L > (1 << 16)) parallel_for(nb, body);
+    else
+        for (size_t b = 0; b < nb; ++b) body(b);
 }
 
 // ProfileStandard::calcStandardProfile (ProfileStandard.cpp:41-117)
-void calc_standard_profile(ProfileSide& s, const std::vector<const uint8_t*>& rows,
-                           const std::vector<int>& w)
+void calc_standard_profile(ProfileSide& s, const ColHist& hist, const std::vector<int>& w)
 {
     int sum2 = 0;
     for (int k = s.first; k < s.last; ++k) sum2 += w[k];
-    int wt[NUMRES];
     for (int r = 0; r < s.L; ++r) {
         if (sum2 == 0) {
             for (int d = 0; d <= MAXAA; ++d) s.at(r + 1, d) = 0;
@@ -537,31 +602,11 @@ void calc_standard_profile(ProfileSide& s, const std::vector<const uint8_t*>& ro
             s.at(r + 1, GAP2) = 0;
             continue;
         }
-        column_weights(rows, w, s.first, s.last, r, wt);
+        const int* wt = &hist.wsum[(size_t)r * NUMRES];
         for (int d = 0; d <= MAXAA; ++d) s.at(r + 1, d) = (int)(10 * (float)wt[d] / (float)sum2);
         s.at(r + 1, GAP1) = (int)(10 * (float)wt[GAP1] / (float)sum2);
         s.at(r + 1, GAP2) = (int)(10 * (float)wt[GAP2] / (float)sum2);
     }
-}
-
-// fn(0..n-1) on up to 16 host threads (the caller's thread included)
-template <typename F>
-void parallel_for(size_t n, F fn)
-{
-    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    const size_t nt = std::min(hw, n);
-    if (nt <= 1) {
-        for (size_t k = 0; k < n; ++k) fn(k);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    auto body = [&]() {
-        for (size_t k; (k = next.fetch_add(1)) < n;) fn(k);
-    };
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; ++t) th.emplace_back(body);
-    body();
-    for (auto& t : th) t.join();
 }
 
 // One profileAlign call (MyersMillerProfileAlign.cpp:36-505) split into the
@@ -581,7 +626,7 @@ using Seqs = std::vector<std::vector<uint8_t>>;
 // Returns false when a group is empty (profileAlign returns 0, :96-99).
 bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim,
                   const std::vector<int>& weights, std::vector<int>& group,
-                  const std::vector<int>& aligned, Step& st)
+                  const std::vector<int>& aligned, Step& st, bool par)
 {
     const int nseq = (int)seqs.size();
     for (int i = 0; i < nseq; ++i)
@@ -674,10 +719,13 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim
     st.p1.prf.assign((size_t)(L1 + 2) * W, 0);
     st.p2.first = n1; st.p2.last = n1 + n2; st.p2.L = L2;
     st.p2.prf.assign((size_t)(L2 + 2) * W, 0);
-    calc_gap_coeff(p, st.p1, st.rows, gaps, gapcoef, lencoef);
-    calc_profile_with_sub(st.p1, st.rows, gaps, mat, w);
-    calc_gap_coeff(p, st.p2, st.rows, gaps, gapcoef, lencoef);
-    calc_standard_profile(st.p2, st.rows, w);
+    ColHist hist;
+    column_hist(st.rows, w, 0, n1, L1, par, hist);
+    calc_gap_coeff(p, st.p1, st.rows, hist, gaps, gapcoef, lencoef);
+    calc_profile_with_sub(st.p1, hist, gaps, mat, w, par);
+    column_hist(st.rows, w, n1, n1 + n2, L2, par, hist);
+    calc_gap_coeff(p, st.p2, st.rows, hist, gaps, gapcoef, lencoef);
+    calc_standard_profile(st.p2, hist, w);
     st.displ.assign(L1 + L2 + 4, 0);
     st.job.prf1 = st.p1.prf.data();
     st.job.prf2 = st.p2.prf.data();
@@ -773,7 +821,7 @@ int cwhip_profile_align(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     std::vector<int> w(weights, weights + nseq);
     Step st;
     for (int k = 0; k < 4; ++k) info[k] = 0;
-    const bool run = prepare_step(*p, seqs, sim, w, g, al, st);
+    const bool run = prepare_step(*p, seqs, sim, w, g, al, st, true);
     for (int i = 0; i <= nseq; ++i) group[i] = g[i];          // profileAlign mutates group
     if (!run) return CWHIP_OK;                                 // an empty group: score 0 (:96-99)
     if (rows_out && stride < st.L1 + st.L2 + 2) return fail(ctx, CWHIP_E_ARG, "row stride too small");
@@ -886,7 +934,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         // (and later apply their edit scripts) on several host threads
         parallel_for(nst, [&](size_t k) {
             std::vector<int> grp = groups[by_level[lv][k]];
-            ok[k] = prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k]) ? 1 : 0;
+            ok[k] = prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k], nst < 4) ? 1 : 0;
         });
         std::vector<DpJob> jobs;
         std::vector<int> live;
@@ -950,7 +998,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         }
         aligned[iseq] = 1;
         Step st;
-        if (prepare_step(P, seqs, sim, weights, group, aligned, st)) {
+        if (prepare_step(P, seqs, sim, weights, group, aligned, st, true)) {
             std::vector<DpJob> jobs{st.job};
             const int rc = run_dp_batch(ctx, jobs, P.end_gap_penalties ? 1 : 0);
             if (rc != CWHIP_OK) return rc;
