@@ -220,11 +220,18 @@ def serving_stage(rate: float, count: int, seed: int = 20260105):
             same &= (dnd == srv.text(jobs[i], 0) and aln == srv.text(jobs[i], 1))
     ctx.close()
     srv.close()
+    kinds_a = np.array(kinds)
+
+    def pct(sel, q):
+        return round(float(np.percentile(lat[sel], q)), 4) if sel.any() else None
+
     return {"rate_per_s": rate, "requests": count, "alignment_jobs": int(sum(kinds)),
             "distance_jobs": int(count - sum(kinds)),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
             "latency_mean_s": round(float(lat.mean()), 4),
+            "distance_jobs_p50_p99_s": [pct(kinds_a == 0, 50), pct(kinds_a == 0, 99)],
+            "alignment_jobs_p50_p99_s": [pct(kinds_a == 1, 50), pct(kinds_a == 1, 99)],
             "throughput_gcups": round(cells / window / 1e9, 2), "window_s": round(float(window), 3),
             "max_concurrent_jobs": st["max_running"], "peak_footprint_bytes": st["peak_bytes"],
             "slots": st["slots"], "spot_check_identical_to_direct_path": bool(same)}
