@@ -5,6 +5,7 @@
 #define CW_SEQ_PAD 64          // code-0 padding on each side of a per-wave LDS sequence
 #define CW_SMALL_COLS 256      // Myers-Miller nodes with N <= this run lanes-own-columns, rows in LDS
 #define CW_STACK_CAP 48        // Myers-Miller stack entries per wave (depth <= log2(L)+2)
+#define CW_TINYQ 64            // queued tiny Myers-Miller subtrees per wave (one per lane)
 #define CW_TAB_BYTES 4096      // int32 [32][32] substitution matrix in LDS
 #define CW_MAX_THREADS 256     // threads per workgroup (4 waves)
 #define CW_RMAX 8              // rows per lane the kernels are instantiated for
@@ -58,7 +59,7 @@ static inline __host__ __device__ int cw_bin_of(int n, int m)
 
 static inline int cw_wave_lds_bytes(int lcap)
 {
-    int b = 2 * lcap + 4 * (4 * (CW_SMALL_COLS + 1) + 6 * CW_STACK_CAP);
+    int b = 2 * lcap + 4 * (4 * (CW_SMALL_COLS + 1) + 6 * CW_STACK_CAP + 2 * CW_TINYQ);
     return (b + 15) & ~15;
 }
 

@@ -395,16 +395,195 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
 }
 
 
+// ---- tiny subtrees, one lane each ---------------------------------------------
+// A node with 2 <= M <= kTinyRows and N <= kTinyCols, and every M == 1 leaf, is
+// not worth a wave: the depth-first walk queues such nodes (two ints each) and
+// every lane then solves one of them with its whole subtree as scalar code:
+// the columns unrolled in registers (the forward half's H/E and the reverse
+// half's H/E kept at the same column index j), the midpoint as the
+// reference's own scan over j = 0..N, the children on a small register stack.
+// The end penalties of every node are 0 or the gap-open penalty, so a node
+// packs into {A | B << 16, M | N << 5 | tb? << 30 | te? << 31}.
+
+constexpr int kTinyRows = 8;
+constexpr int kTinyCols = 16;
+constexpr int kTinyStack = 6;
+
+__device__ __forceinline__ bool pw_tiny(int M, int N)
+{
+    return M == 1 || (M >= 2 && M <= kTinyRows && N >= 1 && N <= kTinyCols);
+}
+
+__device__ __attribute__((noinline)) void pw_tiny_subtree(lcu8* tab, lcu8* s1, lcu8* s2, int go, int ge, int gap1,
+                                                int gap2, int w0, int w1, int& count, int& cells,
+                                                int& nodes, int& err)
+{
+    constexpr int NT = kTinyCols;
+    const int goe = go + ge;
+    int st0[kTinyStack], st1[kTinyStack];
+#pragma unroll
+    for (int k = 0; k < kTinyStack; ++k) st0[k] = st1[k] = 0;
+    st0[0] = w0;
+    st1[0] = w1;
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int e0 = 0, e1 = 0;
+#pragma unroll
+        for (int k = 0; k < kTinyStack; ++k)
+            if (k == sp) { e0 = st0[k]; e1 = st1[k]; }
+        const int A = e0 & 0xffff, B = (e0 >> 16) & 0xffff;
+        const int M = e1 & 31, N = (e1 >> 5) & 0x1ffffff;
+        const int tb = ((e1 >> 30) & 1) ? go : 0, te = ((e1 >> 31) & 1) ? go : 0;
+        if (N <= 0 || M <= 0) continue;
+        if (M == 1) {                                     // :435-478
+            cells += N;
+            const int tegN = te + ge * N, tbgN = tb + ge * N;
+            int midh0 = -(tb + ge) - tegN;
+            const int alt = -(te + ge) - tbgN;
+            if (alt > midh0) midh0 = alt;
+            const int a = s1[A + 1];
+            int bv = -BIG, bj = 0;
+            for (int j = 1; j <= N; ++j) {
+                const int k1 = N - j, k2 = j - 1;
+                const int v = score_at(tab, a * 128 + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
+                              (k2 <= 0 ? 0 : tb + ge * k2);
+                if (v > bv) { bv = v; bj = j; }
+            }
+            if (bv > midh0) {
+                const int r2 = s2[B + bj] >> 2;
+                if (a != gap1 && r2 != gap2 && a == r2) ++count;
+            }
+            continue;
+        }
+        ++nodes;
+        cells += M * N;
+        const int midi = M / 2;
+        // forward half: rows 1..midi from (A,B); H/E[k] hold column k
+        int H[NT + 1], E[NT + 1];
+#pragma unroll
+        for (int k = 0; k <= NT; ++k) {
+            H[k] = -tb - k * ge;
+            E[k] = H[k] - go;
+        }
+        H[0] = 0;
+        for (int i = 1; i <= midi; ++i) {
+            const int a128 = (int)s1[A + i] * 128;
+            int d = H[0];
+            int hc = -tb - i * ge, fc = hc - go;
+            H[0] = hc;
+#pragma unroll
+            for (int k = 1; k <= NT; ++k) {
+                if (k <= N) {
+                    const int sc = score_at(tab, a128 + s2[B + k]);
+                    const int en = max(E[k] - ge, H[k] - goe);
+                    const int fn = max(fc - ge, hc - goe);
+                    const int h = max(max(d + sc, en), fn);
+                    d = H[k];
+                    H[k] = h;
+                    E[k] = en;
+                    hc = h;
+                    fc = fn;
+                }
+            }
+        }
+        // reverse half: rows M..midi+1 from (A+M+1,B+N+1), columns N..1;
+        // RH/RE[k] hold column k (reversed column N+1-k)
+        int RH[NT + 2], RE[NT + 2];
+#pragma unroll
+        for (int k = 0; k <= NT + 1; ++k) {
+            RH[k] = -te - (N + 1 - k) * ge;
+            RE[k] = RH[k] - go;
+        }
+        int rprev = 0;                                   // H of the reverse left column, row r-1
+        for (int r = 1; r <= M - midi; ++r) {
+            const int a128 = (int)s1[A + M + 1 - r] * 128;
+            int d = rprev;
+            int hc = -te - r * ge, fc = hc - go;
+            rprev = hc;
+#pragma unroll
+            for (int k = NT; k >= 1; --k) {
+                if (k <= N) {
+                    const int sc = score_at(tab, a128 + s2[B + k]);
+                    const int en = max(RE[k] - ge, RH[k] - goe);
+                    const int fn = max(fc - ge, hc - goe);
+                    const int h = max(max(d + sc, en), fn);
+                    d = RH[k];
+                    RH[k] = h;
+                    RE[k] = en;
+                    hc = h;
+                    fc = fn;
+                }
+            }
+        }
+        // midpoint (:570-593) in the reference's scan order
+        const int HH0 = -tb - midi * ge;
+        const int RRN = -te - (M - midi) * ge;
+        int vmax = -BIG, jfirst = 0, jc = -1, wmax = -BIG, jw = -1;
+#pragma unroll
+        for (int j = 0; j <= NT; ++j) {
+            if (j <= N) {
+                const int HHj = (j == 0) ? HH0 : H[j];
+                const int DDj = (j == 0) ? HH0 : E[j];
+                const int RRj = (j == N) ? RRN : RH[j + 1];
+                const int SSj = (j == N) ? RRN : RE[j + 1];
+                const int v = HHj + RRj;
+                const int w = DDj + SSj + go;
+                if (v > vmax) { vmax = v; jfirst = j; jc = -1; }
+                if (v == vmax && HHj != DDj && RRj == SSj) jc = j;
+                if (w >= wmax) { wmax = w; jw = j; }
+            }
+        }
+        int type, midj;
+        if (wmax > vmax) { type = 2; midj = jw; }
+        else { type = 1; midj = jfirst > jc ? jfirst : jc; }
+        int c0a, c0b, c1a, c1b;                          // children, packed
+        const int tbf = (e1 >> 30) & 1, tef = (e1 >> 31) & 1;
+        if (type == 1) {
+            c0a = A | (B << 16);
+            c0b = midi | (midj << 5) | (tbf << 30) | (1 << 31);
+            c1a = (A + midi) | ((B + midj) << 16);
+            c1b = (M - midi) | ((N - midj) << 5) | (1 << 30) | (tef << 31);
+        } else {
+            c0a = A | (B << 16);
+            c0b = (midi - 1) | (midj << 5) | (tbf << 30);
+            c1a = (A + midi + 1) | ((B + midj) << 16);
+            c1b = (M - midi - 1) | ((N - midj) << 5) | (tef << 31);
+        }
+        if (sp + 2 > kTinyStack) { err = 1; break; }
+#pragma unroll
+        for (int k = 0; k < kTinyStack; ++k) {
+            if (k == sp) { st0[k] = c1a; st1[k] = c1b; }
+            if (k == sp + 1) { st0[k] = c0a; st1[k] = c0b; }
+        }
+        sp += 2;
+    }
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
 // FullPairwiseAlign::diff (:418-612) restated as a depth-first walk; returns the
 // identity count summed over the M==1 leaves (what tracePath counts).
 __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
                         int gap1, int gap2, int sb1, int sb2, int se1, int se2, int rmax,
                         const Bufs& g, int* g2H, int* g2E, int* g3H, int* g3E,
                         int* smallF, int* smallR, int small_cap, int colcap, int* stack,
-                        int stack_cap, int* status, int* nodes_out, int* cells_out)
+                        int stack_cap, int* tq, int* status, int* nodes_out, int* cells_out)
 {
     const int lane = lane_id();
     int count = 0, sp = 0, nodes = 0, cells = 0;
+    // per-lane totals of the tiny subtrees, summed over the wave at the end
+    int lcount = 0, lcells = 0, lnodes = 0, lerr = 0, nq = 0;
+    auto drain = [&]() {
+        if (lane < nq)
+            pw_tiny_subtree(base.tab, s1, s2, go, ge, gap1, gap2, tq[2 * lane], tq[2 * lane + 1], lcount,
+                            lcells, lnodes, lerr);
+        nq = 0;
+        wave_fence();
+    };
     if (lane == 0) {
         int* e = stack;
         e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
@@ -418,6 +597,14 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         const int tb = uni(e[4]), te = uni(e[5]);
         wave_fence();
         if (N <= 0 || M <= 0) continue;                 // only del()/add(): no identities
+        if (pw_tiny(M, N) && A < 0x10000 && B < 0x10000) {
+            tq[2 * nq] = A | (B << 16);                  // same value from every lane
+            tq[2 * nq + 1] = M | (N << 5) | ((tb != 0) << 30) | ((te != 0) << 31);
+            nq += 1;
+            wave_fence();
+            if (nq == CW_TINYQ) drain();
+            continue;
+        }
         if (M == 1) {                                     // :435-478
             cells += N;
             const int tegN = te + ge * N;                 // N >= 1
@@ -513,6 +700,11 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         sp += 2;
         wave_fence();
     }
+    drain();
+    count += wave_sum(lcount);
+    nodes += wave_sum(lnodes);
+    cells += wave_sum(lcells);
+    if (wave_sum(lerr) != 0 && lane == 0) atomicOr(status, 2);
     *nodes_out = nodes;
     *cells_out = cells;
     return count;
@@ -548,6 +740,7 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
     int* smallF = (int*)(uint8_t*)(wlds + 2 * A.lcap);
     int* smallR = smallF + 2 * (CW_SMALL_COLS + 1);
     int* stack = smallR + 2 * (CW_SMALL_COLS + 1);
+    int* tq = stack + 6 * CW_STACK_CAP;
     load_seq(s1, A.codes + A.offs[si], n, pad, 1);
     load_seq(s2, A.codes + A.offs[sj], m, pad, 4);
     wave_fence();
@@ -606,7 +799,7 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
     int count = 0;
     if (se1 - sb1 + 1 > 0 && se2 - sb2 + 1 > 0 && maxScore > 0) {
         count = mm_count(S, s1, s2, go, ge, A.gap1, A.gap2, sb1, sb2, se1, se2, rmax, g, g2H,
-                         g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, colcap, stack, CW_STACK_CAP,
+                         g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, colcap, stack, CW_STACK_CAP, tq,
                          A.status, &nodes, &mmcells);
     }
     if (lane == 0) {

@@ -493,8 +493,9 @@ int cwhip_autotune(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t* pa
     for (int c = 0; c < CW_NCLASS; ++c) {
         const auto& mem = members[c];
         if (mem.empty()) continue;
-        // a strided sample that fills every wave twice
-        const int64_t want = std::min<int64_t>((int64_t)mem.size(), 2 * waves);
+        // a strided sample that fills every wave four times (fewer than that and
+        // the queue's tail decides the time)
+        const int64_t want = std::min<int64_t>((int64_t)mem.size(), 4 * waves);
         std::vector<int32_t> sp(2 * want);
         for (int64_t i = 0; i < want; ++i) {
             const int64_t k = mem[(size_t)(i * (int64_t)mem.size() / want)];
@@ -503,23 +504,28 @@ int cwhip_autotune(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t* pa
         }
         rc = cwhip_stage_pairs(ctx, p, sp.data(), want);
         if (rc) { ctx->tune = saved; return rc; }
+        // the class's current shape first; a candidate replaces it only when it
+        // is at least 3% faster (best of three runs each)
+        std::vector<std::pair<int, int>> cands{{saved.rmax[c], saved.colcap[c]}};
+        for (const auto& cand : tune_candidates(lmax[c]))
+            if (cand != cands[0]) cands.push_back(cand);
         float best_ms = 1e30f;
-        for (const auto& cand : tune_candidates(lmax[c])) {
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
             for (int k = 0; k < CW_NCLASS; ++k) {
-                ctx->tune.rmax[k] = cand.first;
-                ctx->tune.colcap[k] = cand.second;
+                ctx->tune.rmax[k] = cands[ci].first;
+                ctx->tune.colcap[k] = cands[ci].second;
             }
             float t = 0.f, tmin = 1e30f;
-            for (int rep = 0; rep < 2; ++rep) {
+            for (int rep = 0; rep < 3; ++rep) {
                 rc = cwhip_run_staged(ctx, CWHIP_KERNEL_BATCHED, 1, &t, nullptr);
                 if (rc) { ctx->tune = saved; return rc; }
                 spent += t;
                 tmin = std::min(tmin, t);
             }
-            if (tmin < best_ms) {
+            if (ci == 0 || tmin < 0.97f * best_ms) {
                 best_ms = tmin;
-                best.rmax[c] = cand.first;
-                best.colcap[c] = cand.second;
+                best.rmax[c] = cands[ci].first;
+                best.colcap[c] = cands[ci].second;
             }
         }
     }

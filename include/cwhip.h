@@ -147,7 +147,9 @@ int cwhip_get_tune_table(cwhip_ctx *ctx, cwhip_tune_table *t);
 int cwhip_set_tune_table(cwhip_ctx *ctx, const cwhip_tune_table *t);
 /* Autotuner: for every class present in `pairs` (the caller's loaded
  * sequences), times the batched kernel on a strided sample of that class's
- * pairs under each candidate shape and keeps the fastest.  The table goes to
+ * pairs (four per resident wave) under the class's current shape and each
+ * candidate, best of three runs, and moves to a candidate only when it is at
+ * least 3% faster than the best so far.  The table goes to
  * *out (may be NULL), becomes this context's table and the default of contexts
  * created later on the same device.  *tune_ms (may be NULL) receives the device
  * time spent.  A pair list staged before the call is staged again afterwards. */
