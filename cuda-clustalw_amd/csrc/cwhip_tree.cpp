@@ -13,6 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
+#include <functional>
+#include <memory>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -24,11 +28,50 @@ namespace {
 // NJTree::generateTree (tree/NJTree.cpp:161-751) on a (n+1)^2 symmetric
 // matrix, with ClusterTreeOutput's tree description (treeDesc) and branch
 // lengths (leftBranch/rightBranch).
+// Host threads for the NJ minimum search: T-1 workers spin on a generation
+// counter; run(f) calls f(0..T-1) (f(0) on the caller) and returns when all
+// are done.
+struct SpinPool {
+    int nt = 1;
+    std::vector<std::thread> th;
+    std::atomic<int> gen{0}, done{0};
+    std::atomic<bool> stop{false};
+    std::function<void(int)> job;
+    explicit SpinPool(int n) : nt(std::max(1, n)) {
+        for (int t = 1; t < nt; ++t)
+            th.emplace_back([this, t]() {
+                int seen = 0;
+                for (;;) {
+                    int g;
+                    while ((g = gen.load(std::memory_order_acquire)) == seen) {
+                        if (stop.load(std::memory_order_acquire)) return;
+                        std::this_thread::yield();
+                    }
+                    seen = g;
+                    job(t);
+                    done.fetch_add(1, std::memory_order_acq_rel);
+                }
+            });
+    }
+    void run(const std::function<void(int)>& f) {
+        job = f;
+        done.store(0, std::memory_order_release);
+        gen.fetch_add(1, std::memory_order_acq_rel);
+        f(0);
+        while (done.load(std::memory_order_acquire) < nt - 1) std::this_thread::yield();
+    }
+    ~SpinPool() {
+        stop.store(true, std::memory_order_release);
+        for (auto& t : th) t.join();
+    }
+};
+
 struct NJ {
     int n;
     std::vector<double> d;        // (n+1)^2, SymMatrix semantics
     std::vector<int> desc;        // treeDesc, (n+1)^2
     std::vector<double> left, right;
+    bool sym = true;
     double& D(int i, int j) { return d[(size_t)i * (n + 1) + j]; }
     int& DESC(int i, int j) { return desc[(size_t)i * (n + 1) + j]; }
     void set(int i, int j, double v) { D(i, j) = v; D(j, i) = v; }
@@ -58,6 +101,17 @@ struct NJ {
             sumCols[j] = tmp;
         }
         double fnseqs = (double)last;
+        // D(i, j) == D(j, i) for every pair (a DistMatrix is symmetric): the search
+        // may then read rows, which are contiguous
+        sym = true;
+        for (int i = 1; i <= last && sym; ++i)
+            for (int j = i + 1; j <= last; ++j)
+                if (D(i, j) != D(j, i)) { sym = false; break; }
+        std::vector<int> lv(last + 1);
+        std::vector<double> dq(last + 1);
+        std::unique_ptr<SpinPool> pool;
+        if (last > 256)
+            pool.reset(new SpinPool((int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()))));
         desc.assign((size_t)(n + 1) * (n + 1), 0);
         left.assign(n + 2, 0.0);
         right.assign(n + 2, 0.0);
@@ -67,19 +121,60 @@ struct NJ {
             const double fnseqs2 = fnseqs - 2.0;
             double tmin = 99999.0 * 2.0 * fnseqs2;
             int mini = 0, minj = 0;
-            int jj = (nxt[0] != NIL && nxt[0] == 1) ? nxt[nxt[0]] : nxt[0];
-            for (; jj != NIL; jj = nxt[jj]) {
-                const double djq = sumCols[jj] + sumRows[jj];
-                for (int ii = nxt[0]; ii < jj; ii = nxt[ii]) {
-                    const double diq = sumCols[ii] + sumRows[ii];
-                    const double total = fnseqs2 * D(ii, jj) + 2.0 * sumd - diq - djq;
-                    if (total < tmin) {
-                        tmin = total;
-                        mini = ii;
-                        minj = jj;
+            // the pair (ii < jj) of live nodes with the smallest total, first in
+            // the reference's scan order (jj ascending, then ii ascending); the
+            // scan is split into contiguous jj ranges on host threads and the
+            // ranges' winners are taken in order with the same strict test
+            int m = 0;
+            for (int j = nxt[0]; j != NIL; j = nxt[j]) {
+                lv[m] = j;
+                dq[m] = sumCols[j] + sumRows[j];
+                ++m;
+            }
+            const int T = (pool && m > 192) ? pool->nt : 1;
+            std::vector<double> ctmin(T, tmin);
+            std::vector<int> cmi(T, 0), cmj(T, 0), cut(T + 1, m);
+            cut[0] = 1;
+            if (T > 1) {
+                const double W = 0.5 * (double)m * (double)(m - 1);
+                double acc = 0.0;
+                int t = 1;
+                for (int q = 1; q < m && t < T; ++q) {
+                    acc += q;
+                    if (acc >= W * t / T) cut[t++] = q + 1;
+                }
+                for (; t < T; ++t) cut[t] = m;
+            }
+            auto scan = [&](int t) {
+                double tm = ctmin[t];
+                int bi = 0, bj = 0;
+                for (int q = cut[t]; q < cut[t + 1]; ++q) {
+                    const int jj = lv[q];
+                    const double djq = dq[q];
+                    const double* col = sym ? &d[(size_t)jj * (n + 1)] : nullptr;
+                    for (int r = 0; r < q; ++r) {
+                        const int ii = lv[r];
+                        const double dij = sym ? col[ii] : D(ii, jj);
+                        const double total = fnseqs2 * dij + 2.0 * sumd - dq[r] - djq;
+                        if (total < tm) {
+                            tm = total;
+                            bi = ii;
+                            bj = jj;
+                        }
                     }
                 }
-            }
+                ctmin[t] = tm;
+                cmi[t] = bi;
+                cmj[t] = bj;
+            };
+            if (T > 1) pool->run(scan);
+            else scan(0);
+            for (int t = 0; t < T; ++t)
+                if (ctmin[t] < tmin) {
+                    tmin = ctmin[t];
+                    mini = cmi[t];
+                    minj = cmj[t];
+                }
             double dio = sumCols[mini] + sumRows[mini];
             double djo = sumCols[minj] + sumRows[minj];
             double dmin = D(mini, minj);

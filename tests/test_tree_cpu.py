@@ -22,6 +22,23 @@ def test_nj_tree_reproduces_reference_dnd(setname):
     assert cwhip.nj_tree(O.distance_matrix(n, pairs, dist), names) == dnd
 
 
+@pytest.mark.parametrize("n,levels", [(300, 0), (700, 12), (1000, 40)])
+def test_nj_tree_large_matches_oracle(n, levels):
+    """Sizes where the minimum search runs on several host threads: the tree text
+    must equal the oracle's single-threaded restatement, also when distances are
+    quantised to a few values so that the scan order decides many ties."""
+    rng = np.random.default_rng(n)
+    x = rng.random((n, 6))
+    d = np.zeros((n + 1, n + 1))
+    d[1:, 1:] = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1))
+    if levels:
+        d = np.round(d * levels) / levels
+    d = d.astype(np.float32).astype(np.float64)
+    np.fill_diagonal(d, 0.0)
+    names = ["s%d" % i for i in range(n)]
+    assert cwhip.nj_tree(d, names) == O.nj_dnd(d, names)
+
+
 def test_nj_tree_two_and_three_sequences():
     for n in (2, 3):
         names = ["s%d" % i for i in range(n)]
