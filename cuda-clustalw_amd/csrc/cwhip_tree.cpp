@@ -17,6 +17,7 @@
 #include <functional>
 #include <memory>
 #include <thread>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -419,7 +420,10 @@ struct TreeReader {
         set_info(ptree, parent, type, name, dist);
     }
 
-    float calc_mean(int nptr, float* maxDist, int nSeqs)   // Tree::calcMean
+    // Tree::calcMean.  `pos` (one entry per node, all -1 on entry and on exit)
+    // gives a node's position on nptr's root path: a path holds each node once,
+    // so it finds the position the reference's linear search finds.
+    float calc_mean(int nptr, float* maxDist, int nSeqs, std::vector<int>& pos) const
     {
         float dist = 0.0f, leftSum = 0.0f, rightSum = 0.0f;
         std::vector<int> path;
@@ -427,6 +431,7 @@ struct TreeReader {
         path.reserve(nSeqs);
         *maxDist = 0.0f;
         for (int p = nptr; p >= 0; p = nodes[p].parent) {
+            pos[p] = (int)path.size();
             path.push_back(p);
             dist += nodes[p].dist;
             distToNode.push_back(dist);
@@ -443,8 +448,8 @@ struct TreeReader {
                 dist = 0.0f;
                 bool found = false;
                 while (!found && nodes[p].parent >= 0) {
-                    for (size_t j = 0; j < path.size(); ++j)
-                        if (nodes[p].parent == path[j]) { found = true; n = (int)j; }
+                    const int k = pos[nodes[p].parent];
+                    if (k >= 0) { found = true; n = k; }
                     dist += nodes[p].dist;
                     p = nodes[p].parent;
                 }
@@ -460,6 +465,7 @@ struct TreeReader {
             }
             if (dist > *maxDist) *maxDist = dist;
         }
+        for (int p : path) pos[p] = -1;
         const float leftMean = leftSum / numLeft;
         const float rightMean = rightSum / numRight;
         return leftMean - rightMean;
@@ -532,10 +538,26 @@ struct TreeReader {
         int rootPtr = ptree;
         float minDiff = 0.0f, minDepth = 1.0f, maxDist = 0.0f;
         bool first = true;
-        for (size_t i = 0; i < ptrs.size(); ++i) {
+        // every node's (diff, maxDist) on host threads, then the reference's
+        // selection in node order
+        const size_t np = ptrs.size();
+        std::vector<float> diffs(np, 0.0f), maxd(np, 0.0f);
+        std::atomic<size_t> nexti{0};
+        auto work = [&]() {
+            std::vector<int> pos(nodes.size(), -1);
+            for (size_t i; (i = nexti.fetch_add(1)) < np;)
+                if (nodes[ptrs[i]].parent >= 0) diffs[i] = calc_mean(ptrs[i], &maxd[i], nseqs, pos);
+        };
+        const int nt = np > 512 ? (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency())) : 1;
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+        for (size_t i = 0; i < np; ++i) {
             const int p = ptrs[i];
             if (nodes[p].parent < 0) continue;
-            const float diff = calc_mean(p, &maxDist, nseqs);
+            const float diff = diffs[i];
+            maxDist = maxd[i];
             if (diff == 0 || (diff > 0 && diff < 2 * nodes[p].dist)) {
                 if (maxDist < minDepth || first) {
                     first = false;
@@ -632,17 +654,16 @@ int cwhip_guide_from_tree(int nseq, const char* dnd, const char* const* names, i
     const int root = T.rooted ? seqTree : T.re_root(seqTree, nseq + 1);
     for (int leafp : T.lptr)                                 // orderNodes
         for (int p = leafp; p >= 0; p = T.nodes[p].parent) T.nodes[p].order++;
+    // a sequence's leaf is the last leaf (in tree order) whose folded name is
+    // its folded name
     std::vector<int> olptr(nseq, -1);
+    std::unordered_map<std::string, int> leaf_of;
+    for (int leafp : T.lptr) leaf_of[fold_name(T.nodes[leafp].name, false)] = leafp;
     for (int i = 0; i < nseq; ++i) {
         if (!names[i]) return CWHIP_E_ARG;
-        const std::string want = fold_name(names[i], true);
-        bool found = false;
-        for (int leafp : T.lptr)
-            if (fold_name(T.nodes[leafp].name, false) == want) {
-                olptr[i] = leafp;
-                found = true;
-            }
-        if (!found) return CWHIP_E_ARG;
+        const auto it = leaf_of.find(fold_name(names[i], true));
+        if (it == leaf_of.end()) return CWHIP_E_ARG;
+        olptr[i] = it->second;
     }
     // calcSeqWeights (Tree.cpp:22-100)
     if (T.distance_tree) {
@@ -667,39 +688,54 @@ int cwhip_guide_from_tree(int nseq, const char* dnd, const char* const* names, i
     } else {
         for (int i = 0; i < nseq; ++i) weights[i] = INT_SCALE / nseq;
     }
-    // calcSimilarities (Tree.cpp:327-469)
+    // calcSimilarities (Tree.cpp:327-469).  Row i walks each j < i up to the
+    // first ancestor on i's root path; a node's position on that path comes
+    // from a table instead of the reference's linear search (a path holds each
+    // node once, so both find the same position), the float sums keep their
+    // order.  Rows run on host threads; each writes only its own pairs.
     {
         const size_t W = (size_t)nseq + 1;
-        std::vector<int> path;
-        std::vector<float> distToNode;
-        for (int i = 0; i < nseq; ++i) {
-            path.clear();
-            distToNode.clear();
-            float dist = 0.0f;
-            for (int p = olptr[i]; p >= 0; p = T.nodes[p].parent) {
-                path.push_back(p);
-                dist += T.nodes[p].dist;
-                distToNode.push_back(dist);
-            }
-            sim[(i + 1) * W + (i + 1)] = 0.0;
-            for (int j = 0; j < i; ++j) {
-                int p = olptr[j], n = 0;
-                bool found = false;
-                dist = 0.0f;
-                while (!found && T.nodes[p].parent >= 0) {
-                    for (size_t k = 0; k < path.size(); ++k)
-                        if (T.nodes[p].parent == path[k]) { found = true; n = (int)k; }
+        const int nn = (int)T.nodes.size();
+        std::atomic<int> nexti{0};
+        auto rows = [&]() {
+            std::vector<int> path, pos(nn, -1);
+            std::vector<float> distToNode;
+            for (int i; (i = nexti.fetch_add(1)) < nseq;) {
+                path.clear();
+                distToNode.clear();
+                float dist = 0.0f;
+                for (int p = olptr[i]; p >= 0; p = T.nodes[p].parent) {
+                    pos[p] = (int)path.size();
+                    path.push_back(p);
                     dist += T.nodes[p].dist;
-                    p = T.nodes[p].parent;
+                    distToNode.push_back(dist);
                 }
-                double dm = dist + distToNode[n - 1];
-                if (dm < 0.01) dm = 0.01;
-                if (dm > 1.0) dm = 1.0;
-                const double value = 100.0 - dm * 100.0;
-                sim[(i + 1) * W + (j + 1)] = value;
-                sim[(j + 1) * W + (i + 1)] = value;
+                sim[(i + 1) * W + (i + 1)] = 0.0;
+                for (int j = 0; j < i; ++j) {
+                    int p = olptr[j], n = 0;
+                    bool found = false;
+                    dist = 0.0f;
+                    while (!found && T.nodes[p].parent >= 0) {
+                        const int k = pos[T.nodes[p].parent];
+                        if (k >= 0) { found = true; n = k; }
+                        dist += T.nodes[p].dist;
+                        p = T.nodes[p].parent;
+                    }
+                    double dm = dist + distToNode[n - 1];
+                    if (dm < 0.01) dm = 0.01;
+                    if (dm > 1.0) dm = 1.0;
+                    const double value = 100.0 - dm * 100.0;
+                    sim[(i + 1) * W + (j + 1)] = value;
+                    sim[(j + 1) * W + (i + 1)] = value;
+                }
+                for (int p : path) pos[p] = -1;
             }
-        }
+        };
+        const int nt = nseq > 256 ? (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency())) : 1;
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(rows);
+        rows();
+        for (auto& t : th) t.join();
         for (size_t k = 0; k < W; ++k) sim[k] = sim[k * W] = 0.0;
     }
     // createSets / groupSeqs (Tree.cpp:273-322, 1120-1226)
