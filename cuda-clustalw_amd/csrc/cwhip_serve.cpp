@@ -119,10 +119,16 @@ int run_job(cwhip_ctx* ctx, Job& j)
     }
     std::vector<const char*> nm(n);
     for (int i = 0; i < n; ++i) nm[i] = j.names[i].c_str();
-    int64_t len = 0;
-    cwhip_nj_tree(n, D.data(), nm.data(), nullptr, 0, &len);
-    j.dnd.assign(len + 1, '\0');
-    rc = cwhip_nj_tree(n, D.data(), nm.data(), &j.dnd[0], len + 1, &len);
+    // one tree build: room for every name and two branch lengths per node,
+    // a second call only if the text did not fit
+    int64_t len = 0, cap = 80 * (int64_t)(n + 2) + 64;
+    for (int i = 0; i < n; ++i) cap += (int64_t)j.names[i].size();
+    j.dnd.assign(cap, '\0');
+    rc = cwhip_nj_tree(n, D.data(), nm.data(), &j.dnd[0], cap, &len);
+    if (rc && len + 1 > cap) {
+        j.dnd.assign(len + 1, '\0');
+        rc = cwhip_nj_tree(n, D.data(), nm.data(), &j.dnd[0], len + 1, &len);
+    }
     if (rc) return rc;
     j.dnd.resize(len);
     std::vector<int32_t> w(n), steps((size_t)std::max(n - 1, 1) * (n + 1));

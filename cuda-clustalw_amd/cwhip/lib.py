@@ -339,10 +339,15 @@ def nj_tree(dist_full, names) -> str:
     d = np.ascontiguousarray(dist_full, dtype=np.float64).reshape(-1)
     arr = _names(names)
     ln = C.c_int64()
-    L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), None, 0, C.byref(ln))
-    buf = C.create_string_buffer(ln.value + 1)
-    rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), ln.value + 1,
-                         C.byref(ln))
+    # one call with room for every name and two branch lengths per node (the
+    # tree is built once); a second call only if the text did not fit
+    cap = sum(len(x.encode()) for x in names) + 80 * (n + 2) + 64
+    buf = C.create_string_buffer(cap)
+    rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), cap, C.byref(ln))
+    if rc and ln.value + 1 > cap:
+        buf = C.create_string_buffer(ln.value + 1)
+        rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), ln.value + 1,
+                             C.byref(ln))
     if rc:
         raise CwhipError("cwhip_nj_tree failed: %d" % rc)
     return buf.value.decode()
