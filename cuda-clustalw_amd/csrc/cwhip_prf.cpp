@@ -14,6 +14,9 @@
 
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstring>
@@ -121,24 +124,85 @@ bool emit_script(const int32_t* nodes, int nnodes, Emitter& em, double* cells)
     return true;
 }
 
-// fn(0..n-1) on up to 16 host threads (the caller's thread included)
+// Host threads shared by every context of the process: up to 15 workers that
+// sleep between jobs, so a parallel loop costs a wake-up, not thread start-ups.
+// One loop runs on the pool at a time; a loop started from a pool worker (or
+// while another loop holds the pool) runs on its caller alone.
+class HostPool {
+public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    template <typename F>
+    void run(size_t n, F& fn) {
+        std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
+        if (!busy.owns_lock() || in_worker_ || workers_.empty() || n <= 1) {
+            for (size_t k = 0; k < n; ++k) fn(k);
+            return;
+        }
+        std::atomic<size_t> next{0};
+        std::function<void()> body = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < n;) fn(k);
+        };
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &body;
+            pending_ = (int)workers_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        body();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+private:
+    HostPool() {
+        const int hw = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+        for (int t = 1; t < hw; ++t)
+            workers_.emplace_back([this]() {
+                in_worker_ = true;
+                uint64_t seen = 0;
+                for (;;) {
+                    std::function<void()>* job;
+                    {
+                        std::unique_lock<std::mutex> lk(mu_);
+                        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                        if (stop_) return;
+                        seen = gen_;
+                        job = job_;
+                    }
+                    (*job)();
+                    std::lock_guard<std::mutex> lk(mu_);
+                    if (--pending_ == 0) done_cv_.notify_one();
+                }
+            });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_, run_mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void()>* job_ = nullptr;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    bool stop_ = false;
+    static thread_local bool in_worker_;
+};
+thread_local bool HostPool::in_worker_ = false;
+
+// fn(0..n-1) on the host pool (the caller's thread included)
 template <typename F>
 void parallel_for(size_t n, F fn)
 {
-    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
-    const size_t nt = std::min(hw, n);
-    if (nt <= 1) {
-        for (size_t k = 0; k < n; ++k) fn(k);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    auto body = [&]() {
-        for (size_t k; (k = next.fetch_add(1)) < n;) fn(k);
-    };
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; ++t) th.emplace_back(body);
-    body();
-    for (auto& t : th) t.join();
+    HostPool::get().run(n, fn);
 }
 
 // One device batch of profile pairs: inputs packed (on host threads) into
@@ -931,11 +995,15 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         std::vector<Step> sts(nst);
         std::vector<char> ok(nst, 0);
         // the steps of a level touch disjoint sequences: build their profiles
-        // (and later apply their edit scripts) on several host threads
-        parallel_for(nst, [&](size_t k) {
+        // (and later apply their edit scripts) on several host threads; a level
+        // of few steps builds them in turn, each on all threads
+        auto prep = [&](size_t k) {
             std::vector<int> grp = groups[by_level[lv][k]];
             ok[k] = prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k], nst < 4) ? 1 : 0;
-        });
+        };
+        if (nst >= 4) parallel_for(nst, prep);
+        else
+            for (size_t k = 0; k < nst; ++k) prep(k);
         std::vector<DpJob> jobs;
         std::vector<int> live;
         for (size_t k = 0; k < nst; ++k)
