@@ -128,13 +128,38 @@ __device__ __forceinline__ int left_h(const PSweep& W, int i) {
 
 template <bool V> struct Tag { static constexpr bool value = V; };
 
+// Two waves sharing one half sweep: stripe j reads the bottom row stripe j-1
+// writes, 64 columns at a time.  `pub` (LDS) counts the chunks a stripe has
+// flushed to memory; the next stripe waits on it (`wait`) before it loads a
+// chunk.  A wait that outlasts ~2^24 sleeps sets *err and goes on, so a
+// mistake shows up as a reported error, not a hang.
+struct StripeSync {
+    const int* wait;
+    int* pub;
+    int* err;
+};
+
+__device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
+{
+    int spins = 0;
+    while (uni(*(const volatile int*)sy.wait) < need) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1 << 24)) {
+            *sy.err = 1;                                 // same value from every lane
+            break;
+        }
+    }
+    wave_fence();
+}
+
 // One stripe: local rows row0+1 .. row0+P, R rows per lane (lanes >= k1 hold
 // R-1 rows behind a pass-through row).  The top boundary row comes from inH
 // (inE, or -inf on the first stripe); lane 63's rows leave through outH/outE.
 // DIRECT: R == 1, P < 64, lane P-1 writes its row straight to out.
 template <int R, int MODE, bool DIRECT>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
-                                        const int* inH, const int* inE, int* outH, int* outE)
+                                        const int* inH, const int* inE, int* outH, int* outE,
+                                        StripeSync sy)
 {
     const int lane = lane_id();
     const int ncols = W.ncols;
@@ -166,6 +191,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 
     // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
     auto chunk = [&](int ci, int& h, int& e) {
+        if (!DIRECT && sy.wait) wait_chunks(sy, ci + 1);
         const int col = 64 * ci + 1 + lane;
         if (col <= ncols) {
             h = inH[col];
@@ -265,6 +291,10 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                     outH[col] = coutH;
                     outE[col] = coutE;
                 }
+                if (sy.pub) {
+                    wave_fence();
+                    *(volatile int*)sy.pub = (oc + 63) >> 6;    // same value from every lane
+                }
             }
         }
     };
@@ -285,25 +315,26 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 template <int R, int MODE>
 __device__ __attribute__((noinline)) void pstripe_call(const PSweep W, int row0, int P, int first,
                                                        const int* inH, const int* inE, int* outH,
-                                                       int* outE)
+                                                       int* outE, StripeSync sy)
 {
-    pstripe<R, MODE, false>(W, row0, P, first, inH, inE, outH, outE);
+    pstripe<R, MODE, false>(W, row0, P, first, inH, inE, outH, outE, sy);
 }
 
 template <int MODE>
 __device__ void run_pstripe(int R, bool direct, const PSweep& W, int row0, int P, int first,
-                            const int* inH, const int* inE, int* outH, int* outE)
+                            const int* inH, const int* inE, int* outH, int* outE,
+                            StripeSync sy = StripeSync{nullptr, nullptr, nullptr})
 {
-    if (direct) { pstripe<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE); return; }
+    if (direct) { pstripe<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE, sy); return; }
     switch (R) {
-    case 1: pstripe<1, MODE, false>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 2: pstripe_call<2, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 3: pstripe_call<3, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 4: pstripe_call<4, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 5: pstripe_call<5, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 6: pstripe_call<6, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    case 7: pstripe_call<7, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
-    default: pstripe_call<8, MODE>(W, row0, P, first, inH, inE, outH, outE); break;
+    case 1: pstripe<1, MODE, false>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 2: pstripe_call<2, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 3: pstripe_call<3, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 4: pstripe_call<4, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 5: pstripe_call<5, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 6: pstripe_call<6, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 7: pstripe_call<7, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    default: pstripe_call<8, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
     }
 }
 
@@ -330,6 +361,57 @@ __device__ void psweep(const PSweep& W, int rmax, const Bufs& g, int* finH, int*
         int* oE = last ? finE : ((s & 1) ? g.e0 : g.e1);
         run_pstripe<MODE>(R, direct, W, row0, P, s == 0, inH, inE, oH, oE);
         wave_fence();
+        row0 += P;
+    }
+}
+
+// Member m of an S-wave group runs stripes m, m+S, ... of a half sweep (at
+// least S stripes, each of >= 64 rows).  Member regions (CWP_SCR_ROWS rows of
+// rowcap each): rows 0/1 the top row (member 0, written by fill_top), rows
+// 2/3 and 4/5 two output slots used in turn; stripe j of member j%S writes
+// slot (j/S)&1 and stripe j+1 reads it.  A member starts stripe j+2S only
+// after its stripe j+S, which runs behind stripe j+1, so a slot is never
+// overwritten before it has been read.  prog[j] counts stripe j's flushed
+// chunks.
+constexpr int kMaxSplitStripes = 64;
+
+template <int MODE>
+__device__ void psweep_part(const PSweep& W, int rmax, int m, int S, int* const* regions, int rowcap,
+                            int* finH, int* finE, int* prog, int* err)
+{
+    const int nrows = W.nrows;
+    const int per = WAVE * rmax;
+    int nst = uni((nrows + per - 1) / per);
+    if (nst < S) nst = S;
+    const int base = nrows / nst, rem = nrows % nst;
+    int row0 = 0;
+    for (int j = 0; j < nst; ++j) {
+        const int P = uni(base + (j < rem ? 1 : 0));
+        if (j % S == m) {
+            const int R = uni((P + WAVE - 1) / WAVE);
+            const int* inH;
+            const int* inE;
+            if (j == 0) {
+                inH = regions[0];
+                inE = regions[0] + rowcap;
+            } else {
+                const int slot = ((j - 1) / S) & 1;
+                inH = regions[(j - 1) % S] + (2 + 2 * slot) * rowcap;
+                inE = inH + rowcap;
+            }
+            int* oH;
+            int* oE;
+            if (j == nst - 1) {
+                oH = finH;
+                oE = finE;
+            } else {
+                oH = regions[m] + (2 + 2 * ((j / S) & 1)) * rowcap;
+                oE = oH + rowcap;
+            }
+            const StripeSync sy{j > 0 ? prog + (j - 1) : nullptr, j < nst - 1 ? prog + j : nullptr, err};
+            run_pstripe<MODE>(R, false, W, row0, P, j == 0, inH, inE, oH, oE, sy);
+            wave_fence();
+        }
         row0 += P;
     }
 }
@@ -410,7 +492,8 @@ __device__ __attribute__((noinline)) void do_leaf(const Prob& p, int* nd, int is
 // Forward (half 0, rows 1..midi) or reverse (half 1, rows M..midi+1) sweep of
 // node nd (M >= 2, N >= 1) into the level buffers.
 __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, int half, int rmax,
-                                                  int* scr, int rowcap, LevelBufs lv)
+                                                  int* scr, int rowcap, LevelBufs lv, int m, int S,
+                                                  int* const* regions, int* prog, int* err)
 {
     const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
     const int flags = uni(nd[4]);
@@ -425,8 +508,13 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
         F.ltl = -ext2(p, A + 1, B);
         F.go2 = 0; F.jedge = 0;
-        fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
-        psweep<MODE_FWD>(F, rmax, g, lv.fH, lv.fE);
+        if (S > 1) {
+            if (m == 0) fill_top<MODE_FWD>(p, A, B, M, N, regions[0]);
+            psweep_part<MODE_FWD>(F, rmax, m, S, regions, rowcap, lv.fH, lv.fE, prog, err);
+        } else {
+            fill_top<MODE_FWD>(p, A, B, M, N, g.h0);
+            psweep<MODE_FWD>(F, rmax, g, lv.fH, lv.fE);
+        }
     } else {
         PSweep Rv;
         Rv.p = p;
@@ -434,8 +522,13 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
         Rv.lt0 = 0; Rv.ltl = 0;
         Rv.go2 = go2; Rv.jedge = B + N;
-        fill_top<MODE_REV>(p, A, B, M, N, g.h0);
-        psweep<MODE_REV>(Rv, rmax, g, lv.rH, lv.rK);
+        if (S > 1) {
+            if (m == 0) fill_top<MODE_REV>(p, A, B, M, N, regions[0]);
+            psweep_part<MODE_REV>(Rv, rmax, m, S, regions, rowcap, lv.rH, lv.rK, prog, err);
+        } else {
+            fill_top<MODE_REV>(p, A, B, M, N, g.h0);
+            psweep<MODE_REV>(Rv, rmax, g, lv.rH, lv.rK);
+        }
     }
     wave_fence();
 }
@@ -1137,6 +1230,7 @@ extern "C" __global__ void __launch_bounds__(256) cw_prf_score(PrfArgs a)
 extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 {
     __shared__ int s_lev_b, s_lev_e, s_ctr, s_err, s_alloc;
+    __shared__ int s_prog[2 * kMaxSplitStripes];
     __shared__ __align__(16) int s_local[CWP_WAVES * CWP_LOCAL_INTS];
     const int prob = blockIdx.x;
     const PrfProb pp = a.probs[prob];
@@ -1163,26 +1257,47 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         const int lb = s_lev_b, le = s_lev_e;
         if (lb >= le) break;
         __syncthreads();
+        for (int k = threadIdx.x; k < 2 * kMaxSplitStripes; k += blockDim.x) s_prog[k] = 0;
         if (threadIdx.x == 0) s_ctr = 0;
         __syncthreads();
-        // tasks 2k, 2k+1: the two halves of node lb+k (a local subtree or a leaf
-        // is one task)
-        const int ntask = 2 * (le - lb);
+        // A level of one large node gives each half to S = 2 waves (pipelined
+        // stripes), so all four waves work on it.
+        int S = 1;
+        if (le - lb == 1) {
+            const int* nd0 = nodes + (int64_t)lb * CWP_NODE_INTS;
+            const int M0 = uni(nd0[2]), N0 = uni(nd0[3]);
+            const int per = WAVE * a.rmax;
+            const int half_rows = M0 - M0 / 2;
+            // every stripe keeps >= 64 rows (a stripe of one row per lane must
+            // fill the wave): both halves have >= 128 rows and a stripe holds
+            // up to 128 or more
+            if (!local_fits(M0, N0) && N0 >= 1 && M0 / 2 >= 2 * WAVE && a.rmax >= 2 &&
+                (half_rows + per - 1) / per <= kMaxSplitStripes)
+                S = 2;
+        }
+        // tasks 2S k .. 2S k + 2S - 1: the halves of node lb+k, S members each
+        // (a local subtree or a leaf is one task)
+        const int ntask = 2 * S * (le - lb);
         for (;;) {
             const int got = atomicAdd(&s_ctr, lane == 0 ? 1 : 0);
             const int t = uni(__shfl(got, 0, WAVE));
             if (t >= ntask) break;
-            const int idx = lb + (t >> 1), half = t & 1;
+            const int idx = lb + t / (2 * S), within = t % (2 * S);
+            const int half = within / S, m = within % S;
             int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
             const int M = uni(nd[2]), N = uni(nd[3]);
             if (local_fits(M, N)) {
-                if (half == 0)
+                if (within == 0)
                     local_subtree(p, nodes, idx, s_local + wave * CWP_LOCAL_INTS, &s_alloc, cap, &s_err,
                                   a.out + 4 * prob);
             } else if (N <= 0 || M <= 1) {
-                if (half == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
+                if (within == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
             } else {
-                do_half(p, nd, half, a.rmax, scr, pp.rowcap, level_bufs(lvl, lcap, uni(nd[1]), idx - lb));
+                int* regions[2];
+                for (int r = 0; r < 2; ++r)
+                    regions[r] = a.pool + pp.scratch + (int64_t)(2 * half + r) * CWP_SCR_ROWS * pp.rowcap;
+                do_half(p, nd, half, a.rmax, scr, pp.rowcap, level_bufs(lvl, lcap, uni(nd[1]), idx - lb), m,
+                        S, regions, s_prog + half * kMaxSplitStripes, &s_err);
             }
         }
         __syncthreads();
