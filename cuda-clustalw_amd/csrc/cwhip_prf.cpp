@@ -452,8 +452,29 @@ void column_hist(const std::vector<const uint8_t*>& rows, const std::vector<int>
 // ProfileBase::calcGapCoeff (ProfileBase.cpp:52-298) and its helpers
 // (:310-549).  `rows` are the step's padded rows; `gaps` is the per-column
 // count the caller shares with calcProfileWithSub.
+// sum over rows first..last-1 of per-row contributions to an L-long integer
+// array: row chunks on the host pool (when `par`), each with its own array,
+// added up at the end (integer sums: the order does not matter)
+template <typename RowFn>
+void rows_sum(int first, int last, int L, bool par, std::vector<int>& out, RowFn row_fn)
+{
+    const int n = last - first;
+    const int nch = (par && (int64_t)n * L > (1 << 17)) ? std::min(16, std::max(1, n / 16)) : 1;
+    if (nch == 1) {
+        for (int k = first; k < last; ++k) row_fn(k, out.data());
+        return;
+    }
+    std::vector<std::vector<int>> acc(nch, std::vector<int>(L, 0));
+    parallel_for(nch, [&](size_t c) {
+        const int a = first + (int)((int64_t)n * c / nch), b = first + (int)((int64_t)n * (c + 1) / nch);
+        for (int k = a; k < b; ++k) row_fn(k, acc[c].data());
+    });
+    for (int c = 0; c < nch; ++c)
+        for (int j = 0; j < L; ++j) out[j] += acc[c][j];
+}
+
 void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector<const uint8_t*>& rows,
-                    const ColHist& hist, std::vector<int>& gaps, int gapCoef, int lenCoef)
+                    const ColHist& hist, std::vector<int>& gaps, int gapCoef, int lenCoef, bool par)
 {
     const int L = s.L, first = s.first, last = s.last, numSeq = last - first;
     int pcid = 0;
@@ -479,7 +500,7 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
         s.nPrefPen = p.no_pref_penalties != 0;
         s.gdist = p.gap_dist;
     }
-    for (int i = first; i < last; ++i) {
+    rows_sum(first, last, L, par, gaps, [&](int i, int* acc) {
         const uint8_t* r = rows[i];
         int is = 0, ie = L;
         if (!p.use_end_gaps && !p.end_gap_penalties) {
@@ -490,8 +511,8 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
                 ie = 0;
         }
         for (int j = is; j < ie; ++j)
-            if (is_gap_code(r[j])) gaps[j]++;
-    }
+            if (is_gap_code(r[j])) acc[j]++;
+    });
     std::vector<int> vW, resW, hydW;
     if (!p.dna && !s.nVarPen) {                          // calcVPenalties (:310-353)
         vW.assign(L + 2, 0);
@@ -532,7 +553,7 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
             if (c >= 0 && c <= MAXAA) hyd[c] = true;
         }
         hydW.assign(L + 2, 0);
-        for (int k = first; k < last; ++k) {
+        rows_sum(first, last, L, par, hydW, [&](int k, int* acc) {
             const uint8_t* r = rows[k];
             int i = 0;
             while (i < L) {
@@ -540,9 +561,9 @@ void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector
                 const int st = i;
                 while (i < L && !is_gap_code(r[i]) && hyd[r[i]]) ++i;
                 if (i - st > 3)
-                    for (int j = st; j < i; ++j) hydW[j] += 100;
+                    for (int j = st; j < i; ++j) acc[j] += 100;
             }
-        }
+        });
         const float scale = (float)numSeq;
         for (int i = 0; i < L; ++i) hydW[i] = (int)(hydW[i] / scale);
     }
@@ -785,10 +806,10 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim
     st.p2.prf.assign((size_t)(L2 + 2) * W, 0);
     ColHist hist;
     column_hist(st.rows, w, 0, n1, L1, par, hist);
-    calc_gap_coeff(p, st.p1, st.rows, hist, gaps, gapcoef, lencoef);
+    calc_gap_coeff(p, st.p1, st.rows, hist, gaps, gapcoef, lencoef, par);
     calc_profile_with_sub(st.p1, hist, gaps, mat, w, par);
     column_hist(st.rows, w, n1, n1 + n2, L2, par, hist);
-    calc_gap_coeff(p, st.p2, st.rows, hist, gaps, gapcoef, lencoef);
+    calc_gap_coeff(p, st.p2, st.rows, hist, gaps, gapcoef, lencoef, par);
     calc_standard_profile(st.p2, hist, w);
     st.displ.assign(L1 + L2 + 4, 0);
     st.job.prf1 = st.p1.prf.data();
