@@ -711,11 +711,11 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
     }
     // left boundary H of half-row i (lane L == 0 only)
     auto left = [&](int i) -> int {
-        if (i <= 0) return 0;
-        if (!grp) return lt0 + i * ltl;
-        const int I = A + M + 1 - i;
+        const int ic = i < 1 ? 1 : (i > M ? M : i);
+        const int I = A + M + 1 - ic;                // a valid row for either half
         const int g = go2 ? -(ce.z & (q.row(I).x + ce.x)) : 0;
-        return g - (ce.z & (p1e - q.pl1(I)));
+        const int rv = g - (ce.z & (p1e - q.pl1(I)));
+        return i <= 0 ? 0 : (grp ? rv : lt0 + i * ltl);
     };
     int dleft;
     if (L == 0) {
@@ -735,7 +735,7 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
     auto fetch = [&](int i) {
         const int I = rowI(i);
         nre = q.row(I);
-        nrf = grp ? q.row(I - 1) : nre;
+        nrf = q.row(grp ? I - 1 : I);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const int jj = L * C + c + 1;
@@ -763,22 +763,21 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
         }
         if (i >= 1 && i <= rows && L < nl) {
             int d = dleft, hc = hl, fc = fl;
+            // one recurrence for both halves (the gap masks are 0 or all ones, so
+            // (x + y) & m = (x & m) + (y & m) and max(a - o - e, b - e) = max(a - o, b) - e):
+            //   forward: F = max(H - o1, F) - e1 enters H as F
+            //   reverse: G = max(H, G) - e1     enters H as G - o1   (likewise E / K)
+            // fm selects where the opening penalty goes
+            const int fm = grp ? 0 : -1;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                int h;
-                if (!grp) {
-                    const int fn = max(hc - ((rf.x + cF[c].w) & rf.z), fc - (cF[c].y & rf.z));
-                    const int en = max(hup[c] - ((re.w + cE[c].x) & cE[c].z), vst[c] - (re.y & cE[c].z));
-                    h = max3(d + s[c], fn, en);
-                    vst[c] = en;
-                    fc = fn;
-                } else {
-                    const int g = max(hc, fc) - (cF[c].y & rf.z);
-                    const int k = max(hup[c], vst[c]) - (re.y & cE[c].z);
-                    h = max3(d + s[c], g - ((rf.x + cF[c].x) & rf.z), k - ((re.x + cE[c].x) & cE[c].z));
-                    vst[c] = k;
-                    fc = g;
-                }
+                const int o1 = (rf.x + cF[c].x) & rf.z, e1 = cF[c].y & rf.z;
+                const int o2 = (re.x + cE[c].x) & cE[c].z, e2 = re.y & cE[c].z;
+                const int st1 = max(hc - (o1 & fm), fc) - e1;
+                const int st2 = max(hup[c] - (o2 & fm), vst[c]) - e2;
+                const int h = max3(d + s[c], st1 - (o1 & ~fm), st2 - (o2 & ~fm));
+                vst[c] = st2;
+                fc = st1;
                 d = hup[c];
                 hup[c] = h;
                 hc = h;
