@@ -43,12 +43,13 @@ def ref_cells(lens, pairs, info):
     return n * m, rev, info[:, 7].astype(np.int64)
 
 
-def cpu_baseline(codes, offs, lens, pairs, seconds):
+def cpu_baseline(codes, offs, lens, pairs, seconds, threads=None):
     """The oracle (C restatement of the reference) on a bounded pair sample."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O  # checker/baseline only
     sc = O.Scoring()
-    threads = min(16, os.cpu_count() or 1)
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
     # calibrate with a small sample, then size the sample to ~`seconds`
     stride = max(1, len(pairs) // 64)
     probe = pairs[::stride][:64]
@@ -67,6 +68,44 @@ def cpu_baseline(codes, offs, lens, pairs, seconds):
             "kind": "port",
             "sample": "every %dth pair of the workload (%d pairs, %.1f s, %d OpenMP threads)"
                       % (stride, len(sample), dt, threads)}
+
+
+def small_stage(cpu: bool):
+    """The 100 x ~97 workload (4,950 pairs) that the reference's CPU code runs in
+    seconds: batched and plain GCUPS and cells/s, and the oracle on all pairs
+    with one thread and with all cores."""
+    import cwhip
+    from cwhip import seqgen
+    fam = seqgen.workload("small97")
+    codes, offs, lens = fam.packed()
+    n = len(lens)
+    iu = np.triu_indices(n, 1)
+    pairs = np.stack([iu[0], iu[1]], axis=1).astype(np.int32)
+    conv = int((lens[pairs[:, 0]].astype(np.int64) * lens[pairs[:, 1]]).sum())
+    ctx = cwhip.Context(0)
+    ctx.load_sequences(codes, offs, lens)
+    ctx.stage(cwhip.pair_params(max_len=int(lens.max()) + 1), pairs)
+    out = {"workload": "small97", "sequences": n, "pairs": int(len(pairs)), "conventional_cells": conv}
+    for name, k in (("batched", cwhip.KERNEL_BATCHED), ("plain", cwhip.KERNEL_PLAIN)):
+        ctx.run_staged(kernel=k)
+        ms, kms = ctx.run_staged(kernel=k, reps=5)
+        if name == "batched":
+            info = ctx.fetch_info()
+            fwd, rev, mm = ref_cells(lens, pairs, info)
+            upd = int(fwd.sum() + rev.sum() + mm.sum())
+            out["reference_cell_updates"] = upd
+            out["roofline"] = {"bound": "valu", "achieved": round(upd * OPS_PER_CELL / (kms * 1e-3) / 1e12, 3),
+                               "peak": PEAK_INT32_TOPS, "unit": "Tops/s (int32)",
+                               "frac": round(upd * OPS_PER_CELL / (kms * 1e-3) / 1e12 / PEAK_INT32_TOPS, 4)}
+            out[name] = {"gcups": round(conv / (ms * 1e-3) / 1e9, 3), "ms": round(ms, 3),
+                         "cells_per_s": round(upd / (ms * 1e-3), 1)}
+        else:
+            out[name] = {"gcups": round(conv / (ms * 1e-3) / 1e9, 3), "ms": round(ms, 3)}
+    ctx.close()
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(codes, offs, lens, pairs, 60.0)
+        out["cpu_baseline_1core"] = cpu_baseline(codes, offs, lens, pairs, 60.0, threads=1)
+    return out
 
 
 def batching_stage(cpu: bool, cpu_seconds: float):
@@ -267,6 +306,7 @@ def main():
     ap.add_argument("--no-progressive", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--stage", choices=["all", "batching", "progressive", "serving"], default="all",
                     help="progressive / serving: run only that part and print its object")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: requests per second")
@@ -365,6 +405,9 @@ def main():
         out["batched_over_plain"] = round(ms_p / ms_step, 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(codes, offs, lens, pairs, args.cpu_seconds)
+        out["cpu_baseline_1core"] = cpu_baseline(codes, offs, lens, pairs, 5.0, threads=1)
+    if not args.no_small:
+        out["small"] = small_stage(not args.no_cpu)
     if not args.no_batching:
         out["batching"] = batching_stage(not args.no_cpu, 5.0)
     if not args.no_progressive:
