@@ -215,7 +215,7 @@ def progressive_stage(cpu: bool):
     return out
 
 
-def serving_stage(rate: float, count: int, seed: int = 20260105):
+def serving_stage(rate: float, count: int, seed: int = 20260105, slots: int = 4):
     """Poisson arrivals at `rate` requests/s of `count` serving requests (N ~ U[10,500],
     L ~ lognormal(ln 300, 0.7) clipped [30, 2000]; 80% distances, 20% distances +
     progressive alignment) into the library's request loop (4 slots, admission by
@@ -228,7 +228,7 @@ def serving_stage(rate: float, count: int, seed: int = 20260105):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=count)
     arrive = np.cumsum(gaps) - gaps[0]
-    srv = cwhip.Server(0, slots=4, hbm_budget=128 << 30)
+    srv = cwhip.Server(0, slots=slots, hbm_budget=128 << 30)
     jobs = []
     t0 = time.perf_counter()
     for (kind, fam), ta in zip(reqs, arrive):

@@ -306,19 +306,23 @@ static void fill_args(cwhip_ctx* ctx, DevArgs& a)
     }
 }
 
-// The batched stages as one graph launch; the graph is captured again whenever
-// the launch it was captured for changes (buffers, sizes, shapes).
+// The batched stages as one graph launch.  A launch runs directly the first
+// time it is seen and is captured when the same launch (buffers, sizes, shapes)
+// comes again, so one-off jobs (a server's stream of different jobs) pay no
+// capture and repeated runs of one staged job replay the graph.
 static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int threads, size_t lds)
 {
-    if (!ctx->use_graph) {
-        HIPCHK(cw_launch_batched(blocks, threads, lds, ctx->stream, a));
-        return CWHIP_OK;
-    }
     std::vector<uint8_t> key(sizeof(DevArgs) + 3 * sizeof(int64_t));
     std::memcpy(key.data(), &a, sizeof(DevArgs));
     const int64_t dims[3] = {blocks, threads, (int64_t)lds};
     std::memcpy(key.data() + sizeof(DevArgs), dims, sizeof dims);
-    if (!ctx->gexec || key != ctx->gkey) {
+    const bool replay = ctx->gexec && key == ctx->gkey;
+    if (!ctx->use_graph || (!replay && key != ctx->glast)) {
+        ctx->glast = key;
+        HIPCHK(cw_launch_batched(blocks, threads, lds, ctx->stream, a));
+        return CWHIP_OK;
+    }
+    if (!replay) {
         if (ctx->gexec) {
             HIPCHK(hipGraphExecDestroy(ctx->gexec));
             ctx->gexec = nullptr;
@@ -415,7 +419,8 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     if (ms) *ms = t_all;
     if (kernel_ms) *kernel_ms = t_k;
     int32_t st = 0;
-    HIPCHK(hipMemcpy(&st, ctx->d_status.p, sizeof st, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&st, ctx->d_status.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     if (st) return fail(ctx, CWHIP_E_KERNEL, "kernel status " + std::to_string(st));
     return CWHIP_OK;
 }
@@ -550,7 +555,11 @@ int cwhip_fetch_counts(cwhip_ctx* ctx, int32_t* out_count, int64_t npairs)
 {
     if (!ctx || (!out_count && npairs) || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
     HIPCHK(hipSetDevice(ctx->device));
-    if (npairs) HIPCHK(hipMemcpy(out_count, ctx->d_count.p, npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (npairs) {
+        HIPCHK(hipMemcpyAsync(out_count, ctx->d_count.p, npairs * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return CWHIP_OK;
 }
 
@@ -559,7 +568,11 @@ int cwhip_fetch_info(cwhip_ctx* ctx, int32_t* out_info, int64_t npairs)
 {
     if (!ctx || (!out_info && npairs) || npairs != ctx->npairs) return fail(ctx, CWHIP_E_ARG, "bad fetch arguments");
     HIPCHK(hipSetDevice(ctx->device));
-    if (npairs) HIPCHK(hipMemcpy(out_info, ctx->d_info.p, 8 * npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (npairs) {
+        HIPCHK(hipMemcpyAsync(out_info, ctx->d_info.p, 8 * npairs * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return CWHIP_OK;
 }
 

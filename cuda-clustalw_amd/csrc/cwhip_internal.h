@@ -64,10 +64,11 @@ struct cwhip_ctx {
     // tuning
     int rmax = CW_RMAX, wpg = CW_MAX_THREADS / 64, groups_per_cu = 0;
     cwhip_tune_table tune;                        // batched kernel shapes by class
-    // batched stages captured as a graph; gkey holds the launch it was captured for
+    // batched stages captured as a graph; gkey holds the launch it was captured
+    // for, glast the last launch that ran directly
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
-    std::vector<uint8_t> gkey;
+    std::vector<uint8_t> gkey, glast;
     DevBuf<int32_t> d_bins;
     // profile alignment
     DevBuf<int32_t> p_pool, p_out, p_tiles;

@@ -321,8 +321,10 @@ int cwhip_server_text(cwhip_server* srv, int64_t id, int which, char* out, int64
     if (!srv || !len) return CWHIP_E_ARG;
     std::lock_guard<std::mutex> lk(srv->mu);
     Job* j = find(srv, id);
-    if (!j || j->state != 2 || j->kind != CWHIP_JOB_ALIGNMENT) return CWHIP_E_ARG;
-    const std::string& s = which ? j->aln : j->dnd;
+    if (!j || which < 0 || which > 2) return CWHIP_E_ARG;
+    if (which == 2 ? j->state != 3 : (j->state != 2 || j->kind != CWHIP_JOB_ALIGNMENT))
+        return CWHIP_E_ARG;
+    const std::string& s = which == 2 ? j->err : which ? j->aln : j->dnd;
     *len = (int64_t)s.size();
     if (!out || cap < *len + 1) return CWHIP_E_ARG;
     std::memcpy(out, s.c_str(), s.size() + 1);

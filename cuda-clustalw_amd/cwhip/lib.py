@@ -463,7 +463,11 @@ class Server:
         if rc == -7:
             return False
         if rc:
-            raise CwhipError("job %d failed: %d" % (job, rc))
+            try:
+                why = self.text(job, 2)
+            except CwhipError:
+                why = ""
+            raise CwhipError("job %d failed: %d %s" % (job, rc, why))
         return True
 
     def distances(self, job: int) -> np.ndarray:

@@ -155,8 +155,10 @@ int cwhip_set_tune_table(cwhip_ctx *ctx, const cwhip_tune_table *t);
  * time spent.  A pair list staged before the call is staged again afterwards. */
 int cwhip_autotune(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pairs,
                    int64_t npairs, cwhip_tune_table *out, float *tune_ms);
-/* Batched runs replay the captured stages (queue reset, binning, persistent
- * kernel) as one hipGraph; 0 launches them one by one instead. */
+/* A batched launch that repeats the previous one (same staged job, tables and
+ * shape) is captured once (queue reset, binning, persistent kernel) and replayed
+ * as one hipGraph; a launch seen for the first time runs directly.  0 launches
+ * the stages one by one always. */
 int cwhip_set_graph(cwhip_ctx *ctx, int enable);
 
 
@@ -309,7 +311,8 @@ int cwhip_server_submit(cwhip_server *srv, int kind, int dna, int nseq, const ui
 int cwhip_server_wait(cwhip_server *srv, int64_t job, double timeout_s);
 /* nseq*(nseq-1)/2 float distances in the pair order of the range 0..nseq */
 int cwhip_server_distances(cwhip_server *srv, int64_t job, float *dist);
-/* which = 0: the .dnd text, 1: the .aln text (sizing as cwhip_nj_tree) */
+/* which = 0: the .dnd text, 1: the .aln text (sizing as cwhip_nj_tree); 2: for a job that
+ * failed, the error text of the context that ran it */
 int cwhip_server_text(cwhip_server *srv, int64_t job, int which, char *out, int64_t cap,
                       int64_t *len);
 /* [0] submitted, [1] started, [2] finished (seconds since the server was created),

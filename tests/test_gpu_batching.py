@@ -93,13 +93,50 @@ def test_graph_replays_match_eager(job, plain):
         ms, kms = ctx.run_staged(kernel=cwhip.KERNEL_BATCHED, reps=3)
         assert ms > 0
         np.testing.assert_array_equal(ctx.fetch_counts(), plain[0])
-        # a different staged job captures a new graph; the old one must not run
+        # a different staged job runs directly, then is captured as a new graph;
+        # the old graph must not run for it
         sub = pairs[::5]
         ctx.stage(params, sub)
-        ctx.run_staged(kernel=cwhip.KERNEL_BATCHED)
-        np.testing.assert_array_equal(ctx.fetch_counts(), plain[0][::5])
+        for rep in range(3):
+            ctx.run_staged(kernel=cwhip.KERNEL_BATCHED)
+            np.testing.assert_array_equal(ctx.fetch_counts(), plain[0][::5], err_msg="sub %d" % rep)
     finally:
         ctx.close()
+
+
+def test_graph_capture_beside_other_contexts(job, plain):
+    """Contexts used from several host threads at once: while one thread
+    captures its batched stages, the others launch, read back counts and
+    diagnostics and synchronise; no call may fail and every result must equal
+    the plain kernel's."""
+    import threading
+    codes, offs, lens, pairs, params, sample, cnt_o, dist_o = job
+    subs = [np.arange(k, len(pairs), 6) for k in range(6)]
+    errors = []
+
+    def work(k):
+        try:
+            ctx = cwhip.Context(0)
+            try:
+                ctx.load_sequences(codes, offs, lens)
+                ctx.stage(params, pairs[subs[k]])
+                for rep in range(4):
+                    ctx.run_staged(kernel=cwhip.KERNEL_BATCHED)
+                    np.testing.assert_array_equal(ctx.fetch_counts(), plain[0][subs[k]],
+                                                  err_msg="thread %d rep %d" % (k, rep))
+                    ctx.fetch_info()
+            finally:
+                ctx.close()
+        except Exception as e:                      # reported by the main thread
+            errors.append("thread %d: %r" % (k, e))
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(subs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(240)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
 
 
 @pytest.mark.parametrize("rmax,colcap", [(1, 32), (2, 64), (3, 256), (5, 128), (8, 32), (8, 256)])

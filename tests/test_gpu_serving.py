@@ -84,3 +84,36 @@ def test_serving_admission_by_footprint():
             tiny.submit(0, reqs[0][1].seqs, reqs[0][1].names)
     finally:
         tiny.close()
+
+
+def test_serving_all_slots_busy():
+    """The largest server (16 slots) with more jobs submitted at once than it has
+    slots: every job completes and equals the same job run on the direct path."""
+    reqs = []
+    for i in range(40):
+        kind, fam = seqgen.serving_request(17, i, n_range=(10, 60), len_cap=400)
+        reqs.append((kind, fam))
+    srv = cwhip.Server(0, slots=16, hbm_budget=64 << 30)
+    try:
+        jobs = [srv.submit(kind, fam.seqs, fam.names) for kind, fam in reqs]
+        got = []
+        for (kind, fam), j in zip(reqs, jobs):
+            assert srv.wait(j, 300)
+            got.append(_collect(srv, j, kind))
+        assert srv.stats()["max_running"] >= 4
+    finally:
+        srv.close()
+    ctx = cwhip.Context(0)
+    try:
+        for (kind, fam), ref in zip(reqs, got):
+            codes, offs, lens = fam.packed()
+            ctx.load_sequences(codes, offs, lens)
+            params = cwhip.pair_params(max_len=int(lens.max()))
+            pairs, cnt, dist = ctx.pairwise_range(params, 0, len(lens), 0, len(lens),
+                                                  kernel=cwhip.KERNEL_BATCHED)
+            np.testing.assert_array_equal(dist.view(np.uint32), ref[0].view(np.uint32))
+            if kind == cwhip.JOB_ALIGNMENT:
+                dnd, aln, rows, order = pipeline.align(ctx, fam.names, fam.seqs, False)
+                assert dnd == ref[1] and aln == ref[2]
+    finally:
+        ctx.close()
