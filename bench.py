@@ -212,14 +212,15 @@ def progressive_stage(cpu: bool):
                                "sample": "the whole multiSeqAlign stage (999 profile alignments), 1 thread",
                                "identical_rows": same}
         out["speedup_vs_cpu"] = round(dt / el, 2)
+    ctx.close()
     return out
 
 
 def serving_stage(rate: float, count: int, seed: int = 20260105, slots: int = 4):
     """Poisson arrivals at `rate` requests/s of `count` serving requests (N ~ U[10,500],
     L ~ lognormal(ln 300, 0.7) clipped [30, 2000]; 80% distances, 20% distances +
-    progressive alignment) into the library's request loop (4 slots, admission by
-    device footprint).  Latency is submission to completion; throughput counts
+    progressive alignment) into the library's request loop (`slots` device slots,
+    admission by device footprint).  Latency is submission to completion; throughput counts
     pairwise cells and Myers-Miller cell updates over the whole window.  Three
     requests are then re-run alone on the direct path and compared."""
     import cwhip
@@ -294,6 +295,18 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+def serving_all(args):
+    """The serving stream at the stated rate, then the same mix at higher rates."""
+    out = {"serving": serving_stage(args.serve_rate, args.serve_count)}
+    load = [float(r) for r in args.serve_load.split(",") if r.strip()]
+    if load:
+        keys = ("rate_per_s", "latency_p50_s", "latency_p99_s", "throughput_gcups",
+                "max_concurrent_jobs", "spot_check_identical_to_direct_path")
+        out["serving_load"] = [{k: r[k] for k in keys}
+                               for r in (serving_stage(rate, 120) for rate in load)]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +324,9 @@ def main():
                     help="progressive / serving: run only that part and print its object")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: requests per second")
     ap.add_argument("--serve-count", type=int, default=80, help="serving: number of requests")
+    ap.add_argument("--serve-load", default="32,64",
+                    help="serving: further rates (comma-separated, 120 requests each) for the "
+                         "latency/throughput curve toward saturation; empty for none")
     ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (counter calibration)")
@@ -329,7 +345,7 @@ def main():
         print(json.dumps({"batching": batching_stage(not args.no_cpu, 5.0)}), flush=True)
         return
     if args.stage == "serving":
-        print(json.dumps({"serving": serving_stage(args.serve_rate, args.serve_count)}), flush=True)
+        print(json.dumps(serving_all(args)), flush=True)
         return
 
     import cwhip
@@ -403,6 +419,9 @@ def main():
                         "kernel_ms": round(float(np.mean(kern_p)), 3),
                         "identical_to_batched": bool(np.array_equal(cnt_p, cnt_b))}
         out["batched_over_plain"] = round(ms_p / ms_step, 3)
+    # the later stages make their own contexts; a context left open keeps its
+    # stream, and with it one of the process's hardware queues
+    ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(codes, offs, lens, pairs, args.cpu_seconds)
         out["cpu_baseline_1core"] = cpu_baseline(codes, offs, lens, pairs, 5.0, threads=1)
@@ -413,7 +432,7 @@ def main():
     if not args.no_progressive:
         out["progressive"] = progressive_stage(not args.no_cpu)
     if not args.no_serving:
-        out["serving"] = serving_stage(args.serve_rate, args.serve_count)
+        out.update(serving_all(args))
     print(json.dumps(out), flush=True)
 
 
