@@ -36,3 +36,47 @@ struct PrfArgs {
     int32_t nprob;
     int32_t rmax;
 };
+
+// ---- profile construction on the device ------------------------------------
+// One profile of a step (MyersMillerProfileAlign.cpp:202-240): the gap
+// coefficients of ProfileBase::calcGapCoeff and the residue columns of
+// ProfileWithSub::calcProfileWithSub (profile 1) or
+// ProfileStandard::calcStandardProfile (profile 2), written where the DP
+// kernels read them.  Offsets are in int32 units into PrfArgs::pool, except
+// `rows` (bytes into BuildArgs::rows).
+#define CWP_BUILD_COLS 256    // columns per workgroup of cw_prf_cols (one per thread)
+#define CWP_F_SUB 1           // profile with substitution matrix (else standard profile)
+#define CWP_F_DNA 2           // DNA: no residue-specific, hydrophilic or variable penalties
+#define CWP_F_VAR 4           // variable penalties (vW) of a two-row group with pcid > 60
+#define CWP_F_HYD 8           // hydrophilic gap penalties
+#define CWP_F_PREF 16         // residue-specific gap penalties
+#define CWP_F_EGP 32          // UserParameters::endGapPenalties
+
+struct PrfSide {
+    int32_t n;                // rows of the group (numSeq)
+    int32_t L;                // prfLength
+    int32_t stride;           // bytes per row in the rows buffer (>= L)
+    int32_t flags;            // CWP_F_*
+    int32_t gdist;            // gap separation distance in effect (0: none)
+    int32_t gapcoef, lencoef; // of the step
+    int32_t sum2;             // sum of the rows' weights
+    int32_t col0;             // first cw_prf_cols workgroup of this side
+    int32_t pad;
+    int64_t rows;             // row 0 of the group
+    int64_t rinfo;            // n x {weight, first, end}: [first, end) are the columns whose gaps count
+    int64_t mat;              // 32 x 32 substitution matrix (profile with substitution only)
+    int64_t prf;              // (L + 2) x CWP_PRF_W profile
+    int64_t rec;              // (L + 2) x {gap, len, mask, gap + len} for the DP
+    int64_t pl;               // (L + 2) LENCOL prefix sums for the DP
+    int64_t tmp;              // 4 x (L + 2): gap counts, hydrophilic, residue and variable weights
+};
+
+struct BuildArgs {
+    const PrfSide* sides;
+    const uint8_t* rows;
+    int32_t* pool;
+    int32_t nside;
+    int32_t ncolblk;          // cw_prf_cols workgroups in all
+    uint32_t hyd_mask;        // bit c: code c is a hydrophilic residue
+    int32_t pen[32];          // residue-specific gap penalty of each code (180 - pprob)
+};

@@ -1348,6 +1348,244 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
     }
 }
 
+// ---- profile construction ---------------------------------------------------
+// A code is a residue when it is 0..maxAA (24); gap codes are 30 and 31.
+__device__ static inline bool pgap(int c) { return c > 24; }
+
+__device__ static inline bool phyd(uint32_t mask, int c) { return !pgap(c) && ((mask >> c) & 1u); }
+
+// (int) of a float as the reference's x86-64 build converts it (cvttss2si):
+// NaN and values outside the int range give INT_MIN, where the GPU's
+// conversion would clamp.  Reached when a group's weights sum to 0.
+__device__ static inline int f2i_host(float v)
+{
+    return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000u;
+}
+
+// One thread per column of one profile: the per-column sums over the group's
+// rows that ProfileBase::calcGapCoeff gathers (gap counts between each row's
+// first and last residue when end gaps are not counted, :129-160; hydrophilic
+// stretches of more than 3 residues, calcHydrophilicPen :398-459; residue-
+// specific penalties from the column's residue counts, calcResidueSpecificPen
+// :360-391; the +-5 window identity of a two-row group, calcVPenalties
+// :310-353), and the residue columns of the profile:
+// ProfileWithSub::calcProfileWithSub (ProfileWithSub.cpp:42-147) for profile 1,
+// ProfileStandard::calcStandardProfile (ProfileStandard.cpp:41-117) for
+// profile 2.  Column sums are integer, so the order over rows does not matter.
+extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS) cw_prf_cols(BuildArgs a)
+{
+    constexpr int B = CWP_BUILD_COLS;
+    __shared__ int s_ws[32 * B];     // weighted code counts, [code][column of the block]
+    __shared__ int s_cn[32 * B];     // plain code counts
+    int lo = 0, hi = a.nside - 1;    // the side of this block: the last one with col0 <= blockIdx
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.sides[mid].col0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PrfSide sd = a.sides[lo];
+    const int t = threadIdx.x;
+    const int pos = ((int)blockIdx.x - sd.col0) * B + t;
+    if (pos >= sd.L) return;                 // no barrier below: every sum is this thread's own
+    const int n = sd.n, Lp = sd.L + 2;
+    for (int c = 0; c < 32; ++c) {
+        s_ws[c * B + t] = 0;
+        s_cn[c * B + t] = 0;
+    }
+    const uint8_t* base = a.rows + sd.rows;
+    const int32_t* rinfo = a.pool + sd.rinfo;
+    const bool hydp = (sd.flags & CWP_F_HYD) != 0;
+    int gapc = 0, hyd = 0;
+    for (int k = 0; k < n; ++k) {
+        const uint8_t* r = base + (int64_t)k * sd.stride;
+        const int wk = rinfo[3 * k], f = rinfo[3 * k + 1], e = rinfo[3 * k + 2];
+        const int c = r[pos];
+        if (c < 32) {
+            s_ws[c * B + t] += wk;
+            s_cn[c * B + t] += 1;
+        }
+        if (pgap(c)) {
+            if (pos >= f && pos < e) ++gapc;
+        } else if (hydp && phyd(a.hyd_mask, c)) {
+            // inside a stretch of more than 3 hydrophilic residues
+            int left = 0, right = 0;
+            for (int d = 1; d <= 3 && pos - d >= 0; ++d) {
+                if (!phyd(a.hyd_mask, r[pos - d])) break;
+                ++left;
+            }
+            for (int d = 1; d <= 3 - left && pos + d < sd.L; ++d) {
+                if (!phyd(a.hyd_mask, r[pos + d])) break;
+                ++right;
+            }
+            if (left + right >= 3) hyd += 100;
+        }
+    }
+    int32_t* tmp = a.pool + sd.tmp;
+    int hw = 0, rw = 0, vw = 0;
+    if (hydp) hw = (int)((float)hyd / (float)n);
+    if (sd.flags & CWP_F_PREF) {
+        int w = 0;
+        for (int c = 0; c <= 24; ++c) w += s_cn[c * B + t] * a.pen[c];
+        rw = w / n;
+    }
+    if (sd.flags & CWP_F_VAR) {              // two rows
+        const uint8_t* ra = base;
+        const uint8_t* rb = base + sd.stride;
+        int tt = 0, w = 0;
+        for (int j = pos - 5; j < pos + 5; ++j) {
+            if (j < 0 || j >= sd.L) continue;
+            const int x = ra[j], y = rb[j];
+            if (pgap(x) || pgap(y)) continue;
+            w += (x == y) ? 1 : 0;
+            ++tt;
+        }
+        w += tt;
+        w = tt > 0 ? (w * 100) / (2 * tt) : 100;
+        vw = w < 50 ? 50 : w;
+    }
+    tmp[pos] = gapc;
+    tmp[Lp + pos] = hw;
+    tmp[2 * Lp + pos] = rw;
+    tmp[3 * Lp + pos] = vw;
+    int32_t* row = a.pool + sd.prf + (int64_t)(pos + 1) * CWP_PRF_W;
+    for (int c = 25; c < 30; ++c) row[c] = 0;
+    for (int c = 32; c < CWP_PRF_W; ++c) row[c] = 0;
+    if (sd.flags & CWP_F_SUB) {
+        const int32_t* mat = a.pool + sd.mat;
+        if (gapc == n) {                     // an all-gap column
+            for (int res = 0; res <= 24; ++res) row[res] = mat[res * 32 + 30];
+            row[30] = mat[30 * 32 + 30];
+            row[31] = mat[31 * 32 + 30];
+        } else {
+            int wt[27];
+            for (int u = 0; u < 27; ++u) wt[u] = s_ws[(u < 25 ? u : u + 5) * B + t];
+            const float scale = (float)(n - gapc) / (float)n;
+            for (int q = 0; q < 27; ++q) {
+                const int res = q < 25 ? q : q + 5;
+                int f = 0;
+                for (int u = 0; u < 27; ++u) f += wt[u] * mat[(u < 25 ? u : u + 5) * 32 + res];
+                row[res] = f2i_host(((float)f / (float)sd.sum2) * scale);
+            }
+        }
+    } else {
+        for (int q = 0; q < 27; ++q) {
+            const int res = q < 25 ? q : q + 5;
+            row[res] = sd.sum2 == 0 ? 0 : (int)(10 * (float)s_ws[res * B + t] / (float)sd.sum2);
+        }
+    }
+}
+
+// One workgroup per profile: the gap-distance windows of calcGapCoeff
+// (:180-230), the per-column gap opening and extension of localPenalty
+// (:468-512) and :232-297, the end rows (:283-297), then the DP's per-column
+// records and the LENCOL prefix sums.
+//
+// The reference walks the gap counts left to right: a column with gaps keeps
+// its count; a run of such columns marks the gdist-1 gap-free columns before it
+// (-1 next to the run, -2, ...) and, up to the next run, the gdist gap-free
+// columns after it (-1, -2, ..., -gdist); a later run's marks overwrite an
+// earlier run's.  So a gap-free column p takes p - s for the last run start s
+// in (p, p + gdist - 1], else -(p - e) - 1 for the end e of the run to its left
+// when p - e < gdist, else 0.
+extern "C" __global__ void __launch_bounds__(256) cw_prf_gapcols(BuildArgs a)
+{
+    __shared__ int s_scan[256];
+    __shared__ int s_carry;
+    const PrfSide sd = a.sides[blockIdx.x];
+    const int L = sd.L, Lp = L + 2, n = sd.n, gd = sd.gdist, tid = threadIdx.x;
+    const bool dna = (sd.flags & CWP_F_DNA) != 0, egp = (sd.flags & CWP_F_EGP) != 0;
+    const int32_t* gaps = a.pool + sd.tmp;
+    const int32_t* hydW = gaps + Lp;
+    const int32_t* resW = gaps + 2 * Lp;
+    const int32_t* vW = gaps + 3 * Lp;
+    int32_t* prf = a.pool + sd.prf;
+    for (int pos = tid; pos < L; pos += blockDim.x) {
+        const int g0 = gaps[pos];
+        int gp = g0;
+        if (!dna && gd > 0 && g0 <= 0) {
+            gp = 0;
+            int sstar = -1;
+            for (int s = pos + 1; s <= pos + gd - 1 && s < L; ++s)
+                if (gaps[s] > 0 && gaps[s - 1] <= 0) sstar = s;
+            if (sstar >= 0) {
+                gp = pos - sstar;
+            } else {
+                for (int d = 0; d < gd && pos - d >= 1; ++d) {
+                    if (gaps[pos - d] > 0) break;
+                    if (gaps[pos - d - 1] > 0) {
+                        gp = -d - 1;
+                        break;
+                    }
+                }
+            }
+        }
+        int g, l;
+        if (gp <= 0) {
+            if (!dna) {
+                bool h = false;
+                float gw = 1.0f;
+                if (sd.flags & CWP_F_VAR) gw *= vW[pos] / 100.0;
+                if ((sd.flags & CWP_F_HYD) && hydW[pos] > 0) {
+                    gw *= 0.5;
+                    h = true;
+                }
+                if ((sd.flags & CWP_F_PREF) && !h) gw *= (resW[pos] / 100.0);
+                gw *= sd.gapcoef;
+                g = (int)gw;
+            } else {
+                g = sd.gapcoef;
+            }
+            l = sd.lencoef;
+            if (gp < 0) g = (int)((g * (2.0 + 2.0 * (gd + gp) / gd)));
+        } else {
+            const float scale = ((float)(n - g0) / (float)n) * 1.0f;
+            g = (int)(scale * sd.gapcoef);
+            l = (int)(0.5 * sd.lencoef);
+        }
+        if (g <= 0) g = 1;
+        if (l <= 0) l = 1;
+        if (!egp && pos + 1 == L) g = l = 0;
+        prf[(int64_t)(pos + 1) * CWP_PRF_W + CWP_GAPCOL] = g;
+        prf[(int64_t)(pos + 1) * CWP_PRF_W + CWP_LENCOL] = l;
+    }
+    if (tid < CWP_PRF_W) {
+        int v0 = 0;
+        if (egp && tid == CWP_GAPCOL) v0 = sd.gapcoef;
+        if (egp && tid == CWP_LENCOL) v0 = sd.lencoef;
+        prf[tid] = v0;                                   // row 0
+        prf[(int64_t)(L + 1) * CWP_PRF_W + tid] = 0;     // row L + 1
+    }
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    int32_t* rec = a.pool + sd.rec;
+    int32_t* pl = a.pool + sd.pl;
+    for (int b0 = 0; b0 < Lp; b0 += 256) {
+        const int x = b0 + tid;
+        int l = 0;
+        if (x < Lp) {
+            const int g = prf[(int64_t)x * CWP_PRF_W + CWP_GAPCOL];
+            l = prf[(int64_t)x * CWP_PRF_W + CWP_LENCOL];
+            const bool end = !egp && (x == 0 || x == L);
+            rec[4 * x + 0] = g;
+            rec[4 * x + 1] = l;
+            rec[4 * x + 2] = end ? 0 : -1;
+            rec[4 * x + 3] = g + l;
+        }
+        s_scan[tid] = l;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        if (x < Lp) pl[x] = s_carry + s_scan[tid] - l;
+        __syncthreads();
+        if (tid == 255) s_carry += s_scan[255];
+        __syncthreads();
+    }
+}
+
 // ---- host-side launchers (used by cwhip_prf.cpp) ---------------------------
 hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a)
 {
@@ -1358,5 +1596,14 @@ hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a)
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 {
     hipLaunchKernelGGL(cw_prf_mm, dim3(nprob), dim3(CWP_THREADS), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a)
+{
+    if (ncolblk > 0) hipLaunchKernelGGL(cw_prf_cols, dim3(ncolblk), dim3(CWP_BUILD_COLS), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (nside > 0) hipLaunchKernelGGL(cw_prf_gapcols, dim3(nside), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -74,6 +74,9 @@ struct cwhip_ctx {
     DevBuf<int32_t> p_pool, p_out, p_tiles;
     DevBuf<PrfProb> p_probs;
     PinBuf<int32_t> h_prf_in, h_prf_nodes;      // staging of a profile batch's inputs and node tables
+    DevBuf<uint8_t> p_rows;                     // group rows of a profile batch
+    DevBuf<PrfSide> p_sides;                    // its profiles to build
+    PinBuf<uint8_t> h_rows;                     // staging of the rows
     int prf_rmax = CWP_RMAX;
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;

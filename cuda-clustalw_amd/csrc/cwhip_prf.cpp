@@ -28,6 +28,7 @@
 
 hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a);
+hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a);
 
 namespace {
 
@@ -205,26 +206,28 @@ void parallel_for(size_t n, F fn)
     HostPool::get().run(n, fn);
 }
 
-// One device batch of profile pairs: inputs packed (on host threads) into
-// page-locked staging and uploaded in one copy, the score and Myers-Miller
-// kernels, the node tables (contiguous in the pool) fetched in one copy and
-// turned into edit scripts on host threads.
-int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
+// Offsets of a batch of profile pairs in the pool: inputs (raw profiles,
+// {gap, len, mask, gap + len} records per row and column, LENCOL prefix sums)
+// from 0, then the work area: the node tables of all pairs first (one copy
+// back), then per pair the score matrix, per-wave scratch rows and level buffers.
+struct BatchLayout {
+    std::vector<PrfProb> probs;
+    std::vector<int32_t> tile_base;
+    int64_t in_sz = 0, nodes_sz = 0, total = 0;
+};
+
+void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
 {
     const int n = (int)jobs.size();
-    if (n == 0) return CWHIP_OK;
-    HIPCHK(hipSetDevice(ctx->device));
-    std::vector<PrfProb> probs(n);
-    std::vector<int32_t> tile_base(n + 1, 0);
+    lo.probs.assign(n, PrfProb{});
+    lo.tile_base.assign(n + 1, 0);
     int64_t in_sz = 0, work_sz = 0;
     for (int k = 0; k < n; ++k) {
         const int L1 = jobs[k].L1, L2 = jobs[k].L2;
-        PrfProb& pp = probs[k];
+        PrfProb& pp = lo.probs[k];
         pp.L1 = L1;
         pp.L2 = L2;
         pp.ldS = L1 + 2;
-        pp.flags = (jobs[k].prf1[GAPCOL] != 0 ? 1 : 0) |
-                   (jobs[k].prf1[(int64_t)L1 * W + GAPCOL] != 0 ? 2 : 0);
         pp.node_cap = 2 * L1 + 4;
         pp.rowcap = (int)up4(L2 + 2);
         pp.prf1 = in_sz; in_sz = up4(in_sz + (int64_t)(L1 + 2) * W);
@@ -233,29 +236,105 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
         pp.col = in_sz; in_sz += 4 * (int64_t)(L2 + 2);
         pp.pl1 = in_sz; in_sz = up4(in_sz + L1 + 2);
         pp.pl2 = in_sz; in_sz = up4(in_sz + L2 + 2);
-        tile_base[k + 1] = tile_base[k] + ((L1 + 63) / 64) * ((L2 + 63) / 64);
+        lo.tile_base[k + 1] = lo.tile_base[k] + ((L1 + 63) / 64) * ((L2 + 63) / 64);
     }
-    // work area: node tables of all pairs first (one copy back), then per pair
-    // the score matrix, per-wave scratch rows and level buffers
     for (int k = 0; k < n; ++k) {
-        probs[k].nodes = work_sz;
-        work_sz = up4(work_sz + (int64_t)probs[k].node_cap * CWP_NODE_INTS);
+        lo.probs[k].nodes = work_sz;
+        work_sz = up4(work_sz + (int64_t)lo.probs[k].node_cap * CWP_NODE_INTS);
     }
-    const int64_t nodes_sz = work_sz;
+    lo.nodes_sz = work_sz;
     for (int k = 0; k < n; ++k) {
-        PrfProb& pp = probs[k];
+        PrfProb& pp = lo.probs[k];
         pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(pp.L2 + 2) * pp.ldS);
         pp.scratch = work_sz; work_sz += (int64_t)CWP_WAVES * CWP_SCR_ROWS * pp.rowcap;
         pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)pp.L2 + 2 + pp.node_cap));
     }
-    for (auto& pp : probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
+    for (auto& pp : lo.probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
+    lo.in_sz = in_sz;
+    lo.total = in_sz + work_sz;
+}
+
+// The score and Myers-Miller kernels over a batch whose inputs are in the pool
+// (queued on the context's stream behind whatever wrote them; ev0 is recorded
+// by the caller before its first kernel), then the node tables back in one
+// copy and the edit scripts on host threads.
+int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>& jobs)
+{
+    const int n = (int)jobs.size();
+    HIPCHK(ctx->p_probs.ensure(n));
+    HIPCHK(ctx->p_out.ensure(4 * (size_t)n));
+    HIPCHK(ctx->p_tiles.ensure(n + 1));
+    HIPCHK(hipMemcpyAsync(ctx->p_probs.p, lo.probs.data(), n * sizeof(PrfProb), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->p_tiles.p, lo.tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    PrfArgs a;
+    a.probs = ctx->p_probs.p;
+    a.pool = ctx->p_pool.p;
+    a.out = ctx->p_out.p;
+    a.tile_base = ctx->p_tiles.p;
+    a.nprob = n;
+    a.rmax = ctx->prf_rmax;
+    if (lo.tile_base[n] > 0) HIPCHK(cwp_launch_score(lo.tile_base[n], ctx->stream, a));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(cwp_launch_mm(n, ctx->stream, a));
+    HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
+    HIPCHK(ctx->h_prf_nodes.ensure(lo.nodes_sz + 4 * (size_t)n + 4));
+    int32_t* h_nodes = ctx->h_prf_nodes.p;
+    int32_t* out = h_nodes + lo.nodes_sz;
+    HIPCHK(hipMemcpyAsync(out, ctx->p_out.p, 4 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_nodes, ctx->p_pool.p + lo.in_sz, lo.nodes_sz * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms1 = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms1, ctx->ev0, ctx->ev1));
+    HIPCHK(hipEventElapsedTime(&ms2, ctx->ev1, ctx->ev2));
+    ctx->prf_score_ms += ms1;
+    ctx->prf_mm_ms += ms2;
+    for (int k = 0; k < n; ++k) {
+        const int nn = out[4 * k + 1];
+        if (out[4 * k + 2] != 0 || nn < 1 || nn > lo.probs[k].node_cap)
+            return fail(ctx, CWHIP_E_KERNEL, "profile DP node table overflow");
+    }
+    std::vector<char> bad(n, 0);
+    std::vector<double> cells(n, 0.0);
+    parallel_for(n, [&](size_t k) {
+        Emitter em{jobs[k].displ};
+        if (!emit_script(h_nodes + (lo.probs[k].nodes - lo.in_sz), out[4 * k + 1], em, &cells[k])) {
+            bad[k] = 1;
+            return;
+        }
+        jobs[k].n_displ = em.printPtr - 1;
+        jobs[k].score = out[4 * k + 0];
+        jobs[k].cells = cells[k];
+    });
+    for (int k = 0; k < n; ++k) {
+        if (bad[k]) return fail(ctx, CWHIP_E_KERNEL, "profile DP node table is malformed");
+        ctx->prf_cells += cells[k];
+        ctx->prf_entries += (double)jobs[k].L1 * jobs[k].L2;
+    }
+    return CWHIP_OK;
+}
+
+// One device batch of profile pairs whose profiles the caller built: the
+// inputs packed (on host threads) into page-locked staging and uploaded in one
+// copy, then dp_launch_collect.
+int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
+{
+    const int n = (int)jobs.size();
+    if (n == 0) return CWHIP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    BatchLayout lo;
+    layout_batch(jobs, lo);
+    for (int k = 0; k < n; ++k)
+        lo.probs[k].flags = (jobs[k].prf1[GAPCOL] != 0 ? 1 : 0) |
+                            (jobs[k].prf1[(int64_t)jobs[k].L1 * W + GAPCOL] != 0 ? 2 : 0);
     // host images of the inputs: raw profiles, {gap, len, mask, gap+len}
     // records and LENCOL prefix sums (the mask is 0 at the profile ends when
     // end-gap penalties are off, the rule of openPenalty1/2 and extPenalty1/2)
-    HIPCHK(ctx->h_prf_in.ensure(in_sz + 4));
+    HIPCHK(ctx->h_prf_in.ensure(lo.in_sz + 4));
     int32_t* h_in = ctx->h_prf_in.p;
     parallel_for(n, [&](size_t k) {
-        const PrfProb& pp = probs[k];
+        const PrfProb& pp = lo.probs[k];
         const DpJob& j = jobs[k];
         std::memcpy(&h_in[pp.prf1], j.prf1, sizeof(int32_t) * (size_t)(j.L1 + 2) * W);
         std::memcpy(&h_in[pp.prf2], j.prf2, sizeof(int32_t) * (size_t)(j.L2 + 2) * W);
@@ -277,60 +356,10 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
             }
         }
     });
-    HIPCHK(ctx->p_pool.ensure(in_sz + work_sz + 4));
-    HIPCHK(ctx->p_probs.ensure(n));
-    HIPCHK(ctx->p_out.ensure(4 * (size_t)n));
-    HIPCHK(ctx->p_tiles.ensure(n + 1));
-    HIPCHK(hipMemcpyAsync(ctx->p_pool.p, h_in, in_sz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->p_probs.p, probs.data(), n * sizeof(PrfProb), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->p_tiles.p, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    PrfArgs a;
-    a.probs = ctx->p_probs.p;
-    a.pool = ctx->p_pool.p;
-    a.out = ctx->p_out.p;
-    a.tile_base = ctx->p_tiles.p;
-    a.nprob = n;
-    a.rmax = ctx->prf_rmax;
+    HIPCHK(ctx->p_pool.ensure(lo.total + 4));
+    HIPCHK(hipMemcpyAsync(ctx->p_pool.p, h_in, lo.in_sz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    if (tile_base[n] > 0) HIPCHK(cwp_launch_score(tile_base[n], ctx->stream, a));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    HIPCHK(cwp_launch_mm(n, ctx->stream, a));
-    HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
-    HIPCHK(ctx->h_prf_nodes.ensure(nodes_sz + 4 * (size_t)n + 4));
-    int32_t* h_nodes = ctx->h_prf_nodes.p;
-    int32_t* out = h_nodes + nodes_sz;
-    HIPCHK(hipMemcpyAsync(out, ctx->p_out.p, 4 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(h_nodes, ctx->p_pool.p + in_sz, nodes_sz * sizeof(int32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, ctx->ev0, ctx->ev1));
-    HIPCHK(hipEventElapsedTime(&ms2, ctx->ev1, ctx->ev2));
-    ctx->prf_score_ms += ms1;
-    ctx->prf_mm_ms += ms2;
-    for (int k = 0; k < n; ++k) {
-        const int nn = out[4 * k + 1];
-        if (out[4 * k + 2] != 0 || nn < 1 || nn > probs[k].node_cap)
-            return fail(ctx, CWHIP_E_KERNEL, "profile DP node table overflow");
-    }
-    std::vector<char> bad(n, 0);
-    std::vector<double> cells(n, 0.0);
-    parallel_for(n, [&](size_t k) {
-        Emitter em{jobs[k].displ};
-        if (!emit_script(h_nodes + (probs[k].nodes - in_sz), out[4 * k + 1], em, &cells[k])) {
-            bad[k] = 1;
-            return;
-        }
-        jobs[k].n_displ = em.printPtr - 1;
-        jobs[k].score = out[4 * k + 0];
-        jobs[k].cells = cells[k];
-    });
-    for (int k = 0; k < n; ++k) {
-        if (bad[k]) return fail(ctx, CWHIP_E_KERNEL, "profile DP node table is malformed");
-        ctx->prf_cells += cells[k];
-        ctx->prf_entries += (double)jobs[k].L1 * jobs[k].L2;
-    }
-    return CWHIP_OK;
+    return dp_launch_collect(ctx, lo, jobs);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,312 +435,76 @@ int profile_matrix(const cwhip_msa_params& p, double pcid, int minLen, int mat[N
 
 inline bool is_gap_code(int c) { return c < 0 || c > MAXAA; }
 
-// One profile: rows[first..last) of the step, prfLength L.
-struct ProfileSide {
-    int first, last, L;
-    std::vector<int32_t> prf;   // (L + 2) x W
-    bool nVarPen = true, nHydPen = false, nPrefPen = false;
-    int gdist = 0;
-    int32_t& at(int i, int j) { return prf[(size_t)i * W + j]; }
+// The switches ProfileBase::calcGapCoeff takes for one profile (:64-95): a
+// group of two rows more than 60% identical uses the variable penalties
+// instead of the hydrophilic, residue-specific and gap-distance ones.
+struct SideSpec {
+    int first = 0, last = 0, L = 0;    // member rows [first, last) of the step, prfLength
+    int flags = 0, gdist = 0, sum2 = 0;
 };
-
-// Per column of rows[first..last): the weighted residue counts and the plain
-// counts of every code, gathered in one row-major pass (on host threads over
-// column blocks when `par` and the group is large).  Integer sums, so the
-// order of the additions does not matter.
-struct ColHist {
-    std::vector<int> wsum, cnt;   // [col * NUMRES + code]
-};
-
-void column_hist(const std::vector<const uint8_t*>& rows, const std::vector<int>& w, int first,
-                 int last, int L, bool par, ColHist& h)
-{
-    h.wsum.assign((size_t)L * NUMRES, 0);
-    h.cnt.assign((size_t)L * NUMRES, 0);
-    constexpr int blk = 256;
-    const size_t nb = (size_t)(L + blk - 1) / blk;
-    auto body = [&](size_t b) {
-        const int c0 = (int)b * blk, c1 = std::min(L, c0 + blk);
-        for (int k = first; k < last; ++k) {
-            const uint8_t* r = rows[k];
-            const int wk = w[k];
-            for (int col = c0; col < c1; ++col) {
-                const int c = r[col];
-                if (c < NUMRES) {
-                    h.wsum[(size_t)col * NUMRES + c] += wk;
-                    h.cnt[(size_t)col * NUMRES + c] += 1;
-                }
-            }
-        }
-    };
-    if (par && (int64_t)(last - first) * L > (1 << 18)) parallel_for(nb, body);
-    else
-        for (size_t b = 0; b < nb; ++b) body(b);
-}
-
-// ProfileBase::calcGapCoeff (ProfileBase.cpp:52-298) and its helpers
-// (:310-549).  `rows` are the step's padded rows; `gaps` is the per-column
-// count the caller shares with calcProfileWithSub.
-// sum over rows first..last-1 of per-row contributions to an L-long integer
-// array: row chunks on the host pool (when `par`), each with its own array,
-// added up at the end (integer sums: the order does not matter)
-template <typename RowFn>
-void rows_sum(int first, int last, int L, bool par, std::vector<int>& out, RowFn row_fn)
-{
-    const int n = last - first;
-    const int nch = (par && (int64_t)n * L > (1 << 17)) ? std::min(16, std::max(1, n / 16)) : 1;
-    if (nch == 1) {
-        for (int k = first; k < last; ++k) row_fn(k, out.data());
-        return;
-    }
-    std::vector<std::vector<int>> acc(nch, std::vector<int>(L, 0));
-    parallel_for(nch, [&](size_t c) {
-        const int a = first + (int)((int64_t)n * c / nch), b = first + (int)((int64_t)n * (c + 1) / nch);
-        for (int k = a; k < b; ++k) row_fn(k, acc[c].data());
-    });
-    for (int c = 0; c < nch; ++c)
-        for (int j = 0; j < L; ++j) out[j] += acc[c][j];
-}
-
-void calc_gap_coeff(const cwhip_msa_params& p, ProfileSide& s, const std::vector<const uint8_t*>& rows,
-                    const ColHist& hist, std::vector<int>& gaps, int gapCoef, int lenCoef, bool par)
-{
-    const int L = s.L, first = s.first, last = s.last, numSeq = last - first;
-    int pcid = 0;
-    if (numSeq == 2) {                                   // percentId (:519-547)
-        const uint8_t* s1 = rows[first];
-        const uint8_t* s2 = rows[first + 1];
-        int count = 0, total = 0;
-        for (int i = 0; i < L; ++i) {
-            if (s1[i] < MAXAA) {                          // (>= 0 always holds for codes)
-                ++total;
-                if (s1[i] == s2[i]) ++count;
-            }
-        }
-        pcid = total == 0 ? 0 : (int)(float)(100.0 * (float)count / (float)total);
-    }
-    for (int j = 0; j < L; ++j) gaps[j] = 0;
-    if (!p.no_var_penalties && pcid > 60) {
-        s.nHydPen = s.nPrefPen = true;
-        s.nVarPen = false;
-    } else {
-        s.nVarPen = true;
-        s.nHydPen = p.no_hyd_penalties != 0;
-        s.nPrefPen = p.no_pref_penalties != 0;
-        s.gdist = p.gap_dist;
-    }
-    rows_sum(first, last, L, par, gaps, [&](int i, int* acc) {
-        const uint8_t* r = rows[i];
-        int is = 0, ie = L;
-        if (!p.use_end_gaps && !p.end_gap_penalties) {
-            while (is < L && is_gap_code(r[is])) ++is;
-            if (is < L)
-                while (ie > 0 && is_gap_code(r[ie - 1])) --ie;
-            else
-                ie = 0;
-        }
-        for (int j = is; j < ie; ++j)
-            if (is_gap_code(r[j])) acc[j]++;
-    });
-    std::vector<int> vW, resW, hydW;
-    if (!p.dna && !s.nVarPen) {                          // calcVPenalties (:310-353)
-        vW.assign(L + 2, 0);
-        const uint8_t* a = rows[first];
-        const uint8_t* b = rows[first + 1];
-        for (int i = 0; i < L; ++i) {
-            int t = 0, w = 0;
-            for (int j = i - 5; j < i + 5; ++j) {
-                if (j < 0 || j >= L) continue;
-                if (is_gap_code(a[j]) || is_gap_code(b[j])) continue;
-                w += (a[j] == b[j]) ? 1 : 0;
-                ++t;
-            }
-            w += t;
-            w = t > 0 ? (w * 100) / (2 * t) : 100;
-            vW[i] = w < 50 ? 50 : w;
-        }
-    }
-    if (!p.dna && !s.nPrefPen) {                         // calcResidueSpecificPen (:360-391)
-        static const char pres[] = "ACDEFGHKILMNPQRSTVYW";
-        static const int pprob[] = {87, 87, 104, 69, 80, 139, 100, 104, 68, 79,
-                                    71, 137, 126, 93, 128, 124, 111, 75, 100, 77};
-        int pen[NUMRES];
-        for (int c = 0; c < NUMRES; ++c) pen[c] = 0;
-        for (int k = 0; k < 20; ++k) pen[code_of(pres[k])] = 180 - pprob[k];
-        resW.assign(L + 2, 0);
-        for (int i = 0; i < L; ++i) {
-            int w = 0;
-            for (int c = 0; c <= MAXAA; ++c) w += hist.cnt[(size_t)i * NUMRES + c] * pen[c];
-            resW[i] = w / numSeq;
-        }
-    }
-    if (!p.dna && !s.nHydPen) {                          // calcHydrophilicPen (:398-459)
-        bool hyd[NUMRES];
-        for (int c = 0; c < NUMRES; ++c) hyd[c] = false;
-        for (const char* h = p.hyd_residues ? p.hyd_residues : ""; *h; ++h) {
-            const int c = code_of(*h);
-            if (c >= 0 && c <= MAXAA) hyd[c] = true;
-        }
-        hydW.assign(L + 2, 0);
-        rows_sum(first, last, L, par, hydW, [&](int k, int* acc) {
-            const uint8_t* r = rows[k];
-            int i = 0;
-            while (i < L) {
-                if (is_gap_code(r[i]) || !hyd[r[i]]) { ++i; continue; }
-                const int st = i;
-                while (i < L && !is_gap_code(r[i]) && hyd[r[i]]) ++i;
-                if (i - st > 3)
-                    for (int j = st; j < i; ++j) acc[j] += 100;
-            }
-        });
-        const float scale = (float)numSeq;
-        for (int i = 0; i < L; ++i) hydW[i] = (int)(hydW[i] / scale);
-    }
-    std::vector<int> gapPos(L + 2, 0);
-    if (p.dna || s.gdist <= 0) {
-        for (int i = 0; i < L; ++i) gapPos[i] = gaps[i];
-    } else {
-        const int gdist = s.gdist;
-        auto gapAt = [&](int x) { return x < (int)gaps.size() ? gaps[x] : 0; };
-        int i = 0, j;
-        while (i < L) {
-            if (gaps[i] <= 0) {
-                gapPos[i] = gaps[i];
-                ++i;
-            } else {
-                for (j = -gdist + 1; j < 0; ++j)
-                    if (i + j >= 0 && i + j < L && (gaps[i + j] == 0 || gaps[i + j] < j)) gapPos[i + j] = j;
-                while (gapAt(i) > 0) {
-                    if (i >= L) break;
-                    gapPos[i] = gaps[i];
-                    ++i;
-                }
-                for (j = 0; j < gdist; ++j) {
-                    if (gapAt(i + j) > 0) break;
-                    if (i + j >= 0 && i + j < L && (gaps[i + j] == 0 || gaps[i + j] < -j)) gapPos[i + j] = -j - 1;
-                }
-                i += j;
-            }
-        }
-    }
-    for (int j = 0; j < L; ++j) {
-        int32_t& g = s.at(j + 1, GAPCOL);
-        int32_t& l = s.at(j + 1, LENCOL);
-        if (gapPos[j] <= 0) {
-            if (!p.dna) {                                  // localPenalty (:468-512)
-                bool h = false;
-                float gw = 1.0f;
-                if (!s.nVarPen) gw *= vW[j] / 100.0;
-                if (!s.nHydPen && hydW[j] > 0) {
-                    gw *= 0.5;
-                    h = true;
-                }
-                if (!s.nPrefPen && !h) gw *= (resW[j] / 100.0);
-                gw *= gapCoef;
-                g = (int)gw;
-            } else {
-                g = gapCoef;
-            }
-            l = lenCoef;
-            if (gapPos[j] < 0) g = (int)((g * (2.0 + 2.0 * (s.gdist + gapPos[j]) / s.gdist)));
-        } else {
-            const float scale = ((float)(numSeq - gaps[j]) / (float)numSeq) * 1.0f;
-            g = (int)(scale * gapCoef);
-            l = (int)(0.5 * lenCoef);
-        }
-        if (g <= 0) g = 1;
-        if (l <= 0) l = 1;
-    }
-    if (p.end_gap_penalties) {
-        s.at(0, GAPCOL) = gapCoef;
-        s.at(0, LENCOL) = lenCoef;
-    } else {
-        s.at(0, GAPCOL) = 0;
-        s.at(0, LENCOL) = 0;
-        s.at(L, GAPCOL) = 0;
-        s.at(L, LENCOL) = 0;
-    }
-}
-
-// ProfileWithSub::calcProfileWithSub (ProfileWithSub.cpp:42-147)
-void calc_profile_with_sub(ProfileSide& s, const ColHist& hist, const std::vector<int>& gaps,
-                           const int mat[NUMRES][NUMRES], const std::vector<int>& w, bool par)
-{
-    const int numSeq = s.last - s.first;
-    int sum2 = 0;
-    for (int k = s.first; k < s.last; ++k) sum2 += w[k];
-    auto column = [&](int pos) {
-        if (gaps[pos] == numSeq) {
-            for (int res = 0; res <= MAXAA; ++res) s.at(pos + 1, res) = mat[res][GAP1];
-            s.at(pos + 1, GAP1) = mat[GAP1][GAP1];
-            s.at(pos + 1, GAP2) = mat[GAP2][GAP1];
-            return;
-        }
-        const int* wt = &hist.wsum[(size_t)pos * NUMRES];
-        const float scale = (float)(numSeq - gaps[pos]) / (float)numSeq;
-        // the weighted sum over the codes present in the column (integer, so
-        // the order and the zero terms the reference also adds do not matter)
-        int present[NUMRES], np = 0;
-        for (int aa = 0; aa <= MAXAA; ++aa)
-            if (wt[aa]) present[np++] = aa;
-        if (wt[GAP1]) present[np++] = GAP1;
-        if (wt[GAP2]) present[np++] = GAP2;
-        auto one = [&](int res) {
-            int f = 0;
-            for (int k = 0; k < np; ++k) f += wt[present[k]] * mat[present[k]][res];
-            return (int)(((float)f / (float)sum2) * scale);
-        };
-        for (int res = 0; res <= MAXAA; ++res) s.at(pos + 1, res) = one(res);
-        s.at(pos + 1, GAP1) = one(GAP1);
-        s.at(pos + 1, GAP2) = one(GAP2);
-    };
-    constexpr int blk = 128;
-    const size_t nb = (size_t)(s.L + blk - 1) / blk;
-    auto body = [&](size_t b) {
-        for (int pos = (int)b * blk; pos < std::min(s.L, (int)(b + 1) * blk); ++pos) column(pos);
-    };
-    if (par && (int64_t)numSeq * s.L > (1 << 16)) parallel_for(nb, body);
-    else
-        for (size_t b = 0; b < nb; ++b) body(b);
-}
-
-// ProfileStandard::calcStandardProfile (ProfileStandard.cpp:41-117)
-void calc_standard_profile(ProfileSide& s, const ColHist& hist, const std::vector<int>& w)
-{
-    int sum2 = 0;
-    for (int k = s.first; k < s.last; ++k) sum2 += w[k];
-    for (int r = 0; r < s.L; ++r) {
-        if (sum2 == 0) {
-            for (int d = 0; d <= MAXAA; ++d) s.at(r + 1, d) = 0;
-            s.at(r + 1, GAP1) = 0;
-            s.at(r + 1, GAP2) = 0;
-            continue;
-        }
-        const int* wt = &hist.wsum[(size_t)r * NUMRES];
-        for (int d = 0; d <= MAXAA; ++d) s.at(r + 1, d) = (int)(10 * (float)wt[d] / (float)sum2);
-        s.at(r + 1, GAP1) = (int)(10 * (float)wt[GAP1] / (float)sum2);
-        s.at(r + 1, GAP2) = (int)(10 * (float)wt[GAP2] / (float)sum2);
-    }
-}
 
 // One profileAlign call (MyersMillerProfileAlign.cpp:36-505) split into the
-// part before the DP (prepare) and the part after it (apply).
+// part before the DP (prepare: groups, pcid, the profile matrix, gap
+// coefficients, the profile switches; the profiles are built on the device)
+// and the part after it (apply).
 struct Step {
     std::vector<int> members;          // group-1 sequences, then group-2 ones
     int n1 = 0, n2 = 0, L1 = 0, L2 = 0;
-    std::vector<uint8_t> padded;       // (n1 + n2) padded rows
-    std::vector<const uint8_t*> rows;
-    ProfileSide p1, p2;
+    std::vector<int> w;                // weights of the members
+    std::vector<int32_t> mat;          // NUMRES x NUMRES profile matrix
+    int gapcoef = 0, lencoef = 0;
+    SideSpec s1, s2;
     std::vector<int32_t> displ;
     DpJob job;
 };
 
 using Seqs = std::vector<std::vector<uint8_t>>;
 
+// row r of a group whose profile length is L: the sequence, then GAP1 up to L
+inline int row_at(const std::vector<uint8_t>& r, int i) { return i < (int)r.size() ? r[i] : GAP1; }
+
+void side_spec(const cwhip_msa_params& p, const Seqs& seqs, const Step& st, SideSpec& s, bool sub)
+{
+    const int numSeq = s.last - s.first;
+    int pcid = 0;
+    if (numSeq == 2) {                                   // percentId (ProfileBase.cpp:519-547)
+        const std::vector<uint8_t>& s1 = seqs[st.members[s.first]];
+        const std::vector<uint8_t>& s2 = seqs[st.members[s.first + 1]];
+        int count = 0, total = 0;
+        for (int i = 0; i < s.L; ++i) {
+            const int c1 = row_at(s1, i);
+            if (c1 < MAXAA) {
+                ++total;
+                if (c1 == row_at(s2, i)) ++count;
+            }
+        }
+        pcid = total == 0 ? 0 : (int)(float)(100.0 * (float)count / (float)total);
+    }
+    bool nVarPen, nHydPen, nPrefPen;
+    s.gdist = 0;
+    if (!p.no_var_penalties && pcid > 60) {
+        nHydPen = nPrefPen = true;
+        nVarPen = false;
+    } else {
+        nVarPen = true;
+        nHydPen = p.no_hyd_penalties != 0;
+        nPrefPen = p.no_pref_penalties != 0;
+        s.gdist = p.gap_dist;
+    }
+    s.flags = (sub ? CWP_F_SUB : 0) | (p.end_gap_penalties ? CWP_F_EGP : 0);
+    if (p.dna) s.flags |= CWP_F_DNA;
+    else
+        s.flags |= (nVarPen ? 0 : CWP_F_VAR) | (nHydPen ? 0 : CWP_F_HYD) | (nPrefPen ? 0 : CWP_F_PREF);
+    s.sum2 = 0;
+    for (int k = s.first; k < s.last; ++k) s.sum2 += st.w[k];
+}
+
 // Returns false when a group is empty (profileAlign returns 0, :96-99).
-bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim,
-                  const std::vector<int>& weights, std::vector<int>& group,
-                  const std::vector<int>& aligned, Step& st, bool par)
+// rescnt[s]: residues (codes other than GAP1/GAP2) of sequence s, which gaps
+// inserted by earlier steps do not change.
+bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector<int>& rescnt,
+                  const double* sim, const std::vector<int>& weights, std::vector<int>& group,
+                  const std::vector<int>& aligned, Step& st)
 {
     const int nseq = (int)seqs.size();
     for (int i = 0; i < nseq; ++i)
@@ -742,35 +535,20 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim
     }
     pcid = pcid / (float)count;
     st.members.clear();
-    int L1 = 0, L2 = 0;
+    st.w.clear();
+    int L1 = 0, L2 = 0, len1 = 0, len2 = 0;               // :205-240 residues per row
     for (int g = 1; g <= 2; ++g)
         for (int i = 0; i < nseq; ++i)
             if (group[i + 1] == g) {
                 st.members.push_back(i);
-                int& L = g == 1 ? L1 : L2;
-                L = std::max(L, (int)seqs[i].size());
+                st.w.push_back(weights[i]);
+                (g == 1 ? L1 : L2) = std::max(g == 1 ? L1 : L2, (int)seqs[i].size());
+                (g == 1 ? len1 : len2) += rescnt[i];
             }
     st.n1 = n1;
     st.n2 = n2;
     st.L1 = L1;
     st.L2 = L2;
-    const int Lmax = std::max(L1, L2);
-    const int stride = Lmax + 2;
-    st.padded.assign((size_t)(n1 + n2) * stride, (uint8_t)GAP1);
-    st.rows.resize(n1 + n2);
-    std::vector<int> w(n1 + n2);
-    int len1 = 0, len2 = 0;                               // :205-240 residues per row
-    for (int k = 0; k < n1 + n2; ++k) {
-        const std::vector<uint8_t>& s = seqs[st.members[k]];
-        uint8_t* r = &st.padded[(size_t)k * stride];
-        std::memcpy(r, s.data(), s.size());
-        st.rows[k] = r;
-        w[k] = weights[st.members[k]];
-        int is = 0;
-        for (uint8_t c : s)
-            if (c != GAP1 && c != GAP2) ++is;
-        (k < n1 ? len1 : len2) += is;
-    }
     len1 = (int)(len1 / (float)n1);
     len2 = (int)(len2 / (float)n2);
     const int minLen = std::min(len1, len2);
@@ -778,6 +556,7 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim
     float scale = 1.0f;
     int matAvg = 0;
     profile_matrix(p, pcid, minLen, mat, &scale, &matAvg);
+    st.mat.assign(&mat[0][0], &mat[0][0] + NUMRES * NUMRES);
     int gapcoef, lencoef;                                 // :254-296
     if (p.dna) {
         gapcoef = (int)(100.0 * p.gap_open * scale);
@@ -799,25 +578,156 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const double* sim
         else gapcoef = (int)(scale * matAvg * (float)(p.gap_open / (logdiff * logmin)));
         lencoef = (int)(100.0 * p.gap_extend);
     }
-    std::vector<int> gaps(L1 + L2 + 3, 0);
-    st.p1.first = 0; st.p1.last = n1; st.p1.L = L1;
-    st.p1.prf.assign((size_t)(L1 + 2) * W, 0);
-    st.p2.first = n1; st.p2.last = n1 + n2; st.p2.L = L2;
-    st.p2.prf.assign((size_t)(L2 + 2) * W, 0);
-    ColHist hist;
-    column_hist(st.rows, w, 0, n1, L1, par, hist);
-    calc_gap_coeff(p, st.p1, st.rows, hist, gaps, gapcoef, lencoef, par);
-    calc_profile_with_sub(st.p1, hist, gaps, mat, w, par);
-    column_hist(st.rows, w, n1, n1 + n2, L2, par, hist);
-    calc_gap_coeff(p, st.p2, st.rows, hist, gaps, gapcoef, lencoef, par);
-    calc_standard_profile(st.p2, hist, w);
+    st.gapcoef = gapcoef;
+    st.lencoef = lencoef;
+    st.s1.first = 0; st.s1.last = n1; st.s1.L = L1;
+    st.s2.first = n1; st.s2.last = n1 + n2; st.s2.L = L2;
+    side_spec(p, seqs, st, st.s1, true);
+    side_spec(p, seqs, st, st.s2, false);
     st.displ.assign(L1 + L2 + 4, 0);
-    st.job.prf1 = st.p1.prf.data();
-    st.job.prf2 = st.p2.prf.data();
+    st.job.prf1 = st.job.prf2 = nullptr;
     st.job.L1 = L1;
     st.job.L2 = L2;
     st.job.displ = st.displ.data();
     return true;
+}
+
+// The residue-specific gap penalty of each code (calcResidueSpecificPen,
+// ProfileBase.cpp:360-391) and the hydrophilic residue set.
+void penalty_tables(const cwhip_msa_params& p, BuildArgs& ba)
+{
+    static const char pres[] = "ACDEFGHKILMNPQRSTVYW";
+    static const int pprob[] = {87, 87, 104, 69, 80, 139, 100, 104, 68, 79,
+                                71, 137, 126, 93, 128, 124, 111, 75, 100, 77};
+    for (int c = 0; c < 32; ++c) ba.pen[c] = 0;
+    for (int k = 0; k < 20; ++k) ba.pen[code_of(pres[k])] = 180 - pprob[k];
+    ba.hyd_mask = 0;
+    for (const char* h = p.hyd_residues ? p.hyd_residues : ""; *h; ++h) {
+        const int c = code_of(*h);
+        if (c >= 0 && c <= MAXAA) ba.hyd_mask |= 1u << c;
+    }
+}
+
+// One device batch of steps: the member rows (padded with GAP1 to each
+// profile's length), weights, gap ranges and distinct profile matrices packed
+// on host threads into page-locked staging and uploaded in two copies; the
+// profiles built by cw_prf_cols and cw_prf_gapcols straight into the DP's
+// inputs; then (with `dp`) the score and Myers-Miller kernels.  `lo` receives
+// the batch layout (the profiles of step k are at lo.probs[k].prf1 / prf2).
+int run_built_batch(cwhip_ctx* ctx, const cwhip_msa_params& P, const Seqs& seqs,
+                    const std::vector<Step*>& st, bool dp, BatchLayout& lo)
+{
+    const int n = (int)st.size();
+    if (n == 0) return CWHIP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<DpJob> jobs(n);
+    for (int k = 0; k < n; ++k) jobs[k] = st[k]->job;
+    layout_batch(jobs, lo);
+    const bool egp = P.end_gap_penalties != 0;
+    for (int k = 0; k < n; ++k)        // nonzero-ness of prf1 GAPCOL at rows 0 and L1 (the top call's go1/go2)
+        lo.probs[k].flags = (egp && st[k]->gapcoef != 0 ? 1 : 0) |
+                            (egp && (st[k]->L1 >= 1 || st[k]->gapcoef != 0) ? 2 : 0);
+    // distinct profile matrices of the batch (a handful of series and scales)
+    std::vector<int> mat_of(n);
+    std::vector<int> uniq;
+    for (int k = 0; k < n; ++k) {
+        int u = 0;
+        while (u < (int)uniq.size() &&
+               std::memcmp(st[uniq[u]]->mat.data(), st[k]->mat.data(), sizeof(int32_t) * NUMRES * NUMRES) != 0)
+            ++u;
+        if (u == (int)uniq.size()) uniq.push_back(k);
+        mat_of[k] = u;
+    }
+    // uploaded ints after the DP's layout: matrices, then per side the rows'
+    // {weight, first, end}; then the builder's per-column sums (device only)
+    const int64_t up0 = lo.total;
+    int64_t off = up0 + (int64_t)uniq.size() * NUMRES * NUMRES;
+    std::vector<PrfSide> sides(2 * n);
+    int64_t row_bytes = 0;
+    int colblk = 0;
+    for (int k = 0; k < n; ++k)
+        for (int h = 0; h < 2; ++h) {
+            const SideSpec& ss = h ? st[k]->s2 : st[k]->s1;
+            PrfSide& sd = sides[2 * k + h];
+            sd.n = ss.last - ss.first;
+            sd.L = ss.L;
+            sd.stride = (int)std::max<int64_t>(4, up4(ss.L));
+            sd.flags = ss.flags;
+            sd.gdist = ss.gdist;
+            sd.gapcoef = st[k]->gapcoef;
+            sd.lencoef = st[k]->lencoef;
+            sd.sum2 = ss.sum2;
+            sd.col0 = colblk;
+            sd.pad = 0;
+            colblk += (ss.L + CWP_BUILD_COLS - 1) / CWP_BUILD_COLS;
+            sd.rows = row_bytes;
+            row_bytes += (int64_t)sd.n * sd.stride;
+            sd.rinfo = off;
+            off += 3 * (int64_t)sd.n;
+            sd.mat = up0 + (int64_t)mat_of[k] * NUMRES * NUMRES;
+            sd.prf = h ? lo.probs[k].prf2 : lo.probs[k].prf1;
+            sd.rec = h ? lo.probs[k].col : lo.probs[k].row;
+            sd.pl = h ? lo.probs[k].pl2 : lo.probs[k].pl1;
+        }
+    const int64_t up_sz = off - up0;
+    for (auto& sd : sides) {
+        sd.tmp = off;
+        off = up4(off + 4 * ((int64_t)sd.L + 2));
+    }
+    HIPCHK(ctx->h_prf_in.ensure(up_sz + 4));
+    HIPCHK(ctx->h_rows.ensure(row_bytes + 4));
+    int32_t* h_in = ctx->h_prf_in.p;
+    uint8_t* h_rows = ctx->h_rows.p;
+    for (size_t u = 0; u < uniq.size(); ++u)
+        std::memcpy(h_in + u * NUMRES * NUMRES, st[uniq[u]]->mat.data(), sizeof(int32_t) * NUMRES * NUMRES);
+    const bool trim = !P.use_end_gaps && !P.end_gap_penalties;
+    parallel_for(n, [&](size_t k) {
+        const Step& s = *st[k];
+        for (int h = 0; h < 2; ++h) {
+            const SideSpec& ss = h ? s.s2 : s.s1;
+            const PrfSide& sd = sides[2 * k + h];
+            int32_t* ri = h_in + (sd.rinfo - up0);
+            for (int q = 0; q < sd.n; ++q) {
+                const std::vector<uint8_t>& r = seqs[s.members[ss.first + q]];
+                uint8_t* dst = h_rows + sd.rows + (int64_t)q * sd.stride;
+                const int len = std::min((int)r.size(), sd.L);
+                std::memcpy(dst, r.data(), len);
+                std::memset(dst + len, GAP1, sd.stride - len);
+                int is = 0, ie = sd.L;                   // calcGapCoeff :129-160
+                if (trim) {
+                    while (is < sd.L && is_gap_code(dst[is])) ++is;
+                    if (is < sd.L)
+                        while (ie > 0 && is_gap_code(dst[ie - 1])) --ie;
+                    else
+                        ie = 0;
+                }
+                ri[3 * q] = s.w[ss.first + q];
+                ri[3 * q + 1] = is;
+                ri[3 * q + 2] = ie;
+            }
+        }
+    });
+    HIPCHK(ctx->p_pool.ensure(off + 4));
+    HIPCHK(ctx->p_rows.ensure(row_bytes + 4));
+    HIPCHK(ctx->p_sides.ensure(sides.size()));
+    HIPCHK(hipMemcpyAsync(ctx->p_rows.p, h_rows, row_bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->p_pool.p + up0, h_in, up_sz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->p_sides.p, sides.data(), sides.size() * sizeof(PrfSide), hipMemcpyHostToDevice,
+                          ctx->stream));
+    BuildArgs ba;
+    ba.sides = ctx->p_sides.p;
+    ba.rows = ctx->p_rows.p;
+    ba.pool = ctx->p_pool.p;
+    ba.nside = (int)sides.size();
+    ba.ncolblk = colblk;
+    penalty_tables(P, ba);
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(cwp_launch_build(colblk, (int)sides.size(), ctx->stream, ba));
+    if (!dp) return CWHIP_OK;
+    const int rc = dp_launch_collect(ctx, lo, jobs);
+    if (rc != CWHIP_OK) return rc;
+    for (int k = 0; k < n; ++k) st[k]->job = jobs[k];
+    return CWHIP_OK;
 }
 
 // progTracepath (:1023-1062), addGGaps (:818-934) and the write-back of :464-493.
@@ -838,17 +748,16 @@ void apply_step(Step& st, Seqs& seqs)
         }
     }
     const int alen = (int)path1.size();
-    const int stride = std::max(st.L1, st.L2) + 2;
     for (int k = 0; k < st.n1 + st.n2; ++k) {
         const bool g1 = k < st.n1;
         const std::vector<uint8_t>& path = g1 ? path1 : path2;
         const int rowlen = g1 ? st.L1 : st.L2;
-        const uint8_t* src = &st.padded[(size_t)k * stride];
+        const std::vector<uint8_t>& src = seqs[st.members[k]];
         std::vector<uint8_t> out(alen);
         int ix = 0;
         for (int i = 0; i < alen; ++i) {
             if (path[i] == 2) {
-                out[i] = ix < rowlen ? src[ix] : (uint8_t)ENDALN;
+                out[i] = ix < rowlen ? (uint8_t)row_at(src, ix) : (uint8_t)ENDALN;
                 ++ix;
             } else {
                 out[i] = GAP1;
@@ -856,6 +765,16 @@ void apply_step(Step& st, Seqs& seqs)
         }
         seqs[st.members[k]].swap(out);
     }
+}
+
+// residues (codes other than GAP1 / GAP2) of each row
+std::vector<int> residue_counts(const Seqs& seqs)
+{
+    std::vector<int> r(seqs.size(), 0);
+    for (size_t s = 0; s < seqs.size(); ++s)
+        for (uint8_t c : seqs[s])
+            if (c != GAP1 && c != GAP2) ++r[s];
+    return r;
 }
 
 } // namespace
@@ -906,14 +825,13 @@ int cwhip_profile_align(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     std::vector<int> w(weights, weights + nseq);
     Step st;
     for (int k = 0; k < 4; ++k) info[k] = 0;
-    const bool run = prepare_step(*p, seqs, sim, w, g, al, st, true);
+    const bool run = prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st);
     for (int i = 0; i <= nseq; ++i) group[i] = g[i];          // profileAlign mutates group
     if (!run) return CWHIP_OK;                                 // an empty group: score 0 (:96-99)
     if (rows_out && stride < st.L1 + st.L2 + 2) return fail(ctx, CWHIP_E_ARG, "row stride too small");
-    std::vector<DpJob> jobs{st.job};
-    const int rc = run_dp_batch(ctx, jobs, p->end_gap_penalties ? 1 : 0);
+    BatchLayout lo;
+    const int rc = run_built_batch(ctx, *p, seqs, {&st}, true, lo);
     if (rc != CWHIP_OK) return rc;
-    st.job = jobs[0];
     apply_step(st, seqs);
     const int alen = (int)seqs[st.members[0]].size();
     if (rows_out)
@@ -922,6 +840,39 @@ int cwhip_profile_align(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     info[1] = alen;
     info[2] = st.L1;
     info[3] = st.L2;
+    return CWHIP_OK;
+}
+
+int cwhip_step_profiles(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t* codes,
+                        const int64_t* offsets, const int32_t* lens, const double* sim,
+                        const int32_t* weights, const int32_t* group, const int32_t* aligned,
+                        int32_t* prf1, int64_t cap1, int32_t* prf2, int64_t cap2, int32_t* len12)
+{
+    if (!ctx || !p || nseq < 1 || !codes || !offsets || !lens || !sim || !weights || !group ||
+        !aligned || !len12)
+        return fail(ctx, CWHIP_E_ARG, "bad step profile arguments");
+    Seqs seqs(nseq);
+    for (int s = 0; s < nseq; ++s) {
+        if (lens[s] < 0) return fail(ctx, CWHIP_E_ARG, "negative length");
+        seqs[s].assign(codes + offsets[s], codes + offsets[s] + lens[s]);
+    }
+    std::vector<int> g(group, group + nseq + 1), al(aligned, aligned + nseq + 1);
+    std::vector<int> w(weights, weights + nseq);
+    Step st;
+    len12[0] = len12[1] = -1;
+    if (!prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st)) return CWHIP_OK;
+    len12[0] = st.L1;
+    len12[1] = st.L2;
+    const int64_t n1 = (int64_t)(st.L1 + 2) * W, n2 = (int64_t)(st.L2 + 2) * W;
+    if (!prf1 || !prf2 || cap1 < n1 || cap2 < n2) return fail(ctx, CWHIP_E_ARG, "profile buffers too small");
+    BatchLayout lo;
+    const int rc = run_built_batch(ctx, *p, seqs, {&st}, false, lo);
+    if (rc != CWHIP_OK) return rc;
+    HIPCHK(hipMemcpyAsync(prf1, ctx->p_pool.p + lo.probs[0].prf1, n1 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(prf2, ctx->p_pool.p + lo.probs[0].prf2, n2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return CWHIP_OK;
 }
 
@@ -970,6 +921,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
     const cwhip_msa_params& P = *p;
     auto SIM = [&](int a, int b) { return sim[(size_t)a * (nseq + 1) + b]; };
     std::vector<int> weights(seq_weight, seq_weight + nseq);
+    const std::vector<int> rescnt = residue_counts(seqs);
     std::vector<int> aligned(nseq + 1, 0), maxid(nseq + 1, 0);
     for (int i = 1; i <= nseq; ++i) {                     // MSA.cpp:67-78
         maxid[i] = -1;
@@ -1020,23 +972,19 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         // of few steps builds them in turn, each on all threads
         auto prep = [&](size_t k) {
             std::vector<int> grp = groups[by_level[lv][k]];
-            ok[k] = prepare_step(P, seqs, sim, weights, grp, all_aligned, sts[k], nst < 4) ? 1 : 0;
+            ok[k] = prepare_step(P, seqs, rescnt, sim, weights, grp, all_aligned, sts[k]) ? 1 : 0;
         };
         if (nst >= 4) parallel_for(nst, prep);
         else
             for (size_t k = 0; k < nst; ++k) prep(k);
-        std::vector<DpJob> jobs;
-        std::vector<int> live;
+        std::vector<Step*> live;
         for (size_t k = 0; k < nst; ++k)
-            if (ok[k]) {
-                live.push_back((int)k);
-                jobs.push_back(sts[k].job);
-            }
-        const int rc = run_dp_batch(ctx, jobs, P.end_gap_penalties ? 1 : 0);
+            if (ok[k]) live.push_back(&sts[k]);
+        BatchLayout lo;
+        const int rc = run_built_batch(ctx, P, seqs, live, true, lo);
         if (rc != CWHIP_OK) return rc;
         ++batches;
-        for (size_t q = 0; q < live.size(); ++q) sts[live[q]].job = jobs[q];
-        parallel_for(live.size(), [&](size_t q) { apply_step(sts[live[q]], seqs); });
+        parallel_for(live.size(), [&](size_t q) { apply_step(*live[q], seqs); });
         calls += (int)live.size();
     }
     // Second pass: the delayed (divergent) sequences one by one (:159-335).
@@ -1087,12 +1035,11 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         }
         aligned[iseq] = 1;
         Step st;
-        if (prepare_step(P, seqs, sim, weights, group, aligned, st, true)) {
-            std::vector<DpJob> jobs{st.job};
-            const int rc = run_dp_batch(ctx, jobs, P.end_gap_penalties ? 1 : 0);
+        if (prepare_step(P, seqs, rescnt, sim, weights, group, aligned, st)) {
+            BatchLayout lo;
+            const int rc = run_built_batch(ctx, P, seqs, {&st}, true, lo);
             if (rc != CWHIP_OK) return rc;
             ++batches;
-            st.job = jobs[0];
             apply_step(st, seqs);
             ++calls;
         }

@@ -23,7 +23,7 @@ EXPORTS = [
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning", "cwhip_get_tune_table",
     "cwhip_set_tune_table", "cwhip_autotune", "cwhip_set_graph",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
-    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_nj_tree", "cwhip_server_create",
+    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_step_profiles", "cwhip_nj_tree", "cwhip_server_create",
     "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
     "cwhip_server_text", "cwhip_server_times", "cwhip_server_release", "cwhip_server_stats", "cwhip_guide_from_tree", "cwhip_clustal_text",
 ]
@@ -120,6 +120,11 @@ def load() -> C.CDLL:
                                       P(np.float64, flags="C"), P(np.int32, flags="C"),
                                       P(np.int32, flags="C"), P(np.int32, flags="C"), vp,
                                       C.c_int64, P(np.int32, flags="C")]
+    L.cwhip_step_profiles.argtypes = [vp, C.POINTER(MsaParams), C.c_int, P(np.uint8, flags="C"),
+                                      P(np.int64, flags="C"), P(np.int32, flags="C"),
+                                      P(np.float64, flags="C"), P(np.int32, flags="C"),
+                                      P(np.int32, flags="C"), P(np.int32, flags="C"), vp, C.c_int64,
+                                      vp, C.c_int64, P(np.int32, flags="C")]
     L.cwhip_msa_stats.argtypes = [vp, P(np.float64, flags="C")]
     L.cwhip_nj_tree.argtypes = [C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
                                 C.POINTER(C.c_int64)]
@@ -411,6 +416,32 @@ def profile_align(ctx, seqs, sim, weights, group, aligned, params):
                                         rows.ctypes.data, stride, info), "cwhip_profile_align")
     out = [rows[i, :info[1]].copy() if g[i + 1] in (1, 2) and info[1] > 0 else None for i in range(n)]
     return info, g, out
+
+
+def step_profiles(ctx, seqs, sim, weights, group, aligned, params):
+    """The two int32 profiles ((len+2) x PRF_W) the device builds for one
+    profileAlign step, as (prf1, len1, prf2, len2), or None when a group is empty."""
+    n = len(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    if n > 1:
+        offs[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if n else np.zeros(0, np.uint8)
+    cap = (int(lens.max(initial=0)) + 2) * PRF_W
+    p1 = np.zeros(cap, np.int32)
+    p2 = np.zeros(cap, np.int32)
+    l12 = np.zeros(2, np.int32)
+    ctx._chk(ctx._L.cwhip_step_profiles(ctx._h, C.byref(params), n, codes, offs, lens,
+                                        np.ascontiguousarray(sim, np.float64).reshape(-1),
+                                        np.ascontiguousarray(weights, np.int32),
+                                        np.ascontiguousarray(group, np.int32),
+                                        np.ascontiguousarray(aligned, np.int32),
+                                        p1.ctypes.data, cap, p2.ctypes.data, cap, l12),
+             "cwhip_step_profiles")
+    l1, l2 = int(l12[0]), int(l12[1])
+    if l1 < 0:
+        return None
+    return (p1[:(l1 + 2) * PRF_W].reshape(-1, PRF_W), l1, p2[:(l2 + 2) * PRF_W].reshape(-1, PRF_W), l2)
 
 
 JOB_DISTANCES = 0

@@ -238,6 +238,18 @@ int cwhip_profile_align(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, con
                         const int32_t *weights, int32_t *group, const int32_t *aligned,
                         uint8_t *rows_out, int64_t stride, int32_t *info);
 
+/* The two profiles profileAlign builds for one step (MyersMillerProfileAlign.cpp:
+ * 80-300: ProfileBase::calcGapCoeff, ProfileBase.cpp:52-549; profile 1 by
+ * ProfileWithSub::calcProfileWithSub, ProfileWithSub.cpp:42-147; profile 2 by
+ * ProfileStandard::calcStandardProfile, ProfileStandard.cpp:41-117), built on the
+ * device as cwhip_msa builds them.  Arguments as cwhip_profile_align (group is not
+ * modified here); prf1/prf2 receive (len+2) x CWHIP_PRF_W ints (cap1/cap2 in ints);
+ * len12 receives len1 and len2, or -1 and -1 when a group is empty. */
+int cwhip_step_profiles(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t *codes,
+                        const int64_t *offsets, const int32_t *lens, const double *sim,
+                        const int32_t *weights, const int32_t *group, const int32_t *aligned,
+                        int32_t *prf1, int64_t cap1, int32_t *prf2, int64_t cap2, int32_t *len12);
+
 /* Copies the rows of the last cwhip_msa (sequence s at out + s*stride,
  * aln_len codes each; gaps as code 30 or 31). */
 int cwhip_msa_rows(cwhip_ctx *ctx, uint8_t *out, int64_t stride);

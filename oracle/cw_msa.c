@@ -1021,11 +1021,15 @@ int cwo_step_profiles(const cwo_msa_params *pm, int nseq, int *const *seqs, cons
         free(group);
         return 0;
     }
-    if (n2 > n1)
+    if (n2 > n1) {
         for (i = 0; i < nseq; i++) {
             if (group[i + 1] == 1) group[i + 1] = 2;
             else if (group[i + 1] == 2) group[i + 1] = 1;
         }
+        int t = n1;          /* profile 1 takes the larger group, as profile_align recounts */
+        n1 = n2;
+        n2 = t;
+    }
     double pcid = 0.0;
     int count = 0;
     for (i = 0; i < nseq; i++)
