@@ -44,7 +44,8 @@ struct PrfArgs {
 // ProfileStandard::calcStandardProfile (profile 2), written where the DP
 // kernels read them.  Offsets are in int32 units into PrfArgs::pool, except
 // `rows` (bytes into BuildArgs::rows).
-#define CWP_BUILD_COLS 256    // columns per workgroup of cw_prf_cols (one per thread)
+#define CWP_BUILD_COLS 64     // columns per workgroup of cw_prf_cols
+#define CWP_BUILD_SLICES 4    // row slices per column (CWP_BUILD_COLS x CWP_BUILD_SLICES threads)
 #define CWP_F_SUB 1           // profile with substitution matrix (else standard profile)
 #define CWP_F_DNA 2           // DNA: no residue-specific, hydrophilic or variable penalties
 #define CWP_F_VAR 4           // variable penalties (vW) of a two-row group with pcid > 60

@@ -1362,47 +1362,49 @@ __device__ static inline int f2i_host(float v)
     return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000u;
 }
 
-// One thread per column of one profile: the per-column sums over the group's
-// rows that ProfileBase::calcGapCoeff gathers (gap counts between each row's
-// first and last residue when end gaps are not counted, :129-160; hydrophilic
-// stretches of more than 3 residues, calcHydrophilicPen :398-459; residue-
-// specific penalties from the column's residue counts, calcResidueSpecificPen
-// :360-391; the +-5 window identity of a two-row group, calcVPenalties
-// :310-353), and the residue columns of the profile:
-// ProfileWithSub::calcProfileWithSub (ProfileWithSub.cpp:42-147) for profile 1,
-// ProfileStandard::calcStandardProfile (ProfileStandard.cpp:41-117) for
-// profile 2.  Column sums are integer, so the order over rows does not matter.
-extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS) cw_prf_cols(BuildArgs a)
+// The per-column sums over a group's rows that ProfileBase::calcGapCoeff
+// gathers (gap counts between each row's first and last residue when end gaps
+// are not counted, :129-160; hydrophilic stretches of more than 3 residues,
+// calcHydrophilicPen :398-459; residue-specific penalties from the column's
+// residue counts, calcResidueSpecificPen :360-391; the +-5 window identity of a
+// two-row group, calcVPenalties :310-353), and the residue columns of the
+// profile: ProfileWithSub::calcProfileWithSub (ProfileWithSub.cpp:42-147) for
+// profile 1, ProfileStandard::calcStandardProfile (ProfileStandard.cpp:41-117)
+// for profile 2.  A workgroup takes CWP_BUILD_COLS columns; each column's rows
+// are split over CWP_BUILD_SLICES threads (one per wave) whose code counts meet
+// in LDS.  All sums are integer, so the order over rows does not matter.
+extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS * CWP_BUILD_SLICES) cw_prf_cols(BuildArgs a)
 {
-    constexpr int B = CWP_BUILD_COLS;
-    __shared__ int s_ws[32 * B];     // weighted code counts, [code][column of the block]
-    __shared__ int s_cn[32 * B];     // plain code counts
-    int lo = 0, hi = a.nside - 1;    // the side of this block: the last one with col0 <= blockIdx
+    constexpr int C = CWP_BUILD_COLS, NS = CWP_BUILD_SLICES;
+    __shared__ int s_ws[NS * 32 * C];   // weighted code counts, [slice][code][column of the block]
+    __shared__ int s_cn[NS * 32 * C];   // plain code counts
+    __shared__ int s_gh[NS * 2 * C];    // gap counts and hydrophilic sums, [slice][2][column]
+    int lo = 0, hi = a.nside - 1;       // the side of this block: the last one with col0 <= blockIdx
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.sides[mid].col0 <= (int)blockIdx.x) lo = mid;
         else hi = mid - 1;
     }
     const PrfSide sd = a.sides[lo];
-    const int t = threadIdx.x;
-    const int pos = ((int)blockIdx.x - sd.col0) * B + t;
-    if (pos >= sd.L) return;                 // no barrier below: every sum is this thread's own
+    const int t = threadIdx.x % C, sl = threadIdx.x / C;
+    const int pos = ((int)blockIdx.x - sd.col0) * C + t;
+    const bool live = pos < sd.L;
     const int n = sd.n, Lp = sd.L + 2;
+    int* ws = s_ws + sl * 32 * C;
+    int* cn = s_cn + sl * 32 * C;
     for (int c = 0; c < 32; ++c) {
-        s_ws[c * B + t] = 0;
-        s_cn[c * B + t] = 0;
+        ws[c * C + t] = 0;
+        cn[c * C + t] = 0;
     }
     const uint8_t* base = a.rows + sd.rows;
     const int32_t* rinfo = a.pool + sd.rinfo;
     const bool hydp = (sd.flags & CWP_F_HYD) != 0;
     int gapc = 0, hyd = 0;
-    for (int k = 0; k < n; ++k) {
-        const uint8_t* r = base + (int64_t)k * sd.stride;
-        const int wk = rinfo[3 * k], f = rinfo[3 * k + 1], e = rinfo[3 * k + 2];
-        const int c = r[pos];
+    const int k0 = (int)((int64_t)n * sl / NS), k1 = (int)((int64_t)n * (sl + 1) / NS);
+    auto one = [&](const uint8_t* r, int c, int wk, int f, int e) {
         if (c < 32) {
-            s_ws[c * B + t] += wk;
-            s_cn[c * B + t] += 1;
+            ws[c * C + t] += wk;
+            cn[c * C + t] += 1;
         }
         if (pgap(c)) {
             if (pos >= f && pos < e) ++gapc;
@@ -1419,48 +1421,92 @@ extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS) cw_prf_cols(BuildAr
             }
             if (left + right >= 3) hyd += 100;
         }
-    }
-    int32_t* tmp = a.pool + sd.tmp;
-    int hw = 0, rw = 0, vw = 0;
-    if (hydp) hw = (int)((float)hyd / (float)n);
-    if (sd.flags & CWP_F_PREF) {
-        int w = 0;
-        for (int c = 0; c <= 24; ++c) w += s_cn[c * B + t] * a.pen[c];
-        rw = w / n;
-    }
-    if (sd.flags & CWP_F_VAR) {              // two rows
-        const uint8_t* ra = base;
-        const uint8_t* rb = base + sd.stride;
-        int tt = 0, w = 0;
-        for (int j = pos - 5; j < pos + 5; ++j) {
-            if (j < 0 || j >= sd.L) continue;
-            const int x = ra[j], y = rb[j];
-            if (pgap(x) || pgap(y)) continue;
-            w += (x == y) ? 1 : 0;
-            ++tt;
+    };
+    if (live) {
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {        // four rows' loads in flight
+            const uint8_t* r0 = base + (int64_t)k * sd.stride;
+            const uint8_t* r1 = r0 + sd.stride;
+            const uint8_t* r2 = r1 + sd.stride;
+            const uint8_t* r3 = r2 + sd.stride;
+            const int c0 = r0[pos], c1 = r1[pos], c2 = r2[pos], c3 = r3[pos];
+            const int* ri = rinfo + 3 * k;
+            one(r0, c0, ri[0], ri[1], ri[2]);
+            one(r1, c1, ri[3], ri[4], ri[5]);
+            one(r2, c2, ri[6], ri[7], ri[8]);
+            one(r3, c3, ri[9], ri[10], ri[11]);
         }
-        w += tt;
-        w = tt > 0 ? (w * 100) / (2 * tt) : 100;
-        vw = w < 50 ? 50 : w;
+        for (; k < k1; ++k) {
+            const uint8_t* r = base + (int64_t)k * sd.stride;
+            one(r, r[pos], rinfo[3 * k], rinfo[3 * k + 1], rinfo[3 * k + 2]);
+        }
     }
-    tmp[pos] = gapc;
-    tmp[Lp + pos] = hw;
-    tmp[2 * Lp + pos] = rw;
-    tmp[3 * Lp + pos] = vw;
+    s_gh[(sl * 2) * C + t] = gapc;
+    s_gh[(sl * 2 + 1) * C + t] = hyd;
+    __syncthreads();
+    // slice 0's counts become the column's: each slice adds up 8 of the 32 codes
+    for (int c = sl * 8; c < sl * 8 + 8; ++c) {
+        int w = 0, m = 0;
+        for (int q = 0; q < NS; ++q) {
+            w += s_ws[(q * 32 + c) * C + t];
+            m += s_cn[(q * 32 + c) * C + t];
+        }
+        s_ws[c * C + t] = w;
+        s_cn[c * C + t] = m;
+    }
+    gapc = hyd = 0;
+    for (int q = 0; q < NS; ++q) {
+        gapc += s_gh[(q * 2) * C + t];
+        hyd += s_gh[(q * 2 + 1) * C + t];
+    }
+    __syncthreads();
+    if (!live) return;                       // no barrier below
     int32_t* row = a.pool + sd.prf + (int64_t)(pos + 1) * CWP_PRF_W;
-    for (int c = 25; c < 30; ++c) row[c] = 0;
-    for (int c = 32; c < CWP_PRF_W; ++c) row[c] = 0;
+    if (sl == 0) {
+        int32_t* tmp = a.pool + sd.tmp;
+        int hw = 0, rw = 0, vw = 0;
+        if (hydp) hw = (int)((float)hyd / (float)n);
+        if (sd.flags & CWP_F_PREF) {
+            int w = 0;
+            for (int c = 0; c <= 24; ++c) w += s_cn[c * C + t] * a.pen[c];
+            rw = w / n;
+        }
+        if (sd.flags & CWP_F_VAR) {          // two rows
+            const uint8_t* ra = base;
+            const uint8_t* rb = base + sd.stride;
+            int tt = 0, w = 0;
+            for (int j = pos - 5; j < pos + 5; ++j) {
+                if (j < 0 || j >= sd.L) continue;
+                const int x = ra[j], y = rb[j];
+                if (pgap(x) || pgap(y)) continue;
+                w += (x == y) ? 1 : 0;
+                ++tt;
+            }
+            w += tt;
+            w = tt > 0 ? (w * 100) / (2 * tt) : 100;
+            vw = w < 50 ? 50 : w;
+        }
+        tmp[pos] = gapc;
+        tmp[Lp + pos] = hw;
+        tmp[2 * Lp + pos] = rw;
+        tmp[3 * Lp + pos] = vw;
+        for (int c = 25; c < 30; ++c) row[c] = 0;
+        for (int c = 32; c < CWP_PRF_W; ++c) row[c] = 0;
+    }
+    // the 27 residue columns (codes 0..24, 30, 31), 7 per slice
+    const int q0 = sl * 7, q1 = q0 + 7 < 27 ? q0 + 7 : 27;
     if (sd.flags & CWP_F_SUB) {
         const int32_t* mat = a.pool + sd.mat;
         if (gapc == n) {                     // an all-gap column
-            for (int res = 0; res <= 24; ++res) row[res] = mat[res * 32 + 30];
-            row[30] = mat[30 * 32 + 30];
-            row[31] = mat[31 * 32 + 30];
+            for (int q = q0; q < q1; ++q) {
+                const int res = q < 25 ? q : q + 5;
+                row[res] = mat[res * 32 + 30];        // mat[res][GAP1]
+            }
         } else {
             int wt[27];
-            for (int u = 0; u < 27; ++u) wt[u] = s_ws[(u < 25 ? u : u + 5) * B + t];
+            for (int u = 0; u < 27; ++u) wt[u] = s_ws[(u < 25 ? u : u + 5) * C + t];
             const float scale = (float)(n - gapc) / (float)n;
-            for (int q = 0; q < 27; ++q) {
+            for (int q = q0; q < q1; ++q) {
                 const int res = q < 25 ? q : q + 5;
                 int f = 0;
                 for (int u = 0; u < 27; ++u) f += wt[u] * mat[(u < 25 ? u : u + 5) * 32 + res];
@@ -1468,9 +1514,9 @@ extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS) cw_prf_cols(BuildAr
             }
         }
     } else {
-        for (int q = 0; q < 27; ++q) {
+        for (int q = q0; q < q1; ++q) {
             const int res = q < 25 ? q : q + 5;
-            row[res] = sd.sum2 == 0 ? 0 : (int)(10 * (float)s_ws[res * B + t] / (float)sd.sum2);
+            row[res] = sd.sum2 == 0 ? 0 : (int)(10 * (float)s_ws[res * C + t] / (float)sd.sum2);
         }
     }
 }
@@ -1601,7 +1647,8 @@ hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 
 hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a)
 {
-    if (ncolblk > 0) hipLaunchKernelGGL(cw_prf_cols, dim3(ncolblk), dim3(CWP_BUILD_COLS), 0, st, a);
+    if (ncolblk > 0)
+        hipLaunchKernelGGL(cw_prf_cols, dim3(ncolblk), dim3(CWP_BUILD_COLS * CWP_BUILD_SLICES), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (nside > 0) hipLaunchKernelGGL(cw_prf_gapcols, dim3(nside), dim3(256), 0, st, a);
