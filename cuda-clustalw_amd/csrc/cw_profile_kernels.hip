@@ -1226,11 +1226,21 @@ extern "C" __global__ void __launch_bounds__(256) cw_prf_score(PrfArgs a)
 }
 
 // Myers-Miller recursion of one profile pair per workgroup, level by level.
+// A subtree small enough for one wave's LDS is left to cw_prf_local: its root
+// goes on the batch's list (lane 0 of the wave appends it).
+__device__ __attribute__((noinline)) static void defer_local(const PrfArgs& a, int prob, int idx)
+{
+    if (lane_id() == 0) {
+        const int i = atomicAdd(&a.local_ctr[0], 1);
+        a.local_list[2 * i] = prob;
+        a.local_list[2 * i + 1] = idx;
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 {
     __shared__ int s_lev_b, s_lev_e, s_ctr, s_err, s_alloc;
     __shared__ int s_prog[2 * kMaxSplitStripes];
-    __shared__ __align__(16) int s_local[CWP_WAVES * CWP_LOCAL_INTS];
     const int prob = blockIdx.x;
     const PrfProb pp = a.probs[prob];
     const Prob p = make_prob(a, pp);
@@ -1286,9 +1296,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
             const int M = uni(nd[2]), N = uni(nd[3]);
             if (local_fits(M, N)) {
-                if (within == 0)
-                    local_subtree(p, nodes, idx, s_local + wave * CWP_LOCAL_INTS, &s_alloc, cap, &s_err,
-                                  a.out + 4 * prob);
+                if (within == 0) defer_local(a, prob, idx);
             } else if (N <= 0 || M <= 1) {
                 if (within == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
             } else {
@@ -1345,6 +1353,27 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         a.out[4 * prob + 1] = s_alloc;
         a.out[4 * prob + 2] = s_err;
         a.out[4 * prob + 3] = levels;
+    }
+}
+
+// The subtrees cw_prf_mm listed, each solved by one wave in its LDS
+// (local_subtree), taken from the list by the waves of all workgroups; node
+// slots come from the problem's node count, which cw_prf_mm left in out[1].
+extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_local(PrfArgs a)
+{
+    __shared__ __align__(16) int s_local[CWP_WAVES * CWP_LOCAL_INTS];
+    const int wave = threadIdx.x / WAVE;
+    const int lane = lane_id();
+    const int total = uni(a.local_ctr[0]);
+    for (;;) {
+        const int got = atomicAdd(&a.local_ctr[1], lane == 0 ? 1 : 0);
+        const int i = uni(__shfl(got, 0, WAVE));
+        if (i >= total) break;
+        const int prob = uni(a.local_list[2 * i]), idx = uni(a.local_list[2 * i + 1]);
+        const PrfProb pp = a.probs[prob];
+        const Prob p = make_prob(a, pp);
+        local_subtree(p, a.pool + pp.nodes, idx, s_local + wave * CWP_LOCAL_INTS, a.out + 4 * prob + 1,
+                      pp.node_cap, a.out + 4 * prob + 2, a.out + 4 * prob);
     }
 }
 
@@ -1642,6 +1671,12 @@ hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a)
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 {
     hipLaunchKernelGGL(cw_prf_mm, dim3(nprob), dim3(CWP_THREADS), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t cwp_launch_local(int groups, hipStream_t st, const PrfArgs& a)
+{
+    hipLaunchKernelGGL(cw_prf_local, dim3(groups), dim3(CWP_THREADS), 0, st, a);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@
 hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a);
+hipError_t cwp_launch_local(int groups, hipStream_t st, const PrfArgs& a);
 
 namespace {
 
@@ -214,6 +215,7 @@ struct BatchLayout {
     std::vector<PrfProb> probs;
     std::vector<int32_t> tile_base;
     int64_t in_sz = 0, nodes_sz = 0, total = 0;
+    int64_t list = 0, ctr = 0;       // the subtrees left to cw_prf_local, and its two counters
 };
 
 void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
@@ -250,8 +252,12 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
         pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)pp.L2 + 2 + pp.node_cap));
     }
     for (auto& pp : lo.probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
+    int64_t roots = 0;                 // a local subtree's root is a node of its problem
+    for (const auto& pp : lo.probs) roots += pp.node_cap;
     lo.in_sz = in_sz;
-    lo.total = in_sz + work_sz;
+    lo.list = in_sz + work_sz;
+    lo.ctr = up4(lo.list + 2 * roots);
+    lo.total = lo.ctr + 4;
 }
 
 // The score and Myers-Miller kernels over a batch whose inputs are in the pool
@@ -274,9 +280,13 @@ int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>&
     a.tile_base = ctx->p_tiles.p;
     a.nprob = n;
     a.rmax = ctx->prf_rmax;
+    a.local_list = ctx->p_pool.p + lo.list;
+    a.local_ctr = ctx->p_pool.p + lo.ctr;
+    HIPCHK(hipMemsetAsync(a.local_ctr, 0, 2 * sizeof(int32_t), ctx->stream));
     if (lo.tile_base[n] > 0) HIPCHK(cwp_launch_score(lo.tile_base[n], ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(cwp_launch_mm(n, ctx->stream, a));
+    HIPCHK(cwp_launch_local(std::max(1, ctx->num_cu), ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
     HIPCHK(ctx->h_prf_nodes.ensure(lo.nodes_sz + 4 * (size_t)n + 4));
     int32_t* h_nodes = ctx->h_prf_nodes.p;
