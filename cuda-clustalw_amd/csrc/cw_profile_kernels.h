@@ -5,8 +5,9 @@
 #define CWP_PRF_W 35          // ints per profile row: codes 0..31, GAPCOL 32, LENCOL 33, pad
 #define CWP_GAPCOL 32
 #define CWP_LENCOL 33
-#define CWP_WAVES 4           // waves per workgroup; one workgroup aligns one profile pair
+#define CWP_WAVES 4           // waves per cw_prf_mm workgroup; one workgroup aligns one profile pair
 #define CWP_THREADS (64 * CWP_WAVES)
+#define CWP_LOCAL_WAVES 4     // waves per cw_prf_local workgroup (CWP_LOCAL_INTS of LDS each)
 #define CWP_RMAX 8            // rows per lane the sweep is instantiated for
 #define CWP_NODE_INTS 8       // node record: A B M N flags type midj child
 #define CWP_SCR_ROWS 8        // per-wave scratch rows: 4 ping-pong + 4 final

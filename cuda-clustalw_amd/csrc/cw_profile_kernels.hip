@@ -374,6 +374,7 @@ __device__ void psweep(const PSweep& W, int rmax, const Bufs& g, int* finH, int*
 // overwritten before it has been read.  prog[j] counts stripe j's flushed
 // chunks.
 constexpr int kMaxSplitStripes = 64;
+constexpr int kMaxSplitNodes = CWP_WAVES / 4;    // nodes of a level whose halves can be split
 
 template <int MODE>
 __device__ void psweep_part(const PSweep& W, int rmax, int m, int S, int* const* regions, int rowcap,
@@ -1240,7 +1241,7 @@ __device__ __attribute__((noinline)) static void defer_local(const PrfArgs& a, i
 extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 {
     __shared__ int s_lev_b, s_lev_e, s_ctr, s_err, s_alloc;
-    __shared__ int s_prog[2 * kMaxSplitStripes];
+    __shared__ int s_prog[kMaxSplitNodes * 2 * kMaxSplitStripes];
     const int prob = blockIdx.x;
     const PrfProb pp = a.probs[prob];
     const Prob p = make_prob(a, pp);
@@ -1266,23 +1267,29 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         const int lb = s_lev_b, le = s_lev_e;
         if (lb >= le) break;
         __syncthreads();
-        for (int k = threadIdx.x; k < 2 * kMaxSplitStripes; k += blockDim.x) s_prog[k] = 0;
+        for (int k = threadIdx.x; k < kMaxSplitNodes * 2 * kMaxSplitStripes; k += blockDim.x) s_prog[k] = 0;
         if (threadIdx.x == 0) s_ctr = 0;
         __syncthreads();
-        // A level of one large node gives each half to S = 2 waves (pipelined
-        // stripes), so all four waves work on it.
+        // A level of at most kMaxSplitNodes large nodes gives each half to S
+        // waves (pipelined stripes), S = CWP_WAVES / (2 x nodes) at most, so
+        // every task has a wave of its own and all waves work.  Every stripe
+        // keeps >= 64 rows (a stripe of one row per lane must fill the wave):
+        // the smaller half has >= 64 S rows, and a stripe holds up to 128 or
+        // more.
         int S = 1;
-        if (le - lb == 1) {
-            const int* nd0 = nodes + (int64_t)lb * CWP_NODE_INTS;
-            const int M0 = uni(nd0[2]), N0 = uni(nd0[3]);
-            const int per = WAVE * a.rmax;
-            const int half_rows = M0 - M0 / 2;
-            // every stripe keeps >= 64 rows (a stripe of one row per lane must
-            // fill the wave): both halves have >= 128 rows and a stripe holds
-            // up to 128 or more
-            if (!local_fits(M0, N0) && N0 >= 1 && M0 / 2 >= 2 * WAVE && a.rmax >= 2 &&
-                (half_rows + per - 1) / per <= kMaxSplitStripes)
-                S = 2;
+        if (le - lb <= kMaxSplitNodes) {
+            int sk = CWP_WAVES / (2 * (le - lb));
+            for (int k = lb; k < le; ++k) {
+                const int* ndk = nodes + (int64_t)k * CWP_NODE_INTS;
+                const int M0 = uni(ndk[2]), N0 = uni(ndk[3]);
+                const int per = WAVE * a.rmax;
+                const int half_rows = M0 - M0 / 2;
+                int fit = (M0 / 2) / WAVE;
+                if (local_fits(M0, N0) || N0 < 1 || a.rmax < 2 || (half_rows + per - 1) / per > kMaxSplitStripes)
+                    fit = 1;
+                sk = sk < fit ? sk : fit;
+            }
+            if (sk >= 2) S = sk;
         }
         // tasks 2S k .. 2S k + 2S - 1: the halves of node lb+k, S members each
         // (a local subtree or a leaf is one task)
@@ -1300,11 +1307,15 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             } else if (N <= 0 || M <= 1) {
                 if (within == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
             } else {
-                int* regions[2];
-                for (int r = 0; r < 2; ++r)
-                    regions[r] = a.pool + pp.scratch + (int64_t)(2 * half + r) * CWP_SCR_ROWS * pp.rowcap;
+                // a split half's members use the scratch of waves
+                // (2 (idx - lb) + half) S .. + S - 1 (no wave sweeps alone then)
+                const int grp = 2 * (idx - lb) + half;
+                int* regions[CWP_WAVES / 2];
+                for (int r = 0; r < CWP_WAVES / 2; ++r)
+                    regions[r] = a.pool + pp.scratch +
+                                 (int64_t)((grp * S + r) % CWP_WAVES) * CWP_SCR_ROWS * pp.rowcap;
                 do_half(p, nd, half, a.rmax, scr, pp.rowcap, level_bufs(lvl, lcap, uni(nd[1]), idx - lb), m,
-                        S, regions, s_prog + half * kMaxSplitStripes, &s_err);
+                        S, regions, s_prog + (S > 1 ? grp : 0) * kMaxSplitStripes, &s_err);
             }
         }
         __syncthreads();
@@ -1359,9 +1370,9 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
 // The subtrees cw_prf_mm listed, each solved by one wave in its LDS
 // (local_subtree), taken from the list by the waves of all workgroups; node
 // slots come from the problem's node count, which cw_prf_mm left in out[1].
-extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_local(PrfArgs a)
+extern "C" __global__ void __launch_bounds__(64 * CWP_LOCAL_WAVES) cw_prf_local(PrfArgs a)
 {
-    __shared__ __align__(16) int s_local[CWP_WAVES * CWP_LOCAL_INTS];
+    __shared__ __align__(16) int s_local[CWP_LOCAL_WAVES * CWP_LOCAL_INTS];
     const int wave = threadIdx.x / WAVE;
     const int lane = lane_id();
     const int total = uni(a.local_ctr[0]);
@@ -1676,7 +1687,7 @@ hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 
 hipError_t cwp_launch_local(int groups, hipStream_t st, const PrfArgs& a)
 {
-    hipLaunchKernelGGL(cw_prf_local, dim3(groups), dim3(CWP_THREADS), 0, st, a);
+    hipLaunchKernelGGL(cw_prf_local, dim3(groups), dim3(64 * CWP_LOCAL_WAVES), 0, st, a);
     return hipGetLastError();
 }
 
