@@ -1,0 +1,260 @@
+// cw_fast_kernels.hip -- gfx950 kernel of ClustalW's k-tuple distances
+// (pairwise/FastPairwiseAlign.cpp, the -QUICKTREE path).
+//
+// The reference's pairAlign is a sequential walk per pair: k-tuple matches
+// counted per diagonal, a quicksort of the diagonal counts whose tie order
+// decides which diagonals are "top", then a fragment list kept in score order
+// that every match on a flagged diagonal walks.  Pairs are independent, so one
+// lane runs one pair from start to end and the grid's lanes take pairs from a
+// counter.  A lane's arrays (positions per k-tuple, diagonals, fragments) live
+// in HBM interleaved by lane (element k of lane g at k * lanes + g), so the
+// loops every lane runs in step (clearing, counting positions) coalesce; the
+// data-dependent walks are latency-bound.  All arithmetic is integer except the
+// final double distance, which is IEEE on both sides.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cw_fast_kernels.h"
+
+namespace {
+
+// FastPairwiseAlign.cpp:147-168: protein codes A, C, G, T, U to DNA 0..3;
+// every other code breaks a k-tuple
+__device__ __forceinline__ int dna_of(int c)
+{
+    switch (c) {
+    case 0: return 0;
+    case 2: return 1;
+    case 6: return 2;
+    case 18: case 19: return 3;
+    default: return -1;
+    }
+}
+
+struct Lane {
+    int32_t* sc;
+    int64_t G, g;
+    __device__ __forceinline__ int32_t& at(int64_t arr, int64_t k) const { return sc[(arr + k) * G + g]; }
+};
+
+// residue code of 1-based position p of a sequence (0-based packed codes)
+__device__ __forceinline__ int res_at(const FastArgs& a, const uint8_t* s, int p)
+{
+    const int c = s[p - 1];
+    if (a.dna) return dna_of(c);
+    return c;
+}
+
+// makePPtrs / makeNPtrs (:404-521)
+__device__ void make_ptrs(const FastArgs& a, const Lane& ln, int64_t tptr, int64_t pl, const uint8_t* s,
+                          int length)
+{
+    const int mr = a.dna ? 3 : 24;
+    for (int i = 1; i <= a.limit; ++i) ln.at(pl, i) = 0;
+    for (int i = 1; i <= length; ++i) ln.at(tptr, i) = 0;
+    for (int i = 1; i <= length - a.ktup + 1; ++i) {
+        int code = 0, pot = 1;
+        bool bad = false;
+        for (int j = 1; j <= a.ktup; ++j) {
+            const int r = res_at(a, s, i + j - 1);
+            if (r < 0 || r > mr) {
+                bad = true;
+                break;
+            }
+            code += r * pot;
+            pot *= mr + 1;
+        }
+        if (bad) continue;
+        ++code;
+        const int prev = ln.at(pl, code);
+        if (prev != 0) ln.at(tptr, i) = prev;
+        ln.at(pl, code) = i;
+    }
+}
+
+// desQuickSort (:584-659) over elements 1 .. n-1 (element n stays in place)
+__device__ void des_quick_sort(const FastArgs& a, const Lane& ln, int n)
+{
+    int lst[48], ust[48];
+    int p = 1;
+    lst[1] = 1;
+    ust[1] = n - 1;
+    while (p > 0) {
+        if (lst[p] >= ust[p]) {
+            p--;
+            continue;
+        }
+        int i = lst[p] - 1, j = ust[p];
+        const int piv = ln.at(a.o_displ, j);
+        while (i < j) {
+            for (i = i + 1; ln.at(a.o_displ, i) < piv; i++)
+                ;
+            for (j = j - 1; j > i; j--)
+                if (ln.at(a.o_displ, j) <= piv) break;
+            if (i < j) {
+                int t = ln.at(a.o_displ, i);
+                ln.at(a.o_displ, i) = ln.at(a.o_displ, j);
+                ln.at(a.o_displ, j) = t;
+                t = ln.at(a.o_diag, i);
+                ln.at(a.o_diag, i) = ln.at(a.o_diag, j);
+                ln.at(a.o_diag, j) = t;
+            }
+        }
+        j = ust[p];
+        int t = ln.at(a.o_displ, i);
+        ln.at(a.o_displ, i) = ln.at(a.o_displ, j);
+        ln.at(a.o_displ, j) = t;
+        t = ln.at(a.o_diag, i);
+        ln.at(a.o_diag, i) = ln.at(a.o_diag, j);
+        ln.at(a.o_diag, j) = t;
+        if (i - lst[p] < ust[p] - i) {
+            lst[p + 1] = lst[p];
+            ust[p + 1] = i - 1;
+            lst[p] = i + 1;
+        } else {
+            lst[p + 1] = i + 1;
+            ust[p + 1] = ust[p];
+            ust[p] = i - 1;
+        }
+        p = p + 1;
+    }
+}
+
+// pairAlign (:171-402) followed by the score of :87-100; returns the score
+// (accum[0][maxSoFar], 0 when no fragment) through *score_out.
+__device__ void pair_align(const FastArgs& a, const Lane& ln, const uint8_t* s1, int l1, int l2, int* score_out)
+{
+    const int mr = a.dna ? 3 : 24;
+    const int tl1 = l1 + l2 - 1;
+    for (int i = 1; i <= tl1; ++i) {
+        ln.at(a.o_slopes, i) = 0;
+        ln.at(a.o_displ, i) = 0;
+        ln.at(a.o_diag, i) = i;
+    }
+    for (int c = 1; c <= a.limit; ++c)                  // k-tuple matches per diagonal
+        for (int vn1 = ln.at(a.o_zzc, c); vn1; vn1 = ln.at(a.o_zza, vn1))
+            for (int vn2 = ln.at(a.o_zzd, c); vn2; vn2 = ln.at(a.o_zzb, vn2)) ++ln.at(a.o_displ, vn1 - vn2 + l2);
+    des_quick_sort(a, ln, tl1);
+    int j = tl1 - a.signif + 1;
+    if (j < 1) j = 1;
+    for (int i = tl1; i >= j; i--)                      // WINDOW around the top SIGNIF diagonals
+        if (ln.at(a.o_displ, i) > 0) {
+            const int pos = ln.at(a.o_diag, i);
+            const int l = 1 > pos - a.window ? 1 : pos - a.window;
+            const int m = tl1 < pos + a.window ? tl1 : pos + a.window;
+            for (int q = l; q <= m; q++) ln.at(a.o_slopes, q) = 1;
+        }
+    for (int i = 1; i <= tl1; ++i) ln.at(a.o_displ, i) = 0;
+    int curr = 0, maxsofar = 0;
+    // accum rows 0 (score), 1 and 2 (start in sequence 1 and 2) and 4 (next in
+    // score order); row 3 (the predecessor) is written but never read by the
+    // reference and is left out
+    const int64_t A0 = a.o_acc, A1 = A0 + a.acc_len, A2 = A1 + a.acc_len, A4 = A2 + a.acc_len;
+    ln.at(A0, 0) = 0;        // the score putFrag's walk compares with at the end of the list
+    for (int i = 1; i <= l1 - a.ktup + 1; ++i) {
+        int encrypt = 0, pot = 1;
+        bool bad = false;
+        for (int q = 1; q <= a.ktup; ++q) {
+            const int r = res_at(a, s1, i + q - 1);
+            if (r < 0 || r > mr) {
+                bad = true;
+                break;
+            }
+            encrypt += r * pot;
+            pot *= mr + 1;
+        }
+        if (bad) continue;
+        ++encrypt;
+        for (int vn2 = ln.at(a.o_zzd, encrypt); vn2; vn2 = ln.at(a.o_zzb, vn2)) {
+            const int osptr = i - vn2 + l2;
+            if (ln.at(a.o_slopes, osptr) != 1) continue;
+            int fs = a.ktup, next = maxsofar;
+            for (;;) {                                  // A-loop (:299-393)
+                if (!next) {
+                    ++curr;
+                    if (curr >= 2 * a.max_aln) {        // "(Partial alignment)": stop here
+                        *score_out = maxsofar ? ln.at(A0, maxsofar) : 0;
+                        return;
+                    }
+                    ln.at(a.o_displ, osptr) = curr;
+                    // putFrag (:523-559): into the list kept in descending score order
+                    ln.at(A0, curr) = fs;
+                    ln.at(A1, curr) = i;
+                    ln.at(A2, curr) = vn2;
+                    if (!maxsofar) {
+                        maxsofar = 1;
+                        ln.at(A4, curr) = 0;
+                    } else if (fs >= ln.at(A0, maxsofar)) {
+                        ln.at(A4, curr) = maxsofar;
+                        maxsofar = curr;
+                    } else {
+                        int nx = maxsofar, end;
+                        for (;;) {
+                            end = nx;
+                            nx = ln.at(A4, nx);
+                            if (fs >= ln.at(A0, nx)) break;
+                        }
+                        ln.at(A4, curr) = nx;
+                        ln.at(A4, end) = curr;
+                    }
+                    break;
+                }
+                const int t1 = ln.at(A1, next), t2 = ln.at(A2, next);
+                bool rel;                               // fragRelPos (:561-582)
+                if (i - vn2 == t1 - t2) rel = t1 < i;
+                else rel = t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2;
+                if (!rel) {
+                    next = ln.at(A4, next);
+                    continue;
+                }
+                if (i - vn2 == t1 - t2) {
+                    if (i > t1 + (a.ktup - 1)) fs = ln.at(A0, next) + a.ktup;
+                    else fs = ln.at(A0, next) + (i - t1);
+                } else {
+                    const int d = ln.at(a.o_displ, osptr);
+                    int subt1;
+                    if (d == 0) subt1 = a.ktup;
+                    else if (i > ln.at(A1, d) + (a.ktup - 1)) subt1 = ln.at(A0, d) + a.ktup;
+                    else subt1 = ln.at(A0, d) + (i - ln.at(A1, d));
+                    const int subt2 = ln.at(A0, next) - a.wind_gap + a.ktup;
+                    fs = subt2 > subt1 ? subt2 : subt1;
+                }
+                next = 0;
+            }
+        }
+    }
+    *score_out = maxsofar ? ln.at(A0, maxsofar) : 0;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.lanes) return;
+    const Lane ln{a.scratch, a.lanes, g};
+    for (;;) {
+        const int64_t k = atomicAdd(a.next, 1u);
+        if (k >= a.npairs) break;
+        const int si = a.pairs[2 * k], sj = a.pairs[2 * k + 1];
+        const int l1 = a.lens[si], l2 = a.lens[sj];
+        const uint8_t* s1 = a.codes + a.offs[si];
+        const uint8_t* s2 = a.codes + a.offs[sj];
+        make_ptrs(a, ln, a.o_zza, a.o_zzc, s1, l1);
+        make_ptrs(a, ln, a.o_zzb, a.o_zzd, s2, l2);
+        int sc = 0;                                     // a fragment scores >= ktup, so 0 = none
+        pair_align(a, ln, s1, l1, l2, &sc);
+        double score = 0.0;                             // FastPairwiseAlign.cpp:87-100
+        if (sc != 0) {
+            score = (double)sc;
+            if (a.percent) score = (score / (double)(l1 < l2 ? l1 : l2)) * 100.0;
+        }
+        a.out_dist[k] = (100.0 - score) / 100.0;
+    }
+}
+
+hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a)
+{
+    hipLaunchKernelGGL(cw_fast_pairs, dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}

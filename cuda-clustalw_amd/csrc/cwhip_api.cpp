@@ -96,6 +96,8 @@ void cwhip_destroy(cwhip_ctx* ctx)
     ctx->d_bins.release();
     ctx->p_pool.release(); ctx->p_out.release(); ctx->p_tiles.release(); ctx->p_probs.release();
     ctx->h_prf_in.release(); ctx->h_prf_nodes.release();
+    ctx->p_rows.release(); ctx->p_sides.release(); ctx->h_rows.release();
+    ctx->f_scratch.release(); ctx->f_pairs.release(); ctx->f_dist.release(); ctx->f_ctr.release();
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -1,0 +1,114 @@
+// cwhip_fast.cpp -- host side of the k-tuple distances (cwhip_fast_pairwise_range).
+//
+// FastPairwiseAlign::pairwiseAlign (pairwise/FastPairwiseAlign.cpp:20-145):
+// the pairs of the range in the reference's order, one lane of cw_fast_pairs
+// per pair, the distances back in that order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "cw_fast_kernels.h"
+#include "cwhip_internal.h"
+
+extern "C" {
+
+int cwhip_fast_params_default(int dna, cwhip_fast_params* fp)
+{
+    if (!fp) return CWHIP_E_ARG;
+    std::memset(fp, 0, sizeof *fp);
+    fp->dna = dna ? 1 : 0;                     // UserParameters.cpp:86-94
+    fp->ktup = dna ? 2 : 1;
+    fp->window = dna ? 4 : 5;
+    fp->wind_gap = dna ? 5 : 3;
+    fp->signif = dna ? 4 : 5;
+    fp->percent = 1;
+    fp->max_aln = 0;
+    return CWHIP_OK;
+}
+
+int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i_start, int i_end,
+                              int j_start, int j_end, int32_t* out_pairs, double* out_dist,
+                              int64_t* out_npairs)
+{
+    if (!ctx || !fp || !out_npairs) return fail(ctx, CWHIP_E_ARG, "bad k-tuple arguments");
+    // the reference's fatal range checks (:27-37), and indices within the loaded sequences
+    if (i_start < 0 || i_end < i_start || j_start < 0 || j_end < j_start || i_end > ctx->nseq ||
+        j_end > ctx->nseq)
+        return fail(ctx, CWHIP_E_ARG, "the range for pairwise alignment is incorrect");
+    const int maxk = fp->dna ? 4 : 2;
+    if (fp->ktup < 1 || fp->ktup > maxk || fp->window < 0 || fp->signif < 1)
+        return fail(ctx, CWHIP_E_ARG, "bad k-tuple parameters");
+    std::vector<int32_t> pairs;
+    for (int i = i_start + 1; i <= i_end; ++i)                      // 1-based, :64-80
+        for (int j = std::max(j_start + 2, i + 1); j <= j_end; ++j) {
+            pairs.push_back(i - 1);
+            pairs.push_back(j - 1);
+        }
+    const int64_t np = (int64_t)pairs.size() / 2;
+    *out_npairs = np;
+    if (out_pairs && np) std::memcpy(out_pairs, pairs.data(), pairs.size() * sizeof(int32_t));
+    if (np == 0) return CWHIP_OK;
+    if (!out_dist) return fail(ctx, CWHIP_E_ARG, "null distance buffer");
+    HIPCHK(hipSetDevice(ctx->device));
+    int lmax = 0;
+    for (int s = 0; s < ctx->nseq; ++s) lmax = std::max(lmax, (int)ctx->h_lens[s]);
+    const int max_aln = fp->max_aln > 0 ? fp->max_aln : 2 * lmax;  // Alignment::getMaxAlnLength
+    int limit = 1;
+    for (int q = 0; q < fp->ktup; ++q) limit *= fp->dna ? 4 : 25;
+    FastArgs a;
+    a.dna = fp->dna;
+    a.ktup = fp->ktup;
+    a.window = fp->window;
+    a.wind_gap = fp->wind_gap;
+    a.signif = fp->signif;
+    a.percent = fp->percent ? 1 : 0;
+    a.max_aln = max_aln;
+    a.limit = limit;
+    // per-lane rows; diagonals 1 .. l1 + l2 - 1, fragments 1 .. 2 max_aln - 1
+    const int64_t dnum = std::max<int64_t>(2 * (int64_t)max_aln + 1, 2 * (int64_t)lmax + 2);
+    int64_t rows = 0;
+    a.o_zza = rows; rows += lmax + 2;
+    a.o_zzb = rows; rows += lmax + 2;
+    a.o_zzc = rows; rows += limit + 2;
+    a.o_zzd = rows; rows += limit + 2;
+    a.o_displ = rows; rows += dnum;
+    a.o_slopes = rows; rows += dnum;
+    a.o_diag = rows; rows += dnum;
+    a.acc_len = 2 * (int64_t)max_aln + 2;
+    a.o_acc = rows; rows += 4 * a.acc_len;
+    // lanes: up to one per wave slot of the chip, within 8 GiB of scratch
+    const int64_t cap_lanes = std::max<int64_t>(256, ((int64_t)8 << 30) / (rows * 4) / 256 * 256);
+    int64_t lanes = std::min<int64_t>((int64_t)std::max(1, ctx->num_cu) * 4 * 64, cap_lanes);
+    lanes = std::min<int64_t>(lanes, (np + 255) / 256 * 256);
+    a.lanes = lanes;
+    HIPCHK(ctx->f_scratch.ensure((size_t)(rows * lanes)));
+    HIPCHK(ctx->f_pairs.ensure(pairs.size()));
+    HIPCHK(ctx->f_dist.ensure((size_t)np));
+    HIPCHK(ctx->f_ctr.ensure(1));
+    HIPCHK(hipMemcpyAsync(ctx->f_pairs.p, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->f_ctr.p, 0, sizeof(unsigned int), ctx->stream));
+    a.codes = ctx->d_codes.p;
+    a.offs = ctx->d_offs.p;
+    a.lens = ctx->d_lens.p;
+    a.pairs = ctx->f_pairs.p;
+    a.npairs = np;
+    a.out_dist = ctx->f_dist.p;
+    a.scratch = ctx->f_scratch.p;
+    a.next = ctx->f_ctr.p;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out_dist, ctx->f_dist.p, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->fast_ms = ms;
+    return CWHIP_OK;
+}
+
+float cwhip_fast_last_ms(const cwhip_ctx* ctx) { return ctx ? ctx->fast_ms : 0.f; }
+
+}  // extern "C"

@@ -74,6 +74,10 @@ struct cwhip_ctx {
     DevBuf<int32_t> p_pool, p_out, p_tiles;
     DevBuf<PrfProb> p_probs;
     PinBuf<int32_t> h_prf_in, h_prf_nodes;      // staging of a profile batch's inputs and node tables
+    DevBuf<int32_t> f_scratch, f_pairs;         // k-tuple distances: per-lane arrays, pair list
+    DevBuf<double> f_dist;
+    DevBuf<unsigned int> f_ctr;
+    float fast_ms = 0.f;
     DevBuf<uint8_t> p_rows;                     // group rows of a profile batch
     DevBuf<PrfSide> p_sides;                    // its profiles to build
     PinBuf<uint8_t> h_rows;                     // staging of the rows

@@ -6,6 +6,7 @@ workloads and reads FASTA.
 """
 from . import seqgen  # noqa: F401
 from .lib import (KERNEL_BATCHED, KERNEL_PLAIN, Context, CwhipError,  # noqa: F401
-                  MsaParams, TuneTable, clustal_text, distance, guide_from_tree, load,
+                  FastParams, MsaParams, TuneTable, clustal_text, distance, fast_params,
+                  guide_from_tree, load,
                   JOB_ALIGNMENT, JOB_DISTANCES, Server, msa_params, nj_tree,
                   pair_params)

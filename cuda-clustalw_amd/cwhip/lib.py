@@ -23,7 +23,8 @@ EXPORTS = [
     "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning", "cwhip_get_tune_table",
     "cwhip_set_tune_table", "cwhip_autotune", "cwhip_set_graph",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
-    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_step_profiles", "cwhip_nj_tree", "cwhip_server_create",
+    "cwhip_msa_stats", "cwhip_profile_align", "cwhip_step_profiles", "cwhip_fast_params_default",
+    "cwhip_fast_pairwise_range", "cwhip_fast_last_ms", "cwhip_nj_tree", "cwhip_server_create",
     "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
     "cwhip_server_text", "cwhip_server_times", "cwhip_server_release", "cwhip_server_stats", "cwhip_guide_from_tree", "cwhip_clustal_text",
 ]
@@ -45,6 +46,12 @@ class MsaParams(C.Structure):
                 ("transition_weight", C.c_float), ("neg_matrix", C.c_int32),
                 ("no_weights", C.c_int32), ("divergence_cutoff", C.c_int32),
                 ("hyd_residues", C.c_char_p)]
+
+
+class FastParams(C.Structure):
+    _fields_ = [("dna", C.c_int32), ("ktup", C.c_int32), ("window", C.c_int32),
+                ("wind_gap", C.c_int32), ("signif", C.c_int32), ("percent", C.c_int32),
+                ("max_aln", C.c_int32)]
 
 
 class PairParams(C.Structure):
@@ -126,6 +133,11 @@ def load() -> C.CDLL:
                                       P(np.int32, flags="C"), P(np.int32, flags="C"), vp, C.c_int64,
                                       vp, C.c_int64, P(np.int32, flags="C")]
     L.cwhip_msa_stats.argtypes = [vp, P(np.float64, flags="C")]
+    L.cwhip_fast_params_default.argtypes = [C.c_int, C.POINTER(FastParams)]
+    L.cwhip_fast_pairwise_range.argtypes = [vp, C.POINTER(FastParams), C.c_int, C.c_int, C.c_int,
+                                            C.c_int, vp, vp, C.POINTER(C.c_int64)]
+    L.cwhip_fast_last_ms.restype = C.c_float
+    L.cwhip_fast_last_ms.argtypes = [vp]
     L.cwhip_nj_tree.argtypes = [C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
                                 C.POINTER(C.c_int64)]
     L.cwhip_guide_from_tree.argtypes = [C.c_int, C.c_char_p, vp, P(np.int32, flags="C"),
@@ -252,6 +264,24 @@ class Context:
                                                j_end, kernel, pr.ctypes.data, cnt.ctypes.data,
                                                dist.ctypes.data), "cwhip_pairwise_range")
         return pr[:2 * n].reshape(-1, 2), cnt[:n], dist[:n]
+
+    def fast_pairwise_range(self, i_start, i_end, j_start, j_end, dna=False, **kw):
+        """k-tuple distances (FastPairwiseAlign) of the loaded sequences over the
+        reference's 1-based range rules: (pairs int32 [k, 2], dist float64 [k])."""
+        fp = fast_params(dna, **kw)
+        n = len(self._lens)
+        cap = max(1, sum(max(0, j_end - max(j_start + 2, i + 1) + 1)
+                         for i in range(max(i_start, 0) + 1, min(i_end, n) + 1)))
+        pr = np.zeros(2 * cap, np.int32)
+        dist = np.zeros(cap, np.float64)
+        k = C.c_int64()
+        self._chk(self._L.cwhip_fast_pairwise_range(self._h, C.byref(fp), i_start, i_end, j_start,
+                                                    j_end, pr.ctypes.data, dist.ctypes.data,
+                                                    C.byref(k)), "cwhip_fast_pairwise_range")
+        return pr[:2 * k.value].reshape(-1, 2), dist[:k.value]
+
+    def fast_last_ms(self):
+        return float(self._L.cwhip_fast_last_ms(self._h))
 
     # --- device-resident timing path -------------------------------------
     def stage(self, params, pairs):
@@ -543,6 +573,17 @@ def msa_params(dna: bool = False, **kw) -> MsaParams:
     rc = load().cwhip_msa_params_default(1 if dna else 0, C.byref(p))
     if rc:
         raise CwhipError("cwhip_msa_params_default failed: %d" % rc)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def fast_params(dna: bool = False, **kw) -> FastParams:
+    """The reference's k-tuple defaults (UserParameters.cpp:86-94)."""
+    p = FastParams()
+    rc = load().cwhip_fast_params_default(1 if dna else 0, C.byref(p))
+    if rc:
+        raise CwhipError("cwhip_fast_params_default failed: %d" % rc)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

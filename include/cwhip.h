@@ -18,6 +18,8 @@
  *                            parameters of FullPairwiseAlign.cpp:129-148
  *                            (substitutionMatrix/SubMatrix.cpp:256-390, 689-829)
  *   cwhip_distance        <- the float distance of FullPairwiseAlign.cpp:170-181
+ *   cwhip_fast_pairwise_range <- FastPairwiseAlign::pairwiseAlign (k-tuple
+ *                            distances, pairwise/FastPairwiseAlign.cpp:20-145)
  * The GPU variant of the reference exposes the same operation as the free
  * function cudaFullPairwiseAlign(Alignment*, DistMatrix*, int,int,int,int)
  * (GPU/cuda_clustalw_final/src/cuda/cudaFullPairwiseAlign.h:20).
@@ -105,6 +107,40 @@ int cwhip_pairwise_range(cwhip_ctx *ctx, const cwhip_pair_params *p, int i_start
 /* Same for an explicit list of 0-based (si, sj) pairs. */
 int cwhip_pairwise_pairs(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pairs,
                          int64_t npairs, int kernel, int32_t *out_count, float *out_dist);
+
+/* ---- k-tuple distances (the -QUICKTREE path) ------------------------------
+ * FastPairwiseAlign::pairwiseAlign(Alignment*, DistMatrix*, int iStart,
+ * int iEnd, int jStart, int jEnd) (pairwise/FastPairwiseAlign.cpp:20-145,
+ * declared pairwise/FastPairwiseAlign.h, the same PairwiseAlignBase.h:19-20
+ * interface).  Parameters as UserParameters keeps them for the k-tuple stage
+ * (ktup, window, windowGap, signif, percent; general/UserParameters.cpp:86-94,
+ * 501-532); max_aln is Alignment::getMaxAlnLength (0: twice the longest loaded
+ * sequence, what the reference has after reading the sequences). */
+typedef struct {
+    int32_t dna;
+    int32_t ktup;        /* protein 1..2, DNA 1..4 */
+    int32_t window, wind_gap, signif;
+    int32_t percent;     /* score as percent of the shorter length */
+    int32_t max_aln;
+} cwhip_fast_params;
+
+/* The reference defaults: protein ktup 1, window 5, window gap 3, 5 top
+ * diagonals; DNA 2, 4, 5, 4; percent on. */
+int cwhip_fast_params_default(int dna, cwhip_fast_params *fp);
+
+/* k-tuple distances of the loaded sequences over the reference's range:
+ * sequences i = i_start+1 .. i_end against j = max(j_start+2, i+1) .. j_end
+ * (1-based, :64-80), in that order; *out_npairs receives the count, out_pairs
+ * (2 ints per pair, 0-based, may be NULL) and out_dist (the double distance
+ * (100 - score) / 100 the reference stores with DistMatrix::SetAt) one entry
+ * per pair.  Returns CWHIP_E_ARG for the reference's fatal range checks and for
+ * a range beyond the loaded sequences. */
+int cwhip_fast_pairwise_range(cwhip_ctx *ctx, const cwhip_fast_params *fp, int i_start, int i_end,
+                              int j_start, int j_end, int32_t *out_pairs, double *out_dist,
+                              int64_t *out_npairs);
+
+/* Kernel time (ms) of the last cwhip_fast_pairwise_range. */
+float cwhip_fast_last_ms(const cwhip_ctx *ctx);
 
 /* Distance of FullPairwiseAlign.cpp:170-181 from an identity count and the
  * two gap-free lengths (IEEE single arithmetic, as the reference). */

@@ -159,6 +159,15 @@ enum { CWO_GUIDE_TOO_FEW = -1, CWO_GUIDE_FORMAT = -2, CWO_GUIDE_MISMATCH = -3 };
 int cwo_guide_from_dnd(const char *dnd, int n, const char **names, int *weights,
                        double *sim, int **steps, int *nsteps);
 
+/* FastPairwiseAlign::pairwiseAlign (pairwise/FastPairwiseAlign.cpp:20-145),
+ * the k-tuple distances: sequences i = iStart+1 .. iEnd against
+ * j = max(jStart+2, i+1) .. jEnd (1-based), seqs 0-based code arrays.
+ * max_aln is Alignment::getMaxAlnLength (2 x the longest sequence).  pairs
+ * (2 per pair, 0-based) and dist may be NULL.  Returns the pair count, or -1. */
+long cwo_fast_pairwise(int nseq, const int *const *seqs, const int *lens, int dna, int ktup,
+                       int window, int wind_gap, int signif, int percent, int max_aln, int iStart,
+                       int iEnd, int jStart, int jEnd, int *pairs, double *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,9 @@ def lib():
                                         C.c_int, C.c_int, C.c_int, C.c_int,
                                         P(np.int32, flags="C"), P(np.float32, flags="C"),
                                         C.c_void_p]
+        L.cwo_fast_pairwise.restype = C.c_long
+        L.cwo_fast_pairwise.argtypes = [C.c_int, C.c_void_p, P(np.int32, flags="C")] + \
+            [C.c_int] * 11 + [C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -353,3 +356,38 @@ def prog_diff_trace(prf1, len1, prf2, len2, end_gap_penalties=0):
     lib().cwo_prog_diff_trace(a, len1, b, len2, end_gap_penalties, displ, C.byref(nd),
                               C.byref(st), tr, cap, C.byref(nt))
     return tr[:8 * min(nt.value, cap)].reshape(-1, 8)
+
+
+def fast_params(dna: bool = False, **kw):
+    """The reference's k-tuple defaults (UserParameters.cpp:86-94): protein
+    ktup 1, window 5, window gap 3, 5 top diagonals; DNA 2, 4, 5, 4; percent."""
+    p = dict(ktup=2, window=4, wind_gap=5, signif=4) if dna else \
+        dict(ktup=1, window=5, wind_gap=3, signif=5)
+    p["percent"] = 1
+    p.update(kw)
+    return p
+
+
+def fast_pairwise(seqs, dna=False, ranges=None, max_aln=None, **kw):
+    """k-tuple distances (FastPairwiseAlign) over the range (iStart, iEnd,
+    jStart, jEnd), default the full matrix: (pairs int32 [k,2], dist float64 [k])."""
+    n = len(seqs)
+    fp = fast_params(dna, **kw)
+    iS, iE, jS, jE = ranges if ranges is not None else (0, n, 0, n)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    if max_aln is None:
+        max_aln = 2 * int(lens.max(initial=0))
+    cap = max(1, sum(max(0, jE - max(jS + 2, i + 1) + 1) for i in range(iS + 1, iE + 1)))
+    pairs = np.zeros((cap, 2), np.int32)
+    dist = np.zeros(cap, np.float64)
+    ss = _SeqSet(seqs)
+    try:
+        k = lib().cwo_fast_pairwise(n, C.cast(ss.ptrs, C.c_void_p), ss.lens, 1 if dna else 0,
+                                    fp["ktup"], fp["window"], fp["wind_gap"], fp["signif"],
+                                    fp["percent"], max_aln, iS, iE, jS, jE,
+                                    pairs.ctypes.data, dist.ctypes.data)
+    finally:
+        ss.close()
+    if k < 0:
+        raise ValueError("bad range")
+    return pairs[:k].copy(), dist[:k].copy()

@@ -1,0 +1,113 @@
+"""k-tuple distances on the device (cwhip_fast_pairwise_range, the
+FastPairwiseAlign / -QUICKTREE path) against the oracle, bit for bit: the
+reference's fixtures (protein and DNA), partial ranges, every parameter the
+reference exposes, the partial-alignment cut-off and degenerate sequences."""
+import numpy as np
+import pytest
+
+import cwhip
+import oracle as O
+from cwhip import seqgen
+from fixtures import load
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cwhip.Context(0)
+    yield c
+    c.close()
+
+
+def _load(ctx, seqs):
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(len(seqs), np.int64)
+    if len(seqs) > 1:
+        offs[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) else np.zeros(0, np.uint8)
+    ctx.load_sequences(codes, offs, lens)
+
+
+def _check(ctx, seqs, dna=False, ranges=None, max_aln=None, **kw):
+    n = len(seqs)
+    _load(ctx, seqs)
+    r = ranges if ranges is not None else (0, n, 0, n)
+    if max_aln is not None:
+        kw["max_aln"] = max_aln
+    p, d = ctx.fast_pairwise_range(*r, dna=dna, **kw)
+    po, do = O.fast_pairwise([np.asarray(s, np.int32) for s in seqs], dna=dna, ranges=r,
+                             max_aln=max_aln, **{k: v for k, v in kw.items() if k != "max_aln"})
+    np.testing.assert_array_equal(p, po)
+    np.testing.assert_array_equal(d.view(np.uint64), do.view(np.uint64))
+    return d
+
+
+def _fixture(setname, n=None):
+    names, codes, offs, lens, dna, dnd = load(setname)
+    n = n or len(lens)
+    return [codes[offs[i]:offs[i] + lens[i]] for i in range(n)], dna
+
+
+@pytest.mark.parametrize("setname,n", [("100_97", None), ("100_498", 40), ("1000_97", 150),
+                                        ("100_1002", 30), ("100_1523", 24)])
+def test_fast_matches_oracle_on_fixtures(ctx, setname, n):
+    print("fast", setname, flush=True)
+    seqs, dna = _fixture(setname, n)
+    d = _check(ctx, seqs, dna)
+    assert d.min() >= 0.0 and d.max() <= 1.0
+
+
+def test_fast_partial_ranges(ctx):
+    print("fast ranges", flush=True)
+    seqs, dna = _fixture("100_97", 30)
+    for r in [(0, 30, 0, 30), (5, 12, 0, 30), (0, 30, 20, 30), (3, 9, 7, 15), (10, 10, 0, 30),
+              (0, 30, 29, 30)]:
+        _check(ctx, seqs, dna, ranges=r)
+    _load(ctx, seqs)
+    for bad in [(-1, 5, 0, 5), (5, 4, 0, 5), (0, 5, -1, 5), (0, 5, 5, 4), (0, 31, 0, 30)]:
+        with pytest.raises(cwhip.CwhipError):
+            ctx.fast_pairwise_range(*bad)
+
+
+@pytest.mark.parametrize("kw", [dict(ktup=2), dict(window=1), dict(window=12), dict(signif=1),
+                                dict(signif=40), dict(wind_gap=0), dict(wind_gap=9), dict(percent=0)])
+def test_fast_parameters(ctx, kw):
+    print("fast params", kw, flush=True)
+    seqs, dna = _fixture("100_498", 25)
+    _check(ctx, seqs, dna, **kw)
+
+
+@pytest.mark.parametrize("ktup", [1, 3, 4])
+def test_fast_dna_ktup(ctx, ktup):
+    print("fast dna ktup", ktup, flush=True)
+    seqs, dna = _fixture("100_1002", 16)
+    _check(ctx, seqs, True, ktup=ktup)
+
+
+def test_fast_partial_alignment_cutoff(ctx):
+    """A small maxAlnLength stops pairAlign at 2 x max_aln fragments
+    ("(Partial alignment)"), keeping the best fragment so far."""
+    print("fast partial", flush=True)
+    seqs, dna = _fixture("100_498", 12)
+    for m in (1, 5, 40, 300):
+        _check(ctx, seqs, dna, max_aln=m)
+
+
+def test_fast_degenerate(ctx):
+    print("fast degenerate", flush=True)
+    rng = np.random.default_rng(7)
+    seqs = [rng.integers(0, 25, size=L).astype(np.uint8) for L in (0, 1, 2, 5, 40, 0, 3)]
+    seqs.append(np.full(20, 30, np.uint8))                      # all gaps
+    seqs.append(seqs[4].copy())                                 # identical pair
+    d = _check(ctx, seqs, False)
+    _check(ctx, seqs, False, ktup=2)
+    _check(ctx, [np.array([0, 2, 6, 18, 19, 1, 0, 2], np.uint8)] * 3, True)
+
+
+def test_fast_large_family_sample(ctx):
+    """The 1000 x ~1523 workload's first 64 sequences (2,016 pairs of ~1500
+    residues: long fragment lists, the partial-alignment limit in reach)."""
+    print("fast large", flush=True)
+    fam = seqgen.workload("large1523")
+    _check(ctx, fam.seqs[:64], False)
