@@ -4,9 +4,9 @@
 #include <stdint.h>
 
 // One launch of cw_fast_pairs: the pairs (0-based sequence indices, the
-// reference's visiting order) and the parameters of FastPairwiseAlign.  A
-// lane's arrays are int32 rows in `scratch`, element k of lane g at
-// (row + k) * lanes + g; o_* are the first rows of each array.
+// reference's visiting order) and the parameters of FastPairwiseAlign.  Lane
+// g's arrays are lane_ints int32 at scratch + g * lane_ints; o_* are their
+// offsets there.
 struct FastArgs {
     const uint8_t* codes;
     const int64_t* offs;
@@ -15,7 +15,7 @@ struct FastArgs {
     int64_t npairs;
     double* out_dist;
     int32_t* scratch;
-    int64_t lanes;
+    int64_t lanes, lane_ints;
     unsigned int* next;       // pair counter (0 at launch)
     int32_t dna, ktup, window, wind_gap, signif, percent, max_aln;
     int32_t limit;            // k-tuple codes: 25^ktup (protein) or 4^ktup (DNA)

@@ -6,11 +6,11 @@
 // decides which diagonals are "top", then a fragment list kept in score order
 // that every match on a flagged diagonal walks.  Pairs are independent, so one
 // lane runs one pair from start to end and the grid's lanes take pairs from a
-// counter.  A lane's arrays (positions per k-tuple, diagonals, fragments) live
-// in HBM interleaved by lane (element k of lane g at k * lanes + g), so the
-// loops every lane runs in step (clearing, counting positions) coalesce; the
-// data-dependent walks are latency-bound.  All arithmetic is integer except the
-// final double distance, which is IEEE on both sides.
+// counter.  A lane's arrays (positions per k-tuple, diagonals, fragments) lie
+// contiguously in HBM, so its sequential scans (clearing, the partition scans
+// of the sort) stay within cache lines; the data-dependent walks are
+// latency-bound.  All arithmetic is integer except the final double distance,
+// which is IEEE on both sides.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,9 +32,8 @@ __device__ __forceinline__ int dna_of(int c)
 }
 
 struct Lane {
-    int32_t* sc;
-    int64_t G, g;
-    __device__ __forceinline__ int32_t& at(int64_t arr, int64_t k) const { return sc[(arr + k) * G + g]; }
+    int32_t* sc;     // this lane's rows
+    __device__ __forceinline__ int32_t& at(int64_t arr, int64_t k) const { return sc[arr + k]; }
 };
 
 // residue code of 1-based position p of a sequence (0-based packed codes)
@@ -72,15 +71,18 @@ __device__ void make_ptrs(const FastArgs& a, const Lane& ln, int64_t tptr, int64
     }
 }
 
-// desQuickSort (:584-659) over elements 1 .. n-1 (element n stays in place)
-__device__ void des_quick_sort(const FastArgs& a, const Lane& ln, int n)
+// desQuickSort (:584-659) over elements 1 .. n-1 (element n stays in place).
+// Only positions >= keep are read afterwards (the top SIGNIF diagonals), and a
+// partition's elements never leave it, so a partition wholly below keep is left
+// unsorted: the values at positions >= keep are exactly the full sort's.
+__device__ void des_quick_sort(const FastArgs& a, const Lane& ln, int n, int keep)
 {
     int lst[48], ust[48];
     int p = 1;
     lst[1] = 1;
     ust[1] = n - 1;
     while (p > 0) {
-        if (lst[p] >= ust[p]) {
+        if (lst[p] >= ust[p] || ust[p] < keep) {
             p--;
             continue;
         }
@@ -134,9 +136,9 @@ __device__ void pair_align(const FastArgs& a, const Lane& ln, const uint8_t* s1,
     for (int c = 1; c <= a.limit; ++c)                  // k-tuple matches per diagonal
         for (int vn1 = ln.at(a.o_zzc, c); vn1; vn1 = ln.at(a.o_zza, vn1))
             for (int vn2 = ln.at(a.o_zzd, c); vn2; vn2 = ln.at(a.o_zzb, vn2)) ++ln.at(a.o_displ, vn1 - vn2 + l2);
-    des_quick_sort(a, ln, tl1);
     int j = tl1 - a.signif + 1;
     if (j < 1) j = 1;
+    des_quick_sort(a, ln, tl1, j);
     for (int i = tl1; i >= j; i--)                      // WINDOW around the top SIGNIF diagonals
         if (ln.at(a.o_displ, i) > 0) {
             const int pos = ln.at(a.o_diag, i);
@@ -232,7 +234,7 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.lanes) return;
-    const Lane ln{a.scratch, a.lanes, g};
+    const Lane ln{a.scratch + g * a.lane_ints};
     for (;;) {
         const int64_t k = atomicAdd(a.next, 1u);
         if (k >= a.npairs) break;

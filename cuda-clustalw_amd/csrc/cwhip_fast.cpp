@@ -79,11 +79,12 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     a.acc_len = 2 * (int64_t)max_aln + 2;
     a.o_acc = rows; rows += 4 * a.acc_len;
     // lanes: up to one per wave slot of the chip, within 8 GiB of scratch
-    const int64_t cap_lanes = std::max<int64_t>(256, ((int64_t)8 << 30) / (rows * 4) / 256 * 256);
+    const int64_t cap_lanes = std::max<int64_t>(256, ((int64_t)8 << 30) / ((rows + 32) * 4) / 256 * 256);
     int64_t lanes = std::min<int64_t>((int64_t)std::max(1, ctx->num_cu) * 4 * 64, cap_lanes);
     lanes = std::min<int64_t>(lanes, (np + 255) / 256 * 256);
     a.lanes = lanes;
-    HIPCHK(ctx->f_scratch.ensure((size_t)(rows * lanes)));
+    a.lane_ints = (rows + 31) / 32 * 32;                            // 128-byte aligned lanes
+    HIPCHK(ctx->f_scratch.ensure((size_t)(a.lane_ints * lanes)));
     HIPCHK(ctx->f_pairs.ensure(pairs.size()));
     HIPCHK(ctx->f_dist.ensure((size_t)np));
     HIPCHK(ctx->f_ctr.ensure(1));
@@ -101,11 +102,12 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    HIPCHK(hipMemcpyAsync(out_dist, ctx->f_dist.p, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->fast_ms = ms;
+    HIPCHK(hipMemcpyAsync(out_dist, ctx->f_dist.p, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return CWHIP_OK;
 }
 

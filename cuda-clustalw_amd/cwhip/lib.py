@@ -160,7 +160,7 @@ def load() -> C.CDLL:
     L.cwhip_server_stats.argtypes = [vp, P(np.float64, flags="C")]
     for name in EXPORTS:
         if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance",
-                        "cwhip_server_destroy"):
+                        "cwhip_server_destroy", "cwhip_fast_last_ms"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

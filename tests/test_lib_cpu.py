@@ -29,6 +29,17 @@ def test_library_exports_header_symbols():
     assert set(names) == set(cl.EXPORTS)
 
 
+def test_binding_return_types():
+    """Entry points that return something other than an int status keep their
+    ctypes return type (a float read back as int reads 0)."""
+    import ctypes as C
+    L = cl.load()
+    assert L.cwhip_distance.restype is C.c_float
+    assert L.cwhip_fast_last_ms.restype is C.c_float
+    assert L.cwhip_range_npairs.restype is C.c_int64
+    assert L.cwhip_fast_pairwise_range.restype is C.c_int
+
+
 def test_params_equal_checker():
     p = cwhip.pair_params(max_len=6000)
     m = np.ctypeslib.as_array(p.matrix).reshape(32, 32)
