@@ -23,6 +23,15 @@ struct FastArgs {
     int64_t o_zzc, o_zzd;     // per k-tuple code: its last position (sequence 1, 2)
     int64_t o_displ, o_slopes, o_diag;   // per diagonal
     int64_t o_acc, acc_len;   // fragments: four rows of acc_len
+    int32_t wL, wD;           // cw_fast_wave: longest sequence, diagonal slots (2 wL + 2)
 };
 
 hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a);
+hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a);
+
+// LDS bytes cw_fast_wave needs for sequences of up to L residues
+inline size_t cwf_wave_lds(int L, int limit, int64_t acc_len)
+{
+    const int64_t D = 2 * (int64_t)L + 2;
+    return (size_t)(4 * D + 2 * (4 * ((int64_t)L + 2) + 2 * ((int64_t)limit + 2) + D + 4 * acc_len) + D + 16);
+}

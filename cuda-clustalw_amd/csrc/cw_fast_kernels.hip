@@ -255,6 +255,297 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
     }
 }
 
+// ---- one wave per pair, arrays in LDS ------------------------------------------
+// The same walk with a pair's arrays in LDS (16-bit where values fit).  The
+// lane-parallel parts: k-tuple codes, clearing, the diagonal counts (LDS
+// atomics), the list of flagged diagonals, and for each position of sequence 1
+// the check of every flagged diagonal for a match.  The two position lists are
+// built by two lanes at once.  The sequential parts (the sort, the fragment
+// chain) are run by every lane alike on the same LDS values, so the wave never
+// diverges there; every lane reads what it needs into registers before a wave
+// barrier and writes after it, so no lane reads a location another lane of the
+// same step writes.  For a position i of sequence 1 the reference
+// visits the positions vn2 of sequence 2 with the same k-tuple in descending
+// order and skips unflagged diagonals; taking the flagged diagonals in
+// ascending order (vn2 = i + l2 - diagonal, descending) and keeping those whose
+// k-tuple matches visits the same (i, vn2) in the same order.
+struct WaveLds {
+    int32_t* displ;                 // [D]
+    int16_t *kt1, *kt2;             // k-tuple code + 1 per position, 0: none (positions 1..L)
+    int16_t *tp1, *tp2;             // previous position with the same k-tuple
+    int16_t *pl1, *pl2;             // last position of each code
+    int16_t* diag;                  // [D] diagonal index, then the flagged diagonals in order
+    int16_t *a0, *a1, *a2, *a4;     // fragments: score, start 1, start 2, next in score order
+    uint8_t* slopes;                // [D]
+};
+
+__device__ __forceinline__ WaveLds wave_lds(const FastArgs& a, unsigned char* base)
+{
+    WaveLds w;
+    w.displ = (int32_t*)base;
+    w.kt1 = (int16_t*)(w.displ + a.wD);
+    w.kt2 = w.kt1 + (a.wL + 2);
+    w.tp1 = w.kt2 + (a.wL + 2);
+    w.tp2 = w.tp1 + (a.wL + 2);
+    w.pl1 = w.tp2 + (a.wL + 2);
+    w.pl2 = w.pl1 + (a.limit + 2);
+    w.diag = w.pl2 + (a.limit + 2);
+    w.a0 = w.diag + a.wD;
+    w.a1 = w.a0 + a.acc_len;
+    w.a2 = w.a1 + a.acc_len;
+    w.a4 = w.a2 + a.acc_len;
+    w.slopes = (uint8_t*)(w.a4 + a.acc_len);
+    return w;
+}
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
+
+extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const WaveLds w = wave_lds(a, lds);
+    const int lane = threadIdx.x;
+    const int mr = a.dna ? 3 : 24;
+    for (;;) {
+        const unsigned got = atomicAdd(a.next, lane == 0 ? 1u : 0u);
+        const int64_t k = (int64_t)__shfl((int)got, 0, 64);
+        if (k >= a.npairs) break;
+        const int si = a.pairs[2 * k], sj = a.pairs[2 * k + 1];
+        const int l1 = a.lens[si], l2 = a.lens[sj];
+        const uint8_t* s1 = a.codes + a.offs[si];
+        const uint8_t* s2 = a.codes + a.offs[sj];
+        // k-tuple codes (+1; 0 where a k-tuple does not fit or has a bad residue)
+        for (int h = 0; h < 2; ++h) {
+            const int L = h ? l2 : l1;
+            const uint8_t* s = h ? s2 : s1;
+            int16_t* kt = h ? w.kt2 : w.kt1;
+            for (int p = 1 + lane; p <= L; p += 64) {
+                int code = 0, pot = 1;
+                bool bad = p > L - a.ktup + 1;
+                for (int j = 0; j < a.ktup && !bad; ++j) {
+                    const int r = a.dna ? dna_of(s[p - 1 + j]) : (int)s[p - 1 + j];
+                    if (r < 0 || r > mr) bad = true;
+                    code += r * pot;
+                    pot *= mr + 1;
+                }
+                kt[p] = bad ? 0 : (int16_t)(code + 1);
+            }
+        }
+        for (int c = 1 + lane; c <= a.limit; c += 64) {
+            w.pl1[c] = 0;
+            w.pl2[c] = 0;
+        }
+        const int tl1 = l1 + l2 - 1;
+        for (int d = 1 + lane; d <= tl1; d += 64) {
+            w.slopes[d] = 0;
+            w.displ[d] = 0;
+            w.diag[d] = (int16_t)d;
+        }
+        wave_sync();
+        if ((lane & 31) == 0) {   // makePPtrs / makeNPtrs: lane 0 sequence 1's lists, lane 32 sequence 2's
+            const int h = lane >> 5;
+            const int L = h ? l2 : l1;
+            const int16_t* kt = h ? w.kt2 : w.kt1;
+            int16_t* tp = h ? w.tp2 : w.tp1;
+            int16_t* pl = h ? w.pl2 : w.pl1;
+            for (int i = 1; i <= L; ++i) {
+                const int c = kt[i];
+                int16_t prev = 0;
+                if (c) {
+                    prev = pl[c];
+                    pl[c] = (int16_t)i;
+                }
+                tp[i] = prev;
+            }
+        }
+        wave_sync();
+        // k-tuple matches per diagonal
+        for (int v1 = 1 + lane; v1 <= l1; v1 += 64) {
+            const int c = w.kt1[v1];
+            if (!c) continue;
+            for (int v2 = w.pl2[c]; v2; v2 = w.tp2[v2]) atomicAdd(&w.displ[v1 - v2 + l2], 1);
+        }
+        wave_sync();
+        int jtop = tl1 - a.signif + 1;
+        if (jtop < 1) jtop = 1;
+        // desQuickSort over 1 .. tl1-1, partitions wholly below jtop left unsorted
+        // (every lane runs it alike)
+        {
+            int lst[48], ust[48];
+            int p = 1;
+            lst[1] = 1;
+            ust[1] = tl1 - 1;
+            while (p > 0) {
+                if (lst[p] >= ust[p] || ust[p] < jtop) {
+                    p--;
+                    continue;
+                }
+                int i = lst[p] - 1, j = ust[p];
+                const int piv = w.displ[j];
+                while (i < j) {
+                    for (i = i + 1; w.displ[i] < piv; i++)
+                        ;
+                    for (j = j - 1; j > i; j--)
+                        if (w.displ[j] <= piv) break;
+                    if (i < j) {
+                        const int ti = w.displ[i], tj = w.displ[j];
+                        const int16_t ui = w.diag[i], uj = w.diag[j];
+                        wave_sync();
+                        w.displ[i] = tj;
+                        w.diag[i] = uj;
+                        w.displ[j] = ti;
+                        w.diag[j] = ui;
+                        wave_sync();
+                    }
+                }
+                j = ust[p];
+                const int ti = w.displ[i], tj = w.displ[j];
+                const int16_t ui = w.diag[i], uj = w.diag[j];
+                wave_sync();
+                w.displ[i] = tj;
+                w.diag[i] = uj;
+                w.displ[j] = ti;
+                w.diag[j] = ui;
+                wave_sync();
+                if (i - lst[p] < ust[p] - i) {
+                    lst[p + 1] = lst[p];
+                    ust[p + 1] = i - 1;
+                    lst[p] = i + 1;
+                } else {
+                    lst[p + 1] = i + 1;
+                    ust[p + 1] = ust[p];
+                    ust[p] = i - 1;
+                }
+                p = p + 1;
+            }
+        }
+        // WINDOW around the top SIGNIF diagonals
+        for (int i = tl1; i >= jtop; i--) {
+            const int cnt = w.displ[i];
+            const int pos = w.diag[i];
+            wave_sync();
+            if (cnt > 0) {
+                const int l = 1 > pos - a.window ? 1 : pos - a.window;
+                const int m = tl1 < pos + a.window ? tl1 : pos + a.window;
+                for (int q = l + lane; q <= m; q += 64) w.slopes[q] = 1;
+            }
+            wave_sync();
+        }
+        // clear the diagonals' fragment slots; list the flagged diagonals in order
+        int nf = 0;
+        for (int b0 = 1; b0 <= tl1; b0 += 64) {
+            const int d = b0 + lane;
+            const bool f = d <= tl1 && w.slopes[d] == 1;
+            const uint64_t bal = __ballot(f);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            if (d <= tl1) w.displ[d] = 0;
+            wave_sync();
+            if (f) w.diag[nf + before] = (int16_t)d;
+            nf += __popcll(bal);
+            wave_sync();
+        }
+        int curr = 0, maxsofar = 0;
+        bool stop = false;
+        w.a0[0] = 0;
+        wave_sync();
+        for (int i = 1; i <= l1 - a.ktup + 1 && !stop; ++i) {
+            const int c = w.kt1[i];
+            if (!c) continue;
+            for (int f0 = 0; f0 < nf && !stop; f0 += 64) {
+                const int fi = f0 + lane;
+                int v2c = 0;
+                bool m = false;
+                if (fi < nf) {
+                    v2c = i + l2 - w.diag[fi];
+                    m = v2c >= 1 && v2c <= l2 && w.kt2[v2c] == c;
+                }
+                uint64_t bal = __ballot(m);
+                while (bal && !stop) {
+                    const int b = __builtin_ctzll(bal);
+                    bal &= bal - 1;
+                    const int osptr = w.diag[f0 + b];
+                    const int vn2 = i + l2 - osptr;
+                    int fs = a.ktup, next = maxsofar;
+                    for (;;) {                          // A-loop (:299-393), every lane alike
+                        if (!next) {
+                            ++curr;
+                            if (curr >= 2 * a.max_aln) {
+                                stop = true;            // "(Partial alignment)"
+                                break;
+                            }
+                            wave_sync();
+                            w.displ[osptr] = curr;
+                            w.a0[curr] = (int16_t)fs;
+                            w.a1[curr] = (int16_t)i;
+                            w.a2[curr] = (int16_t)vn2;
+                            wave_sync();
+                            if (!maxsofar) {
+                                maxsofar = 1;
+                                w.a4[curr] = 0;
+                            } else if (fs >= w.a0[maxsofar]) {
+                                w.a4[curr] = (int16_t)maxsofar;
+                                maxsofar = curr;
+                            } else {
+                                int nx = maxsofar, end;
+                                for (;;) {
+                                    end = nx;
+                                    nx = w.a4[nx];
+                                    if (fs >= w.a0[nx]) break;
+                                }
+                                wave_sync();
+                                w.a4[curr] = (int16_t)nx;
+                                w.a4[end] = (int16_t)curr;
+                            }
+                            wave_sync();
+                            break;
+                        }
+                        const int t1 = w.a1[next], t2 = w.a2[next];
+                        bool rel;
+                        if (i - vn2 == t1 - t2) rel = t1 < i;
+                        else rel = t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2;
+                        if (!rel) {
+                            next = w.a4[next];
+                            continue;
+                        }
+                        if (i - vn2 == t1 - t2) {
+                            if (i > t1 + (a.ktup - 1)) fs = w.a0[next] + a.ktup;
+                            else fs = w.a0[next] + (i - t1);
+                        } else {
+                            const int d = w.displ[osptr];
+                            int subt1;
+                            if (d == 0) subt1 = a.ktup;
+                            else if (i > w.a1[d] + (a.ktup - 1)) subt1 = w.a0[d] + a.ktup;
+                            else subt1 = w.a0[d] + (i - w.a1[d]);
+                            const int subt2 = w.a0[next] - a.wind_gap + a.ktup;
+                            fs = subt2 > subt1 ? subt2 : subt1;
+                        }
+                        next = 0;
+                    }
+                }
+            }
+        }
+        const int sc = maxsofar ? (int)w.a0[maxsofar] : 0;
+        if (lane == 0) {
+            double score = 0.0;                         // FastPairwiseAlign.cpp:87-100
+            if (sc != 0) {
+                score = (double)sc;
+                if (a.percent) score = (score / (double)(l1 < l2 ? l1 : l2)) * 100.0;
+            }
+            a.out_dist[k] = (100.0 - score) / 100.0;
+        }
+        wave_sync();
+    }
+}
+
+hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a)
+{
+    hipError_t e = hipFuncSetAttribute((const void*)cw_fast_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cw_fast_wave, dim3(blocks), dim3(64), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
 hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a)
 {
     hipLaunchKernelGGL(cw_fast_pairs, dim3(blocks), dim3(256), 0, st, a);

@@ -84,7 +84,11 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     lanes = std::min<int64_t>(lanes, (np + 255) / 256 * 256);
     a.lanes = lanes;
     a.lane_ints = (rows + 31) / 32 * 32;                            // 128-byte aligned lanes
-    HIPCHK(ctx->f_scratch.ensure((size_t)(a.lane_ints * lanes)));
+    // one wave per pair with its arrays in LDS where they fit (16-bit positions
+    // and fragment numbers), else one lane per pair with its arrays in HBM
+    const size_t wlds = cwf_wave_lds(lmax, limit, a.acc_len);
+    const bool lane_kernel = !(wlds <= 160 * 1024 && 2 * (int64_t)max_aln + 2 <= 32767 && lmax <= 16000);
+    if (lane_kernel) HIPCHK(ctx->f_scratch.ensure((size_t)(a.lane_ints * lanes)));
     HIPCHK(ctx->f_pairs.ensure(pairs.size()));
     HIPCHK(ctx->f_dist.ensure((size_t)np));
     HIPCHK(ctx->f_ctr.ensure(1));
@@ -97,10 +101,18 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     a.pairs = ctx->f_pairs.p;
     a.npairs = np;
     a.out_dist = ctx->f_dist.p;
-    a.scratch = ctx->f_scratch.p;
+    a.scratch = lane_kernel ? ctx->f_scratch.p : nullptr;
     a.next = ctx->f_ctr.p;
+    a.wL = lmax;
+    a.wD = 2 * lmax + 2;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
+    if (!lane_kernel) {
+        const int per_cu = std::max<int>(1, (int)((160 * 1024) / wlds));
+        const int64_t blocks = std::min<int64_t>(np, (int64_t)std::max(1, ctx->num_cu) * std::min(per_cu, 16));
+        HIPCHK(cwf_launch_wave((int)blocks, wlds, ctx->stream, a));
+    } else {
+        HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipEventSynchronize(ctx->ev1));
     float ms = 0;

@@ -94,6 +94,17 @@ def test_fast_partial_alignment_cutoff(ctx):
         _check(ctx, seqs, dna, max_aln=m)
 
 
+def test_fast_hbm_lane_kernel(ctx):
+    """Pairs whose arrays do not fit a wave's LDS (here: fragment numbers beyond
+    16 bits with maxAlnLength 20000) take the one-lane-per-pair kernel with its
+    arrays in HBM; same results."""
+    print("fast lane kernel", flush=True)
+    seqs, dna = _fixture("100_498", 10)
+    _check(ctx, seqs, dna, max_aln=20000)
+    seqs, dna = _fixture("100_1002", 6)
+    _check(ctx, seqs, True, max_aln=20000)
+
+
 def test_fast_degenerate(ctx):
     print("fast degenerate", flush=True)
     rng = np.random.default_rng(7)
