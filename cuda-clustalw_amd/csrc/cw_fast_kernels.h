@@ -33,5 +33,5 @@ hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const F
 inline size_t cwf_wave_lds(int L, int limit, int64_t acc_len)
 {
     const int64_t D = 2 * (int64_t)L + 2;
-    return (size_t)(4 * D + 2 * (4 * ((int64_t)L + 2) + 2 * ((int64_t)limit + 2) + D + 4 * acc_len) + D + 16);
+    return (size_t)(8 * acc_len + 2 * (D + 4 * ((int64_t)L + 2) + 2 * ((int64_t)limit + 2) + D) + D + 16);
 }

@@ -270,35 +270,37 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
 // ascending order (vn2 = i + l2 - diagonal, descending) and keeping those whose
 // k-tuple matches visits the same (i, vn2) in the same order.
 struct WaveLds {
-    int32_t* displ;                 // [D]
+    uint32_t* an;                   // fragments: score << 16 | next in score order
+    uint32_t* ap;                   // fragments: start in sequence 1 << 16 | start in sequence 2
+    int16_t* displ;                 // [D] diagonal counts (two per 32-bit word while counting), then fragments
     int16_t *kt1, *kt2;             // k-tuple code + 1 per position, 0: none (positions 1..L)
     int16_t *tp1, *tp2;             // previous position with the same k-tuple
     int16_t *pl1, *pl2;             // last position of each code
     int16_t* diag;                  // [D] diagonal index, then the flagged diagonals in order
-    int16_t *a0, *a1, *a2, *a4;     // fragments: score, start 1, start 2, next in score order
     uint8_t* slopes;                // [D]
 };
 
 __device__ __forceinline__ WaveLds wave_lds(const FastArgs& a, unsigned char* base)
 {
     WaveLds w;
-    w.displ = (int32_t*)base;
-    w.kt1 = (int16_t*)(w.displ + a.wD);
+    w.an = (uint32_t*)base;
+    w.ap = w.an + a.acc_len;
+    w.displ = (int16_t*)(w.ap + a.acc_len);
+    w.kt1 = w.displ + a.wD;
     w.kt2 = w.kt1 + (a.wL + 2);
     w.tp1 = w.kt2 + (a.wL + 2);
     w.tp2 = w.tp1 + (a.wL + 2);
     w.pl1 = w.tp2 + (a.wL + 2);
     w.pl2 = w.pl1 + (a.limit + 2);
     w.diag = w.pl2 + (a.limit + 2);
-    w.a0 = w.diag + a.wD;
-    w.a1 = w.a0 + a.acc_len;
-    w.a2 = w.a1 + a.acc_len;
-    w.a4 = w.a2 + a.acc_len;
-    w.slopes = (uint8_t*)(w.a4 + a.acc_len);
+    w.slopes = (uint8_t*)(w.diag + a.wD);
     return w;
 }
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
+
+__device__ __forceinline__ int fr_score(uint32_t v) { return (int)(v >> 16); }
+__device__ __forceinline__ int fr_next(uint32_t v) { return (int)(v & 0xffffu); }
 
 extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
 {
@@ -336,7 +338,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
             w.pl2[c] = 0;
         }
         const int tl1 = l1 + l2 - 1;
-        for (int d = 1 + lane; d <= tl1; d += 64) {
+        for (int d = lane; d <= tl1 + 1; d += 64) {     // whole words of the counts
             w.slopes[d] = 0;
             w.displ[d] = 0;
             w.diag[d] = (int16_t)d;
@@ -359,11 +361,16 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
             }
         }
         wave_sync();
-        // k-tuple matches per diagonal
+        // k-tuple matches per diagonal: 16-bit counts, two to a 32-bit word
+        // (a count stays below 2^15, so an add never carries into its neighbour)
+        uint32_t* words = (uint32_t*)w.displ;
         for (int v1 = 1 + lane; v1 <= l1; v1 += 64) {
             const int c = w.kt1[v1];
             if (!c) continue;
-            for (int v2 = w.pl2[c]; v2; v2 = w.tp2[v2]) atomicAdd(&w.displ[v1 - v2 + l2], 1);
+            for (int v2 = w.pl2[c]; v2; v2 = w.tp2[v2]) {
+                const int d = v1 - v2 + l2;
+                atomicAdd(&words[d >> 1], 1u << (16 * (d & 1)));
+            }
         }
         wave_sync();
         int jtop = tl1 - a.signif + 1;
@@ -388,7 +395,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     for (j = j - 1; j > i; j--)
                         if (w.displ[j] <= piv) break;
                     if (i < j) {
-                        const int ti = w.displ[i], tj = w.displ[j];
+                        const int16_t ti = w.displ[i], tj = w.displ[j];
                         const int16_t ui = w.diag[i], uj = w.diag[j];
                         wave_sync();
                         w.displ[i] = tj;
@@ -399,7 +406,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     }
                 }
                 j = ust[p];
-                const int ti = w.displ[i], tj = w.displ[j];
+                const int16_t ti = w.displ[i], tj = w.displ[j];
                 const int16_t ui = w.diag[i], uj = w.diag[j];
                 wave_sync();
                 w.displ[i] = tj;
@@ -446,17 +453,16 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
         }
         int curr = 0, maxsofar = 0;
         bool stop = false;
-        w.a0[0] = 0;
+        w.an[0] = 0;                    // score 0: where putFrag's walk ends
         wave_sync();
         for (int i = 1; i <= l1 - a.ktup + 1 && !stop; ++i) {
             const int c = w.kt1[i];
             if (!c) continue;
             for (int f0 = 0; f0 < nf && !stop; f0 += 64) {
                 const int fi = f0 + lane;
-                int v2c = 0;
                 bool m = false;
                 if (fi < nf) {
-                    v2c = i + l2 - w.diag[fi];
+                    const int v2c = i + l2 - w.diag[fi];
                     m = v2c >= 1 && v2c <= l2 && w.kt2[v2c] == c;
                 }
                 uint64_t bal = __ballot(m);
@@ -473,50 +479,56 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                                 stop = true;            // "(Partial alignment)"
                                 break;
                             }
-                            wave_sync();
-                            w.displ[osptr] = curr;
-                            w.a0[curr] = (int16_t)fs;
-                            w.a1[curr] = (int16_t)i;
-                            w.a2[curr] = (int16_t)vn2;
-                            wave_sync();
+                            // putFrag (:523-559): into the list kept in descending score order
+                            int link = 0, end = 0;
                             if (!maxsofar) {
-                                maxsofar = 1;
-                                w.a4[curr] = 0;
-                            } else if (fs >= w.a0[maxsofar]) {
-                                w.a4[curr] = (int16_t)maxsofar;
-                                maxsofar = curr;
+                                link = 0;
+                            } else if (fs >= fr_score(w.an[maxsofar])) {
+                                link = maxsofar;
                             } else {
-                                int nx = maxsofar, end;
+                                int nx = maxsofar;
+                                uint32_t cur = w.an[nx];
                                 for (;;) {
                                     end = nx;
-                                    nx = w.a4[nx];
-                                    if (fs >= w.a0[nx]) break;
+                                    nx = fr_next(cur);
+                                    cur = w.an[nx];
+                                    if (fs >= fr_score(cur)) break;
                                 }
-                                wave_sync();
-                                w.a4[curr] = (int16_t)nx;
-                                w.a4[end] = (int16_t)curr;
+                                link = nx;
                             }
+                            const uint32_t endv = end ? w.an[end] : 0u;
                             wave_sync();
+                            w.displ[osptr] = (int16_t)curr;
+                            w.an[curr] = ((uint32_t)fs << 16) | (uint32_t)link;
+                            w.ap[curr] = ((uint32_t)i << 16) | (uint32_t)vn2;
+                            if (end) w.an[end] = (endv & 0xffff0000u) | (uint32_t)curr;
+                            wave_sync();
+                            if (!maxsofar) maxsofar = 1;
+                            else if (!end) maxsofar = curr;
                             break;
                         }
-                        const int t1 = w.a1[next], t2 = w.a2[next];
-                        bool rel;
+                        const uint32_t pv = w.ap[next];
+                        const int t1 = (int)(pv >> 16), t2 = (int)(pv & 0xffffu);
+                        bool rel;                       // fragRelPos (:561-582)
                         if (i - vn2 == t1 - t2) rel = t1 < i;
                         else rel = t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2;
                         if (!rel) {
-                            next = w.a4[next];
+                            next = fr_next(w.an[next]);
                             continue;
                         }
+                        const int sn = fr_score(w.an[next]);
                         if (i - vn2 == t1 - t2) {
-                            if (i > t1 + (a.ktup - 1)) fs = w.a0[next] + a.ktup;
-                            else fs = w.a0[next] + (i - t1);
+                            fs = i > t1 + (a.ktup - 1) ? sn + a.ktup : sn + (i - t1);
                         } else {
                             const int d = w.displ[osptr];
                             int subt1;
-                            if (d == 0) subt1 = a.ktup;
-                            else if (i > w.a1[d] + (a.ktup - 1)) subt1 = w.a0[d] + a.ktup;
-                            else subt1 = w.a0[d] + (i - w.a1[d]);
-                            const int subt2 = w.a0[next] - a.wind_gap + a.ktup;
+                            if (d == 0) {
+                                subt1 = a.ktup;
+                            } else {
+                                const int d1 = (int)(w.ap[d] >> 16), ds = fr_score(w.an[d]);
+                                subt1 = i > d1 + (a.ktup - 1) ? ds + a.ktup : ds + (i - d1);
+                            }
+                            const int subt2 = sn - a.wind_gap + a.ktup;
                             fs = subt2 > subt1 ? subt2 : subt1;
                         }
                         next = 0;
@@ -524,7 +536,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                 }
             }
         }
-        const int sc = maxsofar ? (int)w.a0[maxsofar] : 0;
+        const int sc = maxsofar ? fr_score(w.an[maxsofar]) : 0;
         if (lane == 0) {
             double score = 0.0;                         // FastPairwiseAlign.cpp:87-100
             if (sc != 0) {
