@@ -216,6 +216,51 @@ def progressive_stage(cpu: bool):
     return out
 
 
+def ktuple_stage(cpu: bool):
+    """k-tuple distances (FastPairwiseAlign, the -QUICKTREE path) over all
+    499,500 pairs of the 1000 x ~1523 workload: the kernel time of one
+    cwhip_fast_pairwise_range (HIP events on the context's stream), pairs/s; 300
+    pairs spread over the matrix compared bit for bit with the oracle; the CPU
+    baseline is the oracle on one thread over the first 60 sequences' 1,770
+    pairs."""
+    import cwhip
+    from cwhip import seqgen
+    fam = seqgen.workload("large1523")
+    codes, offs, lens = fam.packed()
+    n = len(lens)
+    ctx = cwhip.Context(0)
+    ctx.load_sequences(codes, offs, lens)
+    ctx.fast_pairwise_range(0, 8, 0, 8)                 # warm-up
+    t = time.perf_counter()
+    pairs, dist = ctx.fast_pairwise_range(0, n, 0, n)
+    wall = time.perf_counter() - t
+    ms = ctx.fast_last_ms()
+    ctx.close()
+    out = {"workload": "large1523", "sequences": n, "pairs": int(len(dist)),
+           "params": "protein ktup 1, window 5, window gap 3, 5 top diagonals, percent",
+           "kernel_ms": round(ms, 1), "wall_s": round(wall, 3),
+           "pairs_per_s": round(len(dist) / (ms * 1e-3), 1)}
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O  # checker/baseline only
+    rng = np.random.default_rng(11)
+    pick = np.sort(rng.choice(n, 25, replace=False))    # 300 pairs among 25 sequences
+    sub = [np.asarray(fam.seqs[i], np.int32) for i in pick]
+    po, do = O.fast_pairwise(sub, max_aln=2 * int(lens.max()))
+    index = {(int(a), int(b)): k for k, (a, b) in enumerate(pairs)}
+    got = np.array([dist[index[(int(pick[a]), int(pick[b]))]] for a, b in po])
+    out["identical_sample"] = bool(np.array_equal(got.view(np.uint64), do.view(np.uint64)))
+    if cpu:
+        seqs60 = [np.asarray(s, np.int32) for s in fam.seqs[:60]]
+        t = time.perf_counter()
+        _, d60 = O.fast_pairwise(seqs60, max_aln=2 * int(lens.max()))
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(len(d60) / dt, 1), "unit": "pairs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "the first 60 sequences' %d pairs, %.1f s, one thread" % (len(d60), dt)}
+        out["speedup_vs_cpu"] = round(out["pairs_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def serving_stage(rate: float, count: int, seed: int = 20260105, slots: int = 4):
     """Poisson arrivals at `rate` requests/s of `count` serving requests (N ~ U[10,500],
     L ~ lognormal(ln 300, 0.7) clipped [30, 2000]; 80% distances, 20% distances +
@@ -317,10 +362,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-progressive", action="store_true")
+    ap.add_argument("--no-ktuple", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-small", action="store_true")
-    ap.add_argument("--stage", choices=["all", "batching", "progressive", "serving"], default="all",
+    ap.add_argument("--stage", choices=["all", "batching", "progressive", "serving", "ktuple"], default="all",
                     help="progressive / serving: run only that part and print its object")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: requests per second")
     ap.add_argument("--serve-count", type=int, default=80, help="serving: number of requests")
@@ -343,6 +389,9 @@ def main():
         return
     if args.stage == "batching":
         print(json.dumps({"batching": batching_stage(not args.no_cpu, 5.0)}), flush=True)
+        return
+    if args.stage == "ktuple":
+        print(json.dumps({"ktuple": ktuple_stage(not args.no_cpu)}), flush=True)
         return
     if args.stage == "serving":
         print(json.dumps(serving_all(args)), flush=True)
@@ -431,6 +480,8 @@ def main():
         out["batching"] = batching_stage(not args.no_cpu, 5.0)
     if not args.no_progressive:
         out["progressive"] = progressive_stage(not args.no_cpu)
+    if not args.no_ktuple:
+        out["ktuple"] = ktuple_stage(not args.no_cpu)
     if not args.no_serving:
         out.update(serving_all(args))
     print(json.dumps(out), flush=True)
