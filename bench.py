@@ -196,7 +196,8 @@ def progressive_stage(cpu: bool):
     out["roofline"] = {"bound": "valu", "achieved": round(ach, 4), "peak": PEAK_INT32_TOPS,
                        "unit": "Tops/s (int32)", "frac": round(ach / PEAK_INT32_TOPS, 5),
                        "traffic": None,
-                       "note": "device time of cw_prf_score + cw_prf_mm summed over the "
+                       "note": "device time of the profile kernels (cw_prf_cols, cw_prf_gapcols, "
+                               "cw_prf_score, cw_prf_mm, cw_prf_local) summed over the "
                                "dependency-level batches"}
     if cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
