@@ -953,14 +953,40 @@ extern "C" __global__ void __launch_bounds__(256) cw_bin_scatter(DevArgs A)
         if (bin[u] >= 0) A.order_out[at[bin[u]] + rank[u]] = (int32_t)(base + threadIdx.x + 256 * u);
 }
 
-// The batched run's stages on one stream: reset, count, scan, scatter, then
-// the persistent kernel over the binned order (captured into one hipGraph by
-// the host).
+// The same counting sort by one workgroup for a batch of at most
+// kBinSmallPairs pairs (counts and cursors in LDS): one launch instead of four.
+constexpr int kBinSmallPairs = 65536;
+
+extern "C" __global__ void __launch_bounds__(1024) cw_bin_small(DevArgs A)
+{
+    __shared__ int cnt[CW_NBINS], at[CW_NBINS];
+    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
+    if (threadIdx.x == 0) { *A.queue = 0u; *A.status = 0; }
+    __syncthreads();
+    for (int64_t p = threadIdx.x; p < A.pair_end; p += blockDim.x) atomicAdd(&cnt[pair_bin(A, p)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {                 // bin CW_NBINS-1 first
+        int run = 0;
+        for (int b = CW_NBINS - 1; b >= 0; --b) {
+            at[b] = run;
+            run += cnt[b];
+        }
+    }
+    __syncthreads();
+    for (int64_t p = threadIdx.x; p < A.pair_end; p += blockDim.x)
+        A.order_out[atomicAdd(&at[pair_bin(A, p)], 1)] = (int32_t)p;
+}
+
+// The batched run's stages on one stream: reset, count, scan, scatter (or
+// cw_bin_small for a small batch), then the persistent kernel over the binned
+// order (captured into one hipGraph by the host).
 hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a)
 {
     const int nb = (int)((a.pair_end + kBinItems - 1) / kBinItems);
-    hipLaunchKernelGGL(cw_bin_reset, dim3(1), dim3(256), 0, st, a);
-    if (nb > 0) {
+    if (a.pair_end <= kBinSmallPairs) {
+        hipLaunchKernelGGL(cw_bin_small, dim3(1), dim3(1024), 0, st, a);
+    } else {
+        hipLaunchKernelGGL(cw_bin_reset, dim3(1), dim3(256), 0, st, a);
         hipLaunchKernelGGL(cw_bin_count, dim3(nb), dim3(256), 0, st, a);
         hipLaunchKernelGGL(cw_bin_scan, dim3(1), dim3(64), 0, st, a);
         hipLaunchKernelGGL(cw_bin_scatter, dim3(nb), dim3(256), 0, st, a);
