@@ -382,8 +382,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the pairwise stage shards by pair lists "
-              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only (the pairwise stage would shard by pair lists, "
+              "one context per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.stage == "progressive":
         print(json.dumps({"progressive": progressive_stage(not args.no_cpu)}), flush=True)
