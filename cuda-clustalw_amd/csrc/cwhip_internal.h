@@ -81,6 +81,7 @@ struct cwhip_ctx {
     DevBuf<uint8_t> p_rows;                     // group rows of a profile batch
     DevBuf<PrfSide> p_sides;                    // its profiles to build
     PinBuf<uint8_t> h_rows;                     // staging of the rows
+    std::vector<PrfSide> h_sides;               // host image of p_sides
     int prf_rmax = CWP_RMAX;
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;

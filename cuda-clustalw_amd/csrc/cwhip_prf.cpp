@@ -652,7 +652,8 @@ int run_built_batch(cwhip_ctx* ctx, const cwhip_msa_params& P, const Seqs& seqs,
     // {weight, first, end}; then the builder's per-column sums (device only)
     const int64_t up0 = lo.total;
     int64_t off = up0 + (int64_t)uniq.size() * NUMRES * NUMRES;
-    std::vector<PrfSide> sides(2 * n);
+    std::vector<PrfSide>& sides = ctx->h_sides;     // kept by the context: its upload is queued
+    sides.assign(2 * n, PrfSide{});
     int64_t row_bytes = 0;
     int colblk = 0;
     for (int k = 0; k < n; ++k)
