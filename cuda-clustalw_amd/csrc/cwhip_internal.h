@@ -11,15 +11,22 @@
 #include "cw_pairwise_kernels.h"
 #include "cw_profile_kernels.h"
 
+// Buffers grow on demand: the first allocation is the size asked for, a later
+// growth at least half again the current size, so a run of slowly growing
+// batches (the steps of a progressive alignment) reallocates a few times
+// rather than at every batch (a device free synchronises the device).
+inline size_t grown(size_t have, size_t count) { return have ? std::max(count, have + have / 2) : count; }
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
+        const size_t want = grown(n, count);
         if (p) { (void)hipFree(p); p = nullptr; n = 0; }
-        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) n = count;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
         return e;
     }
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
@@ -33,9 +40,10 @@ struct PinBuf {
     size_t n = 0;
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
+        const size_t want = grown(n, count);
         if (p) { (void)hipHostFree(p); p = nullptr; n = 0; }
-        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
-        if (e == hipSuccess) n = count;
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = want;
         return e;
     }
     void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }

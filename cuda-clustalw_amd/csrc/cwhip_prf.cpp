@@ -742,7 +742,9 @@ int run_built_batch(cwhip_ctx* ctx, const cwhip_msa_params& P, const Seqs& seqs,
 }
 
 // progTracepath (:1023-1062), addGGaps (:818-934) and the write-back of :464-493.
-void apply_step(Step& st, Seqs& seqs)
+// The members' rows are independent: with `par`, a large group's rows are
+// rewritten on the host pool.
+void apply_step(Step& st, Seqs& seqs, bool par = false)
 {
     std::vector<uint8_t> path1, path2;
     path1.reserve(st.L1 + st.L2 + 2);
@@ -759,7 +761,8 @@ void apply_step(Step& st, Seqs& seqs)
         }
     }
     const int alen = (int)path1.size();
-    for (int k = 0; k < st.n1 + st.n2; ++k) {
+    auto row = [&](size_t kk) {
+        const int k = (int)kk;
         const bool g1 = k < st.n1;
         const std::vector<uint8_t>& path = g1 ? path1 : path2;
         const int rowlen = g1 ? st.L1 : st.L2;
@@ -775,7 +778,11 @@ void apply_step(Step& st, Seqs& seqs)
             }
         }
         seqs[st.members[k]].swap(out);
-    }
+    };
+    const size_t nrow = (size_t)(st.n1 + st.n2);
+    if (par && nrow * (size_t)alen > (1u << 16)) parallel_for(nrow, row);
+    else
+        for (size_t k = 0; k < nrow; ++k) row(k);
 }
 
 // residues (codes other than GAP1 / GAP2) of each row
@@ -995,7 +1002,9 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         const int rc = run_built_batch(ctx, P, seqs, live, true, lo);
         if (rc != CWHIP_OK) return rc;
         ++batches;
-        parallel_for(live.size(), [&](size_t q) { apply_step(*live[q], seqs); });
+        if (live.size() >= 4) parallel_for(live.size(), [&](size_t q) { apply_step(*live[q], seqs); });
+        else
+            for (Step* st : live) apply_step(*st, seqs, true);
         calls += (int)live.size();
     }
     // Second pass: the delayed (divergent) sequences one by one (:159-335).
@@ -1051,7 +1060,7 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
             const int rc = run_built_batch(ctx, P, seqs, {&st}, true, lo);
             if (rc != CWHIP_OK) return rc;
             ++batches;
-            apply_step(st, seqs);
+            apply_step(st, seqs, true);
             ++calls;
         }
         out_order[ix++] = iseq;
