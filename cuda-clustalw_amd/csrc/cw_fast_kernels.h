@@ -29,9 +29,15 @@ struct FastArgs {
 hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a);
 hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a);
 
-// LDS bytes cw_fast_wave needs for sequences of up to L residues
-inline size_t cwf_wave_lds(int L, int limit, int64_t acc_len)
+// LDS bytes cw_fast_wave needs for sequences of up to L residues: the
+// fragments (8 bytes each), the diagonal counts and list (int16), the k-tuple
+// codes of both sequences, then one region that holds first the position
+// lists and the diagonal flags, later the fragment score buckets (scores
+// 0 .. ktup L, 4 bytes each).
+inline size_t cwf_wave_lds(int L, int limit, int64_t acc_len, int ktup)
 {
     const int64_t D = 2 * (int64_t)L + 2;
-    return (size_t)(8 * acc_len + 2 * (D + 4 * ((int64_t)L + 2) + 2 * ((int64_t)limit + 2) + D) + D + 16);
+    const int64_t lists = 4 * ((int64_t)L + 2) + 4 * ((int64_t)limit + 2) + D;
+    const int64_t buckets = 4 * ((int64_t)ktup * L + 2);
+    return (size_t)(8 * acc_len + 4 * D + 4 * ((int64_t)L + 2) + (lists > buckets ? lists : buckets) + 16);
 }

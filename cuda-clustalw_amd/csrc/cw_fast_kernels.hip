@@ -269,31 +269,37 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
 // order and skips unflagged diagonals; taking the flagged diagonals in
 // ascending order (vn2 = i + l2 - diagonal, descending) and keeping those whose
 // k-tuple matches visits the same (i, vn2) in the same order.
+// A fragment is one 8-byte record, so a step of the walk down the fragment
+// list is a single LDS read; the values read there are the same in every lane
+// and are taken into scalar registers, so the walk branches on the scalar unit.
 struct WaveLds {
-    uint32_t* an;                   // fragments: score << 16 | next in score order
-    uint32_t* ap;                   // fragments: start in sequence 1 << 16 | start in sequence 2
+    uint2* fr;                      // fragments: x = score << 16 | next in score order,
+                                    //            y = start in sequence 1 << 16 | start in sequence 2
     int16_t* displ;                 // [D] diagonal counts (two per 32-bit word while counting), then fragments
     int16_t *kt1, *kt2;             // k-tuple code + 1 per position, 0: none (positions 1..L)
+    int16_t* diag;                  // [D] diagonal index, then the flagged diagonals in order
+    // one region, first:
     int16_t *tp1, *tp2;             // previous position with the same k-tuple
     int16_t *pl1, *pl2;             // last position of each code
-    int16_t* diag;                  // [D] diagonal index, then the flagged diagonals in order
     uint8_t* slopes;                // [D]
+    // then, while the fragments are built:
+    uint32_t* bk;                   // per score: newest fragment | oldest fragment << 16 (0: none)
 };
 
 __device__ __forceinline__ WaveLds wave_lds(const FastArgs& a, unsigned char* base)
 {
     WaveLds w;
-    w.an = (uint32_t*)base;
-    w.ap = w.an + a.acc_len;
-    w.displ = (int16_t*)(w.ap + a.acc_len);
+    w.fr = (uint2*)base;
+    w.displ = (int16_t*)(w.fr + a.acc_len);
     w.kt1 = w.displ + a.wD;
     w.kt2 = w.kt1 + (a.wL + 2);
-    w.tp1 = w.kt2 + (a.wL + 2);
+    w.diag = w.kt2 + (a.wL + 2);
+    w.tp1 = w.diag + a.wD;          // 4-byte aligned: wD and wL + 2 + wL + 2 are even
     w.tp2 = w.tp1 + (a.wL + 2);
     w.pl1 = w.tp2 + (a.wL + 2);
     w.pl2 = w.pl1 + (a.limit + 2);
-    w.diag = w.pl2 + (a.limit + 2);
-    w.slopes = (uint8_t*)(w.diag + a.wD);
+    w.slopes = (uint8_t*)(w.pl2 + (a.limit + 2));
+    w.bk = (uint32_t*)w.tp1;
     return w;
 }
 
@@ -301,6 +307,8 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_RE
 
 __device__ __forceinline__ int fr_score(uint32_t v) { return (int)(v >> 16); }
 __device__ __forceinline__ int fr_next(uint32_t v) { return (int)(v & 0xffffu); }
+// a value every lane holds alike, in a scalar register
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
 {
@@ -310,7 +318,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
     const int mr = a.dna ? 3 : 24;
     for (;;) {
         const unsigned got = atomicAdd(a.next, lane == 0 ? 1u : 0u);
-        const int64_t k = (int64_t)__shfl((int)got, 0, 64);
+        const int64_t k = (int64_t)uni(got);         // lane 0's count: the pair of this wave
         if (k >= a.npairs) break;
         const int si = a.pairs[2 * k], sj = a.pairs[2 * k + 1];
         const int l1 = a.lens[si], l2 = a.lens[sj];
@@ -451,12 +459,16 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
             nf += __popcll(bal);
             wave_sync();
         }
-        int curr = 0, maxsofar = 0;
+        // the position lists and flags are done with: their region holds the
+        // score buckets from here on (scores 1 .. ktup l1)
+        const int smax = a.ktup * l1;
+        for (int q = lane; q <= smax; q += 64) w.bk[q] = 0u;
+        int curr = 0, maxsofar = 0, topscore = 0;
         bool stop = false;
-        w.an[0] = 0;                    // score 0: where putFrag's walk ends
+        w.fr[0] = make_uint2(0u, 0u);   // score 0: where the list ends
         wave_sync();
         for (int i = 1; i <= l1 - a.ktup + 1 && !stop; ++i) {
-            const int c = w.kt1[i];
+            const int c = (int)uni((uint32_t)(int)w.kt1[i]);
             if (!c) continue;
             for (int f0 = 0; f0 < nf && !stop; f0 += 64) {
                 const int fi = f0 + lane;
@@ -469,7 +481,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                 while (bal && !stop) {
                     const int b = __builtin_ctzll(bal);
                     bal &= bal - 1;
-                    const int osptr = w.diag[f0 + b];
+                    const int osptr = (int)uni((uint32_t)(int)w.diag[f0 + b]);
                     const int vn2 = i + l2 - osptr;
                     int fs = a.ktup, next = maxsofar;
                     for (;;) {                          // A-loop (:299-393), every lane alike
@@ -479,53 +491,86 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                                 stop = true;            // "(Partial alignment)"
                                 break;
                             }
-                            // putFrag (:523-559): into the list kept in descending score order
+                            // putFrag (:523-559): into the list kept in descending score
+                            // order.  The list is ordered by score, descending, and
+                            // among equal scores by fragment number, descending (a
+                            // new fragment goes before the first one with a score
+                            // <= its own), so the place the reference's walk finds
+                            // is read off the score buckets: the new fragment
+                            // follows the oldest fragment of the lowest score above
+                            // fs (end; none: it heads the list) and precedes the
+                            // newest fragment of the highest score <= fs (link;
+                            // none: the list's end).  Lanes 0..31 look at 32 scores
+                            // at and below fs, lanes 32..63 at 32 scores above it.
                             int link = 0, end = 0;
-                            if (!maxsofar) {
-                                link = 0;
-                            } else if (fs >= fr_score(w.an[maxsofar])) {
+                            if (maxsofar && fs >= topscore) {
                                 link = maxsofar;
-                            } else {
-                                int nx = maxsofar;
-                                uint32_t cur = w.an[nx];
-                                for (;;) {
-                                    end = nx;
-                                    nx = fr_next(cur);
-                                    cur = w.an[nx];
-                                    if (fs >= fr_score(cur)) break;
+                            } else if (maxsofar) {
+                                int dn = fs, up = fs + 1;
+                                bool need_dn = true, need_up = true;
+                                while (need_dn || need_up) {
+                                    const int sq = lane < 32 ? dn - 31 + lane : up + (lane - 32);
+                                    const bool look = lane < 32 ? (need_dn && sq >= 1) : (need_up && sq <= topscore);
+                                    const uint32_t v = look ? w.bk[sq] : 0u;
+                                    const uint64_t bal = __ballot(v != 0u);
+                                    const uint32_t bd = (uint32_t)bal, bu = (uint32_t)(bal >> 32);
+                                    if (need_dn) {
+                                        if (bd) {
+                                            const int l = 31 - __builtin_clz(bd);
+                                            link = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) & 0xffffu);
+                                            need_dn = false;
+                                        } else if (dn - 31 <= 1) {
+                                            need_dn = false;        // no score in 1 .. fs
+                                        } else {
+                                            dn -= 32;
+                                        }
+                                    }
+                                    if (need_up) {                  // found by topscore at the latest
+                                        if (bu) {
+                                            const int l = 32 + __builtin_ctz(bu);
+                                            end = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) >> 16);
+                                            need_up = false;
+                                        } else {
+                                            up += 32;
+                                        }
+                                    }
                                 }
-                                link = nx;
                             }
-                            const uint32_t endv = end ? w.an[end] : 0u;
+                            const uint32_t endv = end ? uni(w.fr[end].x) : 0u;
+                            const uint32_t bv = uni(w.bk[fs]);
                             wave_sync();
                             w.displ[osptr] = (int16_t)curr;
-                            w.an[curr] = ((uint32_t)fs << 16) | (uint32_t)link;
-                            w.ap[curr] = ((uint32_t)i << 16) | (uint32_t)vn2;
-                            if (end) w.an[end] = (endv & 0xffff0000u) | (uint32_t)curr;
+                            w.fr[curr] = make_uint2(((uint32_t)fs << 16) | (uint32_t)link,
+                                                    ((uint32_t)i << 16) | (uint32_t)vn2);
+                            if (end) w.fr[end].x = (endv & 0xffff0000u) | (uint32_t)curr;
+                            w.bk[fs] = (bv ? (bv & 0xffff0000u) : ((uint32_t)curr << 16)) | (uint32_t)curr;
                             wave_sync();
                             if (!maxsofar) maxsofar = 1;
                             else if (!end) maxsofar = curr;
+                            if (!end) topscore = fs;
                             break;
                         }
-                        const uint32_t pv = w.ap[next];
-                        const int t1 = (int)(pv >> 16), t2 = (int)(pv & 0xffffu);
+                        const uint2 rv = w.fr[next];
+                        const uint32_t rx = uni(rv.x), ry = uni(rv.y);
+                        const int t1 = (int)(ry >> 16), t2 = (int)(ry & 0xffffu);
                         bool rel;                       // fragRelPos (:561-582)
                         if (i - vn2 == t1 - t2) rel = t1 < i;
                         else rel = t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2;
                         if (!rel) {
-                            next = fr_next(w.an[next]);
+                            next = fr_next(rx);
                             continue;
                         }
-                        const int sn = fr_score(w.an[next]);
+                        const int sn = fr_score(rx);
                         if (i - vn2 == t1 - t2) {
                             fs = i > t1 + (a.ktup - 1) ? sn + a.ktup : sn + (i - t1);
                         } else {
-                            const int d = w.displ[osptr];
+                            const int d = (int)uni((uint32_t)(int)w.displ[osptr]);
                             int subt1;
                             if (d == 0) {
                                 subt1 = a.ktup;
                             } else {
-                                const int d1 = (int)(w.ap[d] >> 16), ds = fr_score(w.an[d]);
+                                const uint2 dv = w.fr[d];
+                                const int d1 = (int)(uni(dv.y) >> 16), ds = fr_score(uni(dv.x));
                                 subt1 = i > d1 + (a.ktup - 1) ? ds + a.ktup : ds + (i - d1);
                             }
                             const int subt2 = sn - a.wind_gap + a.ktup;
@@ -536,7 +581,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                 }
             }
         }
-        const int sc = maxsofar ? fr_score(w.an[maxsofar]) : 0;
+        const int sc = maxsofar ? fr_score(w.fr[maxsofar].x) : 0;
         if (lane == 0) {
             double score = 0.0;                         // FastPairwiseAlign.cpp:87-100
             if (sc != 0) {

@@ -86,8 +86,12 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     a.lane_ints = (rows + 31) / 32 * 32;                            // 128-byte aligned lanes
     // one wave per pair with its arrays in LDS where they fit (16-bit positions
     // and fragment numbers), else one lane per pair with its arrays in HBM
-    const size_t wlds = cwf_wave_lds(lmax, limit, a.acc_len);
-    const bool lane_kernel = !(wlds <= 160 * 1024 && 2 * (int64_t)max_aln + 2 <= 32767 && lmax <= 16000);
+    // (a fragment's score is at most ktup times its start in sequence 1 when the
+    // window gap is not negative, which bounds the score buckets and the 16-bit
+    // scores)
+    const size_t wlds = cwf_wave_lds(lmax, limit, a.acc_len, fp->ktup);
+    const bool lane_kernel = !(wlds <= 160 * 1024 && 2 * (int64_t)max_aln + 2 <= 32767 && lmax <= 16000 &&
+                               fp->wind_gap >= 0);
     if (lane_kernel) HIPCHK(ctx->f_scratch.ensure((size_t)(a.lane_ints * lanes)));
     HIPCHK(ctx->f_pairs.ensure(pairs.size()));
     HIPCHK(ctx->f_dist.ensure((size_t)np));
