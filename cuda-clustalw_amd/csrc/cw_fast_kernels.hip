@@ -307,6 +307,8 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_RE
 
 __device__ __forceinline__ int fr_score(uint32_t v) { return (int)(v >> 16); }
 __device__ __forceinline__ int fr_next(uint32_t v) { return (int)(v & 0xffffu); }
+// lane l receives src of lane l - 1, lane 0 receives old (DPP wave_shr:1)
+__device__ __forceinline__ int wave_shr1(int old, int src) { return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xf, 0xf, false); }
 // a value every lane holds alike, in a scalar register
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
@@ -464,6 +466,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
         const int smax = a.ktup * l1;
         for (int q = lane; q <= smax; q += 64) w.bk[q] = 0u;
         int curr = 0, maxsofar = 0, topscore = 0;
+        uint32_t top = 0, tx = 0, ty = 0;   // the list's first 64 fragments, one per lane
         bool stop = false;
         w.fr[0] = make_uint2(0u, 0u);   // score 0: where the list ends
         wave_sync();
@@ -483,84 +486,45 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     bal &= bal - 1;
                     const int osptr = (int)uni((uint32_t)(int)w.diag[f0 + b]);
                     const int vn2 = i + l2 - osptr;
-                    int fs = a.ktup, next = maxsofar;
-                    for (;;) {                          // A-loop (:299-393), every lane alike
-                        if (!next) {
-                            ++curr;
-                            if (curr >= 2 * a.max_aln) {
-                                stop = true;            // "(Partial alignment)"
-                                break;
-                            }
-                            // putFrag (:523-559): into the list kept in descending score
-                            // order.  The list is ordered by score, descending, and
-                            // among equal scores by fragment number, descending (a
-                            // new fragment goes before the first one with a score
-                            // <= its own), so the place the reference's walk finds
-                            // is read off the score buckets: the new fragment
-                            // follows the oldest fragment of the lowest score above
-                            // fs (end; none: it heads the list) and precedes the
-                            // newest fragment of the highest score <= fs (link;
-                            // none: the list's end).  Lanes 0..31 look at 32 scores
-                            // at and below fs, lanes 32..63 at 32 scores above it.
-                            int link = 0, end = 0;
-                            if (maxsofar && fs >= topscore) {
-                                link = maxsofar;
-                            } else if (maxsofar) {
-                                int dn = fs, up = fs + 1;
-                                bool need_dn = true, need_up = true;
-                                while (need_dn || need_up) {
-                                    const int sq = lane < 32 ? dn - 31 + lane : up + (lane - 32);
-                                    const bool look = lane < 32 ? (need_dn && sq >= 1) : (need_up && sq <= topscore);
-                                    const uint32_t v = look ? w.bk[sq] : 0u;
-                                    const uint64_t bal = __ballot(v != 0u);
-                                    const uint32_t bd = (uint32_t)bal, bu = (uint32_t)(bal >> 32);
-                                    if (need_dn) {
-                                        if (bd) {
-                                            const int l = 31 - __builtin_clz(bd);
-                                            link = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) & 0xffffu);
-                                            need_dn = false;
-                                        } else if (dn - 31 <= 1) {
-                                            need_dn = false;        // no score in 1 .. fs
-                                        } else {
-                                            dn -= 32;
-                                        }
-                                    }
-                                    if (need_up) {                  // found by topscore at the latest
-                                        if (bu) {
-                                            const int l = 32 + __builtin_ctz(bu);
-                                            end = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) >> 16);
-                                            need_up = false;
-                                        } else {
-                                            up += 32;
-                                        }
-                                    }
+                    // A-loop (:299-393): the first fragment, in list order, that
+                    // (i, vn2) can extend (fragRelPos, :561-582).  Lane l holds the
+                    // list's l-th fragment (top: its number, 0 past the list's end;
+                    // tx, ty: its record), so the first 64 are tested at once; the
+                    // rest of the list is walked in LDS.
+                    int fs = a.ktup, nx = 0;
+                    uint32_t nrx = 0, nry = 0;
+                    {
+                        const int t1v = (int)(ty >> 16), t2v = (int)(ty & 0xffffu);
+                        const bool relv = top != 0u && (i - vn2 == t1v - t2v
+                                                            ? t1v < i
+                                                            : (t1v + a.ktup - 1 < i && t2v + a.ktup - 1 < vn2));
+                        const uint64_t rb = __ballot(relv);
+                        if (rb) {
+                            const int l = __builtin_ctzll(rb);
+                            nx = __builtin_amdgcn_readlane((int)top, l);
+                            nrx = (uint32_t)__builtin_amdgcn_readlane((int)tx, l);
+                            nry = (uint32_t)__builtin_amdgcn_readlane((int)ty, l);
+                        } else {
+                            int next = fr_next((uint32_t)__builtin_amdgcn_readlane((int)tx, 63));
+                            while (next) {
+                                const uint2 rv = w.fr[next];
+                                const uint32_t rx = uni(rv.x), ry = uni(rv.y);
+                                const int t1 = (int)(ry >> 16), t2 = (int)(ry & 0xffffu);
+                                const bool rel = i - vn2 == t1 - t2 ? t1 < i
+                                                                    : (t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2);
+                                if (rel) {
+                                    nx = next;
+                                    nrx = rx;
+                                    nry = ry;
+                                    break;
                                 }
+                                next = fr_next(rx);
                             }
-                            const uint32_t endv = end ? uni(w.fr[end].x) : 0u;
-                            const uint32_t bv = uni(w.bk[fs]);
-                            wave_sync();
-                            w.displ[osptr] = (int16_t)curr;
-                            w.fr[curr] = make_uint2(((uint32_t)fs << 16) | (uint32_t)link,
-                                                    ((uint32_t)i << 16) | (uint32_t)vn2);
-                            if (end) w.fr[end].x = (endv & 0xffff0000u) | (uint32_t)curr;
-                            w.bk[fs] = (bv ? (bv & 0xffff0000u) : ((uint32_t)curr << 16)) | (uint32_t)curr;
-                            wave_sync();
-                            if (!maxsofar) maxsofar = 1;
-                            else if (!end) maxsofar = curr;
-                            if (!end) topscore = fs;
-                            break;
                         }
-                        const uint2 rv = w.fr[next];
-                        const uint32_t rx = uni(rv.x), ry = uni(rv.y);
-                        const int t1 = (int)(ry >> 16), t2 = (int)(ry & 0xffffu);
-                        bool rel;                       // fragRelPos (:561-582)
-                        if (i - vn2 == t1 - t2) rel = t1 < i;
-                        else rel = t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2;
-                        if (!rel) {
-                            next = fr_next(rx);
-                            continue;
-                        }
-                        const int sn = fr_score(rx);
+                    }
+                    if (nx) {
+                        const int t1 = (int)(nry >> 16), t2 = (int)(nry & 0xffffu);
+                        const int sn = fr_score(nrx);
                         if (i - vn2 == t1 - t2) {
                             fs = i > t1 + (a.ktup - 1) ? sn + a.ktup : sn + (i - t1);
                         } else {
@@ -576,8 +540,93 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             const int subt2 = sn - a.wind_gap + a.ktup;
                             fs = subt2 > subt1 ? subt2 : subt1;
                         }
-                        next = 0;
                     }
+                    ++curr;
+                    if (curr >= 2 * a.max_aln) {
+                        stop = true;                    // "(Partial alignment)"
+                        break;
+                    }
+                    // putFrag (:523-559): into the list kept in descending score
+                    // order.  The list is ordered by score, descending, and
+                    // among equal scores by fragment number, descending (a
+                    // new fragment goes before the first one with a score
+                    // <= its own), so the place the reference's walk finds
+                    // is read off the score buckets: the new fragment
+                    // follows the oldest fragment of the lowest score above
+                    // fs (end; none: it heads the list) and precedes the
+                    // newest fragment of the highest score <= fs (link;
+                    // none: the list's end).  Lanes 0..31 look at 32 scores
+                    // at and below fs, lanes 32..63 at 32 scores above it.
+                    int link = 0, end = 0;
+                    if (maxsofar && fs >= topscore) {
+                        link = maxsofar;
+                    } else if (maxsofar) {
+                        int dn = fs, up = fs + 1;
+                        bool need_dn = true, need_up = true;
+                        while (need_dn || need_up) {
+                            const int sq = lane < 32 ? dn - 31 + lane : up + (lane - 32);
+                            const bool look = lane < 32 ? (need_dn && sq >= 1) : (need_up && sq <= topscore);
+                            const uint32_t v = look ? w.bk[sq] : 0u;
+                            const uint64_t sb = __ballot(v != 0u);
+                            const uint32_t bd = (uint32_t)sb, bu = (uint32_t)(sb >> 32);
+                            if (need_dn) {
+                                if (bd) {
+                                    const int l = 31 - __builtin_clz(bd);
+                                    link = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) & 0xffffu);
+                                    need_dn = false;
+                                } else if (dn - 31 <= 1) {
+                                    need_dn = false;        // no score in 1 .. fs
+                                } else {
+                                    dn -= 32;
+                                }
+                            }
+                            if (need_up) {                  // found by topscore at the latest
+                                if (bu) {
+                                    const int l = 32 + __builtin_ctz(bu);
+                                    end = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) >> 16);
+                                    need_up = false;
+                                } else {
+                                    up += 32;
+                                }
+                            }
+                        }
+                    }
+                    const uint32_t endv = end ? uni(w.fr[end].x) : 0u;
+                    const uint32_t bv = uni(w.bk[fs]);
+                    // list position of the new fragment where it is among the first 64
+                    int pos = 0;
+                    if (end) {
+                        const uint64_t eb = __ballot(top == (uint32_t)end);
+                        pos = eb ? __builtin_ctzll(eb) + 1 : 65;
+                    }
+                    wave_sync();
+                    const uint32_t nrec_x = ((uint32_t)fs << 16) | (uint32_t)link;
+                    const uint32_t nrec_y = ((uint32_t)i << 16) | (uint32_t)vn2;
+                    w.displ[osptr] = (int16_t)curr;
+                    w.fr[curr] = make_uint2(nrec_x, nrec_y);
+                    if (end) w.fr[end].x = (endv & 0xffff0000u) | (uint32_t)curr;
+                    w.bk[fs] = (bv ? (bv & 0xffff0000u) : ((uint32_t)curr << 16)) | (uint32_t)curr;
+                    wave_sync();
+                    // the same in the lanes: end's successor, and the new fragment
+                    // at lane pos with the ones after it one lane further on
+                    const uint32_t st = (uint32_t)wave_shr1(0, (int)top);
+                    const uint32_t sx = (uint32_t)wave_shr1(0, (int)tx);
+                    const uint32_t sy = (uint32_t)wave_shr1(0, (int)ty);
+                    if (lane == pos - 1) tx = (tx & 0xffff0000u) | (uint32_t)curr;
+                    if (pos < 64) {
+                        if (lane > pos) {
+                            top = st;
+                            tx = sx;
+                            ty = sy;
+                        } else if (lane == pos) {
+                            top = (uint32_t)curr;
+                            tx = nrec_x;
+                            ty = nrec_y;
+                        }
+                    }
+                    if (!maxsofar) maxsofar = 1;
+                    else if (!end) maxsofar = curr;
+                    if (!end) topscore = fs;
                 }
             }
         }
