@@ -398,12 +398,31 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     continue;
                 }
                 int i = lst[p] - 1, j = ust[p];
+                const int top_p = ust[p];
                 const int piv = w.displ[j];
                 while (i < j) {
-                    for (i = i + 1; w.displ[i] < piv; i++)
-                        ;
-                    for (j = j - 1; j > i; j--)
-                        if (w.displ[j] <= piv) break;
+                    // the two scans, 64 positions at a time: i to the first
+                    // position after i with a count >= piv (the pivot at top_p
+                    // stops it), j to the last one in (i, j) with a count <= piv,
+                    // else to i
+                    for (int b = i + 1;; b += 64) {
+                        const int q = b + lane;
+                        const uint64_t hb = __ballot(q <= top_p && w.displ[q] >= piv);
+                        if (hb) {
+                            i = b + __builtin_ctzll(hb);
+                            break;
+                        }
+                    }
+                    int jn = i;
+                    for (int e = j - 1; e > i; e -= 64) {
+                        const int q = e - lane;
+                        const uint64_t hb = __ballot(q > i && w.displ[q] <= piv);
+                        if (hb) {
+                            jn = e - __builtin_ctzll(hb);
+                            break;
+                        }
+                    }
+                    j = jn;
                     if (i < j) {
                         const int16_t ti = w.displ[i], tj = w.displ[j];
                         const int16_t ui = w.diag[i], uj = w.diag[j];
@@ -490,7 +509,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     // (i, vn2) can extend (fragRelPos, :561-582).  Lane l holds the
                     // list's l-th fragment (top: its number, 0 past the list's end;
                     // tx, ty: its record), so the first 64 are tested at once; the
-                    // rest of the list is walked in LDS.
+                    // rest of the list is searched by all lanes over the records.
                     int fs = a.ktup, nx = 0;
                     uint32_t nrx = 0, nry = 0;
                     {
@@ -504,21 +523,32 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             nx = __builtin_amdgcn_readlane((int)top, l);
                             nrx = (uint32_t)__builtin_amdgcn_readlane((int)tx, l);
                             nry = (uint32_t)__builtin_amdgcn_readlane((int)ty, l);
-                        } else {
-                            int next = fr_next((uint32_t)__builtin_amdgcn_readlane((int)tx, 63));
-                            while (next) {
-                                const uint2 rv = w.fr[next];
-                                const uint32_t rx = uni(rv.x), ry = uni(rv.y);
-                                const int t1 = (int)(ry >> 16), t2 = (int)(ry & 0xffffu);
+                        } else if (fr_next((uint32_t)__builtin_amdgcn_readlane((int)tx, 63)) != 0) {
+                            // past the 64th: list order is that of score << 16 |
+                            // number, descending, so the first fragment after the
+                            // 64th that (i, vn2) can extend has the largest such key
+                            // below the 64th's among all fragments
+                            const uint32_t key63 = ((uint32_t)__builtin_amdgcn_readlane((int)tx, 63) & 0xffff0000u) |
+                                                   (uint32_t)__builtin_amdgcn_readlane((int)top, 63);
+                            uint32_t best = 0;
+                            for (int f = 1 + lane; f <= curr; f += 64) {
+                                const uint2 r = w.fr[f];
+                                const uint32_t key = (r.x & 0xffff0000u) | (uint32_t)f;
+                                const int t1 = (int)(r.y >> 16), t2 = (int)(r.y & 0xffffu);
                                 const bool rel = i - vn2 == t1 - t2 ? t1 < i
                                                                     : (t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2);
-                                if (rel) {
-                                    nx = next;
-                                    nrx = rx;
-                                    nry = ry;
-                                    break;
-                                }
-                                next = fr_next(rx);
+                                if (rel && key < key63 && key > best) best = key;
+                            }
+                            for (int o = 32; o; o >>= 1) {
+                                const uint32_t q = (uint32_t)__shfl_xor((int)best, o, 64);
+                                best = q > best ? q : best;
+                            }
+                            best = uni(best);
+                            if (best) {
+                                nx = (int)(best & 0xffffu);
+                                const uint2 rv = w.fr[nx];
+                                nrx = uni(rv.x);
+                                nry = uni(rv.y);
                             }
                         }
                     }
