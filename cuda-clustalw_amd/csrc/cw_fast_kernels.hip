@@ -486,24 +486,32 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
         for (int q = lane; q <= smax; q += 64) w.bk[q] = 0u;
         int curr = 0, maxsofar = 0, topscore = 0;
         uint32_t top = 0, tx = 0, ty = 0;   // the list's first 64 fragments, one per lane
+        // the first 64 flagged diagonals one per lane, with the latest fragment on
+        // each (its number, and score << 16 | start in sequence 1)
+        const int dg = lane < nf ? (int)w.diag[lane] : 0;
+        uint32_t fl_d = 0, fl_rec = 0;
+        int k1 = 0;                         // k-tuple codes of 64 positions of sequence 1
         bool stop = false;
         w.fr[0] = make_uint2(0u, 0u);   // score 0: where the list ends
         wave_sync();
         for (int i = 1; i <= l1 - a.ktup + 1 && !stop; ++i) {
-            const int c = (int)uni((uint32_t)(int)w.kt1[i]);
+            if (((i - 1) & 63) == 0) k1 = i + lane <= l1 ? (int)w.kt1[i + lane] : 0;
+            const int c = __builtin_amdgcn_readlane(k1, (i - 1) & 63);
             if (!c) continue;
             for (int f0 = 0; f0 < nf && !stop; f0 += 64) {
                 const int fi = f0 + lane;
                 bool m = false;
                 if (fi < nf) {
-                    const int v2c = i + l2 - w.diag[fi];
+                    const int v2c = i + l2 - (f0 == 0 ? dg : (int)w.diag[fi]);
                     m = v2c >= 1 && v2c <= l2 && w.kt2[v2c] == c;
                 }
                 uint64_t bal = __ballot(m);
                 while (bal && !stop) {
                     const int b = __builtin_ctzll(bal);
                     bal &= bal - 1;
-                    const int osptr = (int)uni((uint32_t)(int)w.diag[f0 + b]);
+                    const int fb = f0 + b;          // the diagonal's place in the flagged list
+                    const int osptr = f0 == 0 ? __builtin_amdgcn_readlane(dg, b)
+                                              : (int)uni((uint32_t)(int)w.diag[fb]);
                     const int vn2 = i + l2 - osptr;
                     // A-loop (:299-393): the first fragment, in list order, that
                     // (i, vn2) can extend (fragRelPos, :561-582).  Lane l holds the
@@ -558,13 +566,24 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                         if (i - vn2 == t1 - t2) {
                             fs = i > t1 + (a.ktup - 1) ? sn + a.ktup : sn + (i - t1);
                         } else {
-                            const int d = (int)uni((uint32_t)(int)w.displ[osptr]);
+                            // the latest fragment on this diagonal (displ[osptr])
+                            int d;
+                            uint32_t drec = 0;
+                            if (fb < 64) {
+                                d = __builtin_amdgcn_readlane((int)fl_d, fb);
+                                drec = (uint32_t)__builtin_amdgcn_readlane((int)fl_rec, fb);
+                            } else {
+                                d = (int)uni((uint32_t)(int)w.displ[osptr]);
+                                if (d) {
+                                    const uint2 dv = w.fr[d];
+                                    drec = (uni(dv.x) & 0xffff0000u) | (uni(dv.y) >> 16);
+                                }
+                            }
                             int subt1;
                             if (d == 0) {
                                 subt1 = a.ktup;
                             } else {
-                                const uint2 dv = w.fr[d];
-                                const int d1 = (int)(uni(dv.y) >> 16), ds = fr_score(uni(dv.x));
+                                const int d1 = (int)(drec & 0xffffu), ds = (int)(drec >> 16);
                                 subt1 = i > d1 + (a.ktup - 1) ? ds + a.ktup : ds + (i - d1);
                             }
                             const int subt2 = sn - a.wind_gap + a.ktup;
@@ -588,8 +607,10 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                     // none: the list's end).  Lanes 0..31 look at 32 scores
                     // at and below fs, lanes 32..63 at 32 scores above it.
                     int link = 0, end = 0;
+                    uint32_t bv = 0;                    // the bucket of fs (none above topscore)
                     if (maxsofar && fs >= topscore) {
                         link = maxsofar;
+                        if (fs == topscore) bv = uni(w.bk[fs]);
                     } else if (maxsofar) {
                         int dn = fs, up = fs + 1;
                         bool need_dn = true, need_up = true;
@@ -599,6 +620,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             const uint32_t v = look ? w.bk[sq] : 0u;
                             const uint64_t sb = __ballot(v != 0u);
                             const uint32_t bd = (uint32_t)sb, bu = (uint32_t)(sb >> 32);
+                            if (dn == fs) bv = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
                             if (need_dn) {
                                 if (bd) {
                                     const int l = 31 - __builtin_clz(bd);
@@ -621,13 +643,14 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             }
                         }
                     }
-                    const uint32_t endv = end ? uni(w.fr[end].x) : 0u;
-                    const uint32_t bv = uni(w.bk[fs]);
-                    // list position of the new fragment where it is among the first 64
+                    // list position of the new fragment where it is among the first
+                    // 64, and end's record (from its lane where it is one of them)
                     int pos = 0;
+                    uint32_t endv = 0;
                     if (end) {
                         const uint64_t eb = __ballot(top == (uint32_t)end);
                         pos = eb ? __builtin_ctzll(eb) + 1 : 65;
+                        endv = eb ? (uint32_t)__builtin_amdgcn_readlane((int)tx, pos - 1) : uni(w.fr[end].x);
                     }
                     wave_sync();
                     const uint32_t nrec_x = ((uint32_t)fs << 16) | (uint32_t)link;
@@ -653,6 +676,10 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             tx = nrec_x;
                             ty = nrec_y;
                         }
+                    }
+                    if (lane == fb) {
+                        fl_d = (uint32_t)curr;
+                        fl_rec = ((uint32_t)fs << 16) | (uint32_t)i;
                     }
                     if (!maxsofar) maxsofar = 1;
                     else if (!end) maxsofar = curr;
