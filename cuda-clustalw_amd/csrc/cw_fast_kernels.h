@@ -17,6 +17,13 @@ struct FastArgs {
     int32_t* scratch;
     int64_t lanes, lane_ints;
     unsigned int* next;       // pair counter (0 at launch)
+    // cw_fast_wave in two passes: the first lists the pairs that run out of
+    // fragment slots (ovf_list, count ovf_n; null: none can), the second takes
+    // those (plist, count plist_n; null: work item k is pair k)
+    int32_t* ovf_list;
+    unsigned int* ovf_n;
+    const int32_t* plist;
+    const unsigned int* plist_n;
     int32_t dna, ktup, window, wind_gap, signif, percent, max_aln;
     int32_t limit;            // k-tuple codes: 25^ktup (protein) or 4^ktup (DNA)
     int64_t o_zza, o_zzb;     // per position: the previous position with its k-tuple (sequence 1, 2)
@@ -29,8 +36,8 @@ struct FastArgs {
 hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a);
 hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a);
 
-// LDS bytes cw_fast_wave needs for sequences of up to L residues: the
-// fragments (8 bytes each), the diagonal counts and list (int16), the k-tuple
+// LDS bytes cw_fast_wave needs for sequences of up to L residues and acc_len
+// (even) fragment slots: the fragments (6 bytes each), the diagonal counts and list (int16), the k-tuple
 // codes of both sequences, then one region that holds first the position
 // lists and the diagonal flags, later the fragment score buckets (scores
 // 0 .. ktup L, 4 bytes each).
@@ -39,5 +46,5 @@ inline size_t cwf_wave_lds(int L, int limit, int64_t acc_len, int ktup)
     const int64_t D = 2 * (int64_t)L + 2;
     const int64_t lists = 4 * ((int64_t)L + 2) + 4 * ((int64_t)limit + 2) + D;
     const int64_t buckets = 4 * ((int64_t)ktup * L + 2);
-    return (size_t)(8 * acc_len + 4 * D + 4 * ((int64_t)L + 2) + (lists > buckets ? lists : buckets) + 16);
+    return (size_t)(6 * acc_len + 4 * D + 4 * ((int64_t)L + 2) + (lists > buckets ? lists : buckets) + 16);
 }

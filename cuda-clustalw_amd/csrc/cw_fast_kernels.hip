@@ -269,12 +269,13 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
 // order and skips unflagged diagonals; taking the flagged diagonals in
 // ascending order (vn2 = i + l2 - diagonal, descending) and keeping those whose
 // k-tuple matches visits the same (i, vn2) in the same order.
-// A fragment is one 8-byte record, so a step of the walk down the fragment
-// list is a single LDS read; the values read there are the same in every lane
-// and are taken into scalar registers, so the walk branches on the scalar unit.
+// The reference keeps the fragments in a list linked in score order; here
+// that order is implicit (see the fragment loop), so a fragment is its score
+// and its start: 6 bytes.  Values every lane reads alike are taken into scalar
+// registers, so the sequential parts branch on the scalar unit.
 struct WaveLds {
-    uint2* fr;                      // fragments: x = score << 16 | next in score order,
-                                    //            y = start in sequence 1 << 16 | start in sequence 2
+    uint32_t* fp;                   // fragments: start in sequence 1 << 16 | start in sequence 2
+    uint16_t* fsc;                  // fragments: score
     int16_t* displ;                 // [D] diagonal counts (two per 32-bit word while counting), then fragments
     int16_t *kt1, *kt2;             // k-tuple code + 1 per position, 0: none (positions 1..L)
     int16_t* diag;                  // [D] diagonal index, then the flagged diagonals in order
@@ -289,8 +290,9 @@ struct WaveLds {
 __device__ __forceinline__ WaveLds wave_lds(const FastArgs& a, unsigned char* base)
 {
     WaveLds w;
-    w.fr = (uint2*)base;
-    w.displ = (int16_t*)(w.fr + a.acc_len);
+    w.fp = (uint32_t*)base;
+    w.fsc = (uint16_t*)(w.fp + a.acc_len);
+    w.displ = (int16_t*)(w.fsc + a.acc_len);   // acc_len is even
     w.kt1 = w.displ + a.wD;
     w.kt2 = w.kt1 + (a.wL + 2);
     w.diag = w.kt2 + (a.wL + 2);
@@ -306,7 +308,6 @@ __device__ __forceinline__ WaveLds wave_lds(const FastArgs& a, unsigned char* ba
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
 
 __device__ __forceinline__ int fr_score(uint32_t v) { return (int)(v >> 16); }
-__device__ __forceinline__ int fr_next(uint32_t v) { return (int)(v & 0xffffu); }
 // lane l receives src of lane l - 1, lane 0 receives old (DPP wave_shr:1)
 __device__ __forceinline__ int wave_shr1(int old, int src) { return __builtin_amdgcn_update_dpp(old, src, 0x138, 0xf, 0xf, false); }
 // a value every lane holds alike, in a scalar register
@@ -318,10 +319,13 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
     const WaveLds w = wave_lds(a, lds);
     const int lane = threadIdx.x;
     const int mr = a.dna ? 3 : 24;
+    // a second pass takes the pairs a first one with fewer fragment slots listed
+    const int64_t nwork = a.plist ? (int64_t)*a.plist_n : a.npairs;
     for (;;) {
         const unsigned got = atomicAdd(a.next, lane == 0 ? 1u : 0u);
-        const int64_t k = (int64_t)uni(got);         // lane 0's count: the pair of this wave
-        if (k >= a.npairs) break;
+        const int64_t kk = (int64_t)uni(got);        // lane 0's count: the work item of this wave
+        if (kk >= nwork) break;
+        const int64_t k = a.plist ? (int64_t)a.plist[kk] : kk;
         const int si = a.pairs[2 * k], sj = a.pairs[2 * k + 1];
         const int l1 = a.lens[si], l2 = a.lens[sj];
         const uint8_t* s1 = a.codes + a.offs[si];
@@ -491,8 +495,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
         const int dg = lane < nf ? (int)w.diag[lane] : 0;
         uint32_t fl_d = 0, fl_rec = 0;
         int k1 = 0;                         // k-tuple codes of 64 positions of sequence 1
-        bool stop = false;
-        w.fr[0] = make_uint2(0u, 0u);   // score 0: where the list ends
+        bool stop = false, ovf = false;
         wave_sync();
         for (int i = 1; i <= l1 - a.ktup + 1 && !stop; ++i) {
             if (((i - 1) & 63) == 0) k1 = i + lane <= l1 ? (int)w.kt1[i + lane] : 0;
@@ -531,7 +534,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             nx = __builtin_amdgcn_readlane((int)top, l);
                             nrx = (uint32_t)__builtin_amdgcn_readlane((int)tx, l);
                             nry = (uint32_t)__builtin_amdgcn_readlane((int)ty, l);
-                        } else if (fr_next((uint32_t)__builtin_amdgcn_readlane((int)tx, 63)) != 0) {
+                        } else if (curr > 64) {
                             // past the 64th: list order is that of score << 16 |
                             // number, descending, so the first fragment after the
                             // 64th that (i, vn2) can extend has the largest such key
@@ -540,9 +543,9 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                                                    (uint32_t)__builtin_amdgcn_readlane((int)top, 63);
                             uint32_t best = 0;
                             for (int f = 1 + lane; f <= curr; f += 64) {
-                                const uint2 r = w.fr[f];
-                                const uint32_t key = (r.x & 0xffff0000u) | (uint32_t)f;
-                                const int t1 = (int)(r.y >> 16), t2 = (int)(r.y & 0xffffu);
+                                const uint32_t r = w.fp[f];
+                                const uint32_t key = ((uint32_t)w.fsc[f] << 16) | (uint32_t)f;
+                                const int t1 = (int)(r >> 16), t2 = (int)(r & 0xffffu);
                                 const bool rel = i - vn2 == t1 - t2 ? t1 < i
                                                                     : (t1 + a.ktup - 1 < i && t2 + a.ktup - 1 < vn2);
                                 if (rel && key < key63 && key > best) best = key;
@@ -554,9 +557,8 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             best = uni(best);
                             if (best) {
                                 nx = (int)(best & 0xffffu);
-                                const uint2 rv = w.fr[nx];
-                                nrx = uni(rv.x);
-                                nry = uni(rv.y);
+                                nrx = best & 0xffff0000u;
+                                nry = uni(w.fp[nx]);
                             }
                         }
                     }
@@ -574,10 +576,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                                 drec = (uint32_t)__builtin_amdgcn_readlane((int)fl_rec, fb);
                             } else {
                                 d = (int)uni((uint32_t)(int)w.displ[osptr]);
-                                if (d) {
-                                    const uint2 dv = w.fr[d];
-                                    drec = (uni(dv.x) & 0xffff0000u) | (uni(dv.y) >> 16);
-                                }
+                                if (d) drec = (uni((uint32_t)w.fsc[d]) << 16) | (uni(w.fp[d]) >> 16);
                             }
                             int subt1;
                             if (d == 0) {
@@ -595,77 +594,53 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                         stop = true;                    // "(Partial alignment)"
                         break;
                     }
+                    if (curr >= a.acc_len) {
+                        stop = ovf = true;              // no slot left: the pair goes to the second pass
+                        break;
+                    }
                     // putFrag (:523-559): into the list kept in descending score
-                    // order.  The list is ordered by score, descending, and
-                    // among equal scores by fragment number, descending (a
-                    // new fragment goes before the first one with a score
-                    // <= its own), so the place the reference's walk finds
-                    // is read off the score buckets: the new fragment
-                    // follows the oldest fragment of the lowest score above
-                    // fs (end; none: it heads the list) and precedes the
-                    // newest fragment of the highest score <= fs (link;
-                    // none: the list's end).  Lanes 0..31 look at 32 scores
-                    // at and below fs, lanes 32..63 at 32 scores above it.
-                    int link = 0, end = 0;
-                    uint32_t bv = 0;                    // the bucket of fs (none above topscore)
-                    if (maxsofar && fs >= topscore) {
-                        link = maxsofar;
-                        if (fs == topscore) bv = uni(w.bk[fs]);
-                    } else if (maxsofar) {
-                        int dn = fs, up = fs + 1;
-                        bool need_dn = true, need_up = true;
-                        while (need_dn || need_up) {
-                            const int sq = lane < 32 ? dn - 31 + lane : up + (lane - 32);
-                            const bool look = lane < 32 ? (need_dn && sq >= 1) : (need_up && sq <= topscore);
-                            const uint32_t v = look ? w.bk[sq] : 0u;
-                            const uint64_t sb = __ballot(v != 0u);
-                            const uint32_t bd = (uint32_t)sb, bu = (uint32_t)(sb >> 32);
-                            if (dn == fs) bv = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
-                            if (need_dn) {
-                                if (bd) {
-                                    const int l = 31 - __builtin_clz(bd);
-                                    link = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) & 0xffffu);
-                                    need_dn = false;
-                                } else if (dn - 31 <= 1) {
-                                    need_dn = false;        // no score in 1 .. fs
-                                } else {
-                                    dn -= 32;
-                                }
-                            }
-                            if (need_up) {                  // found by topscore at the latest
-                                if (bu) {
-                                    const int l = 32 + __builtin_ctz(bu);
-                                    end = (int)((uint32_t)__builtin_amdgcn_readlane((int)v, l) >> 16);
-                                    need_up = false;
-                                } else {
-                                    up += 32;
-                                }
+                    // order.  That list is ordered by score, descending, and among
+                    // equal scores by fragment number, descending (a new fragment
+                    // goes before the first one with a score <= its own), so the
+                    // order is kept without links: the search above reads it off
+                    // the keys, and the first 64 places live in the lanes.  The new
+                    // fragment follows the oldest fragment of the lowest score above
+                    // fs (end; none: it heads the list), which the score buckets
+                    // give: lanes look at 64 scores from fs up.
+                    int end = 0;
+                    uint32_t bv = 0;                    // oldest fragment of score fs (0: none)
+                    if (maxsofar && fs < topscore) {
+                        for (int up = fs;; up += 64) {  // ends at topscore at the latest
+                            const int sq = up + lane;
+                            const uint32_t v = sq <= topscore ? w.bk[sq] : 0u;
+                            if (up == fs) bv = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+                            const uint64_t sb = __ballot(v != 0u && sq > fs);
+                            if (sb) {
+                                end = __builtin_amdgcn_readlane((int)v, __builtin_ctzll(sb));
+                                break;
                             }
                         }
+                    } else if (maxsofar && fs == topscore) {
+                        bv = uni(w.bk[fs]);
                     }
-                    // list position of the new fragment where it is among the first
-                    // 64, and end's record (from its lane where it is one of them)
+                    // list position of the new fragment where it is among the first 64
                     int pos = 0;
-                    uint32_t endv = 0;
                     if (end) {
                         const uint64_t eb = __ballot(top == (uint32_t)end);
                         pos = eb ? __builtin_ctzll(eb) + 1 : 65;
-                        endv = eb ? (uint32_t)__builtin_amdgcn_readlane((int)tx, pos - 1) : uni(w.fr[end].x);
                     }
                     wave_sync();
-                    const uint32_t nrec_x = ((uint32_t)fs << 16) | (uint32_t)link;
                     const uint32_t nrec_y = ((uint32_t)i << 16) | (uint32_t)vn2;
                     w.displ[osptr] = (int16_t)curr;
-                    w.fr[curr] = make_uint2(nrec_x, nrec_y);
-                    if (end) w.fr[end].x = (endv & 0xffff0000u) | (uint32_t)curr;
-                    w.bk[fs] = (bv ? (bv & 0xffff0000u) : ((uint32_t)curr << 16)) | (uint32_t)curr;
+                    w.fp[curr] = nrec_y;
+                    w.fsc[curr] = (uint16_t)fs;
+                    if (!bv) w.bk[fs] = (uint32_t)curr;
                     wave_sync();
-                    // the same in the lanes: end's successor, and the new fragment
-                    // at lane pos with the ones after it one lane further on
+                    // the same in the lanes: the new fragment at lane pos, the ones
+                    // after it one lane further on
                     const uint32_t st = (uint32_t)wave_shr1(0, (int)top);
                     const uint32_t sx = (uint32_t)wave_shr1(0, (int)tx);
                     const uint32_t sy = (uint32_t)wave_shr1(0, (int)ty);
-                    if (lane == pos - 1) tx = (tx & 0xffff0000u) | (uint32_t)curr;
                     if (pos < 64) {
                         if (lane > pos) {
                             top = st;
@@ -673,7 +648,7 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                             ty = sy;
                         } else if (lane == pos) {
                             top = (uint32_t)curr;
-                            tx = nrec_x;
+                            tx = (uint32_t)fs << 16;
                             ty = nrec_y;
                         }
                     }
@@ -687,8 +662,11 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
                 }
             }
         }
-        const int sc = maxsofar ? fr_score(w.fr[maxsofar].x) : 0;
-        if (lane == 0) {
+        const int sc = maxsofar ? topscore : 0;     // the list head's score
+        if (lane == 0 && ovf) {
+            const unsigned o = atomicAdd(a.ovf_n, 1u);
+            a.ovf_list[o] = (int32_t)k;
+        } else if (lane == 0) {
             double score = 0.0;                         // FastPairwiseAlign.cpp:87-100
             if (sc != 0) {
                 score = (double)sc;

@@ -1,8 +1,9 @@
 // cwhip_fast.cpp -- host side of the k-tuple distances (cwhip_fast_pairwise_range).
 //
 // FastPairwiseAlign::pairwiseAlign (pairwise/FastPairwiseAlign.cpp:20-145):
-// the pairs of the range in the reference's order, one lane of cw_fast_pairs
-// per pair, the distances back in that order.
+// the pairs of the range in the reference's order, one wave of cw_fast_wave
+// per pair with its arrays in LDS (one lane of cw_fast_pairs per pair, arrays
+// in HBM, where they do not fit), the distances back in that order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,10 +96,10 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     if (lane_kernel) HIPCHK(ctx->f_scratch.ensure((size_t)(a.lane_ints * lanes)));
     HIPCHK(ctx->f_pairs.ensure(pairs.size()));
     HIPCHK(ctx->f_dist.ensure((size_t)np));
-    HIPCHK(ctx->f_ctr.ensure(1));
+    HIPCHK(ctx->f_ctr.ensure(2));
     HIPCHK(hipMemcpyAsync(ctx->f_pairs.p, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->f_ctr.p, 0, sizeof(unsigned int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->f_ctr.p, 0, 2 * sizeof(unsigned int), ctx->stream));
     a.codes = ctx->d_codes.p;
     a.offs = ctx->d_offs.p;
     a.lens = ctx->d_lens.p;
@@ -106,13 +107,46 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     a.npairs = np;
     a.out_dist = ctx->f_dist.p;
     a.scratch = lane_kernel ? ctx->f_scratch.p : nullptr;
+    a.ovf_list = nullptr;
+    a.ovf_n = nullptr;
+    a.plist = nullptr;
+    a.plist_n = nullptr;
     a.next = ctx->f_ctr.p;
     a.wL = lmax;
     a.wD = 2 * lmax + 2;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (!lane_kernel) {
+        // Where the slots for every fragment a pair may make (2 maxAlnLength)
+        // leave room for two waves per CU only, a first pass runs three with
+        // at least two thirds of the slots; the few pairs that make more
+        // fragments than that are listed and run again with all the slots.
+        const int64_t full = a.acc_len;
+        const size_t third = (size_t)(160 * 1024 / 3) / 512 * 512;   // LDS is granted in 512-byte blocks
+        const size_t fixed = cwf_wave_lds(lmax, limit, 0, fp->ktup);
+        int64_t cap1 = full;
+        if (wlds > third && fixed < third) {
+            const int64_t c = (int64_t)((third - fixed) / 6) & ~(int64_t)1;
+            if (3 * c >= 2 * full) cap1 = c;
+        }
+        const int ncu = std::max(1, ctx->num_cu);
+        if (cap1 < full) {
+            HIPCHK(ctx->f_scratch.ensure((size_t)np));
+            a.acc_len = cap1;
+            a.ovf_list = ctx->f_scratch.p;
+            a.ovf_n = ctx->f_ctr.p + 1;
+            const size_t l1 = cwf_wave_lds(lmax, limit, cap1, fp->ktup);
+            const int per_cu = std::max<int>(1, (int)((160 * 1024) / l1));
+            HIPCHK(cwf_launch_wave((int)std::min<int64_t>(np, (int64_t)ncu * std::min(per_cu, 16)), l1,
+                                   ctx->stream, a));
+            HIPCHK(hipMemsetAsync(ctx->f_ctr.p, 0, sizeof(unsigned int), ctx->stream));
+            a.acc_len = full;
+            a.ovf_list = nullptr;
+            a.ovf_n = nullptr;
+            a.plist = ctx->f_scratch.p;
+            a.plist_n = ctx->f_ctr.p + 1;
+        }
         const int per_cu = std::max<int>(1, (int)((160 * 1024) / wlds));
-        const int64_t blocks = std::min<int64_t>(np, (int64_t)std::max(1, ctx->num_cu) * std::min(per_cu, 16));
+        const int64_t blocks = std::min<int64_t>(np, (int64_t)ncu * std::min(per_cu, 16));
         HIPCHK(cwf_launch_wave((int)blocks, wlds, ctx->stream, a));
     } else {
         HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
