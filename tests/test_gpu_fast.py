@@ -122,3 +122,16 @@ def test_fast_large_family_sample(ctx):
     print("fast large", flush=True)
     fam = seqgen.workload("large1523")
     _check(ctx, fam.seqs[:64], False)
+
+
+def test_fast_two_pass_overflow(ctx):
+    """Long low-complexity sequences make thousands of fragments per pair: with
+    sequences this long the first pass runs with about two thirds of the
+    fragment slots, these pairs run out and are run again with all of them,
+    some up to the partial-alignment limit; same results."""
+    print("fast two-pass", flush=True)
+    rng = np.random.default_rng(11)
+    seqs = [rng.choice(np.array([0, 4, 9], np.uint8), size=L) for L in (1400, 1350, 1420, 1380)]
+    seqs.append(rng.integers(0, 20, size=1410).astype(np.uint8))
+    d = _check(ctx, seqs, False)
+    assert d.min() >= 0.0 and d.max() <= 1.0
