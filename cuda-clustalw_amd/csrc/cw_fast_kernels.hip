@@ -1,10 +1,12 @@
-// cw_fast_kernels.hip -- gfx950 kernel of ClustalW's k-tuple distances
+// cw_fast_kernels.hip -- gfx950 kernels of ClustalW's k-tuple distances
 // (pairwise/FastPairwiseAlign.cpp, the -QUICKTREE path).
 //
 // The reference's pairAlign is a sequential walk per pair: k-tuple matches
 // counted per diagonal, a quicksort of the diagonal counts whose tie order
 // decides which diagonals are "top", then a fragment list kept in score order
-// that every match on a flagged diagonal walks.  Pairs are independent, so one
+// that every match on a flagged diagonal walks.  Pairs are independent.
+// cw_fast_wave (below) runs one pair per wave with its arrays in LDS; this
+// first kernel, cw_fast_pairs, is for pairs whose arrays do not fit there: one
 // lane runs one pair from start to end and the grid's lanes take pairs from a
 // counter.  A lane's arrays (positions per k-tuple, diagonals, fragments) lie
 // contiguously in HBM, so its sequential scans (clearing, the partition scans
@@ -256,15 +258,16 @@ extern "C" __global__ void __launch_bounds__(256) cw_fast_pairs(FastArgs a)
 }
 
 // ---- one wave per pair, arrays in LDS ------------------------------------------
-// The same walk with a pair's arrays in LDS (16-bit where values fit).  The
+// pairAlign with a pair's arrays in LDS (16-bit where values fit).  The
 // lane-parallel parts: k-tuple codes, clearing, the diagonal counts (LDS
 // atomics), the list of flagged diagonals, and for each position of sequence 1
 // the check of every flagged diagonal for a match.  The two position lists are
-// built by two lanes at once.  The sequential parts (the sort, the fragment
-// chain) are run by every lane alike on the same LDS values, so the wave never
-// diverges there; every lane reads what it needs into registers before a wave
-// barrier and writes after it, so no lane reads a location another lane of the
-// same step writes.  For a position i of sequence 1 the reference
+// built by two lanes at once.  The sequential parts (the sort, one match after
+// another) are run by every lane alike, so the wave never diverges there; the
+// sort's scans and the searches of the fragment list use all 64 lanes and a
+// ballot.  Every lane reads what it needs into registers before a wave barrier
+// and writes after it, so no lane reads a location another lane of the same
+// step writes.  For a position i of sequence 1 the reference
 // visits the positions vn2 of sequence 2 with the same k-tuple in descending
 // order and skips unflagged diagonals; taking the flagged diagonals in
 // ascending order (vn2 = i + l2 - diagonal, descending) and keeping those whose
