@@ -135,3 +135,44 @@ def test_fast_two_pass_overflow(ctx):
     seqs.append(rng.integers(0, 20, size=1410).astype(np.uint8))
     d = _check(ctx, seqs, False)
     assert d.min() >= 0.0 and d.max() <= 1.0
+
+
+def _random_case(seed):
+    """A seeded family and parameter set: DNA or protein, 2-7 sequences of 0-400
+    residues (about one case in seven: 1100-1500, the two-pass path), full or
+    2-5 letter alphabets, related or unrelated sequences, a gap code now and
+    then, every parameter the reference exposes, now and then a tiny
+    maxAlnLength."""
+    rng = np.random.default_rng(seed)
+    dna = bool(rng.integers(0, 2))
+    nseq = int(rng.integers(2, 8))
+    longmode = rng.random() < 0.15
+    alpha = int(rng.integers(2, 6)) if rng.random() < 0.4 else (4 if dna else 20)
+    pool = (np.array([0, 2, 6, 18, 19][:max(2, min(alpha, 5))], np.uint8) if dna
+            else rng.choice(25, size=alpha, replace=False).astype(np.uint8))
+    size = (lambda: int(rng.integers(1100, 1500))) if longmode else (lambda: int(rng.integers(0, 400)))
+    base = rng.choice(pool, size=size())
+    seqs = []
+    for _ in range(nseq):
+        L = size()
+        x = base[:L].copy() if (len(base) and rng.random() < 0.6) else rng.choice(pool, size=L)
+        if len(x) < L:
+            x = np.concatenate([x, rng.choice(pool, size=L - len(x))])
+        m = rng.random(len(x)) < rng.random() * 0.5
+        x[m] = rng.choice(pool, size=int(m.sum()))
+        if rng.random() < 0.1 and len(x):
+            x[rng.integers(0, len(x))] = 30
+        seqs.append(x.astype(np.uint8))
+    kw = dict(ktup=int(rng.integers(1, (4 if dna else 2) + 1)), window=int(rng.integers(0, 13)),
+              wind_gap=int(rng.integers(0, 10)), signif=int(rng.integers(1, 41)),
+              percent=int(rng.integers(0, 2)))
+    max_aln = int(rng.integers(1, 60)) if rng.random() < 0.15 else None
+    return seqs, dna, kw, max_aln
+
+
+def test_fast_random_cases(ctx):
+    """300 seeded random families and parameter sets, bit for bit."""
+    print("fast random", flush=True)
+    for seed in range(300):
+        seqs, dna, kw, max_aln = _random_case(seed)
+        _check(ctx, seqs, dna, max_aln=max_aln, **kw)
