@@ -150,3 +150,35 @@ def test_gpu_profile_align_step_by_step(ctx, setname):
     ref_order, ref_rows = parse_aln(text(setname + ".aln"))
     for i, nm in enumerate(names):
         assert row_text(seqs[i]) == ref_rows[nm], nm
+
+
+def test_gpu_msa_random_families(ctx):
+    """40 seeded random sets: two families of 1-19 and 1-14 members, lengths
+    15-350 and 0.3-1.3 times that, random divergence, a third of them DNA; the
+    device alignment equals the oracle's row for row."""
+    for seed in range(40):
+        rng = np.random.default_rng(seed)
+        dna = rng.random() < 0.3
+        n1, n2 = int(rng.integers(1, 20)), int(rng.integers(1, 15))
+        L = int(rng.integers(15, 350))
+        a = seqgen.family(seed * 7 + 1, n1, L, branch_mean=float(rng.uniform(0.1, 1.0)), name_prefix="fa")
+        b = seqgen.family(seed * 7 + 2, n2, max(5, int(L * rng.uniform(0.3, 1.3))),
+                          branch_mean=float(rng.uniform(0.3, 2.0)), name_prefix="fb")
+        seqs, names = a.seqs + b.seqs, a.names + b.names
+        if dna:
+            seqs = [np.array([(0, 2, 6, 18)[c % 4] for c in s], np.uint8) for s in seqs]
+        n = len(seqs)
+        print("msa random", seed, n, L, dna, flush=True)
+        codes = np.concatenate(seqs)
+        lens = np.array([len(s) for s in seqs], np.int32)
+        offs = np.zeros(n, np.int64)
+        offs[1:] = np.cumsum(lens[:-1])
+        pairs = O.range_pairs(n, 0, n, 0, n)
+        cnt, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring(dna))
+        dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+        w, sim, steps = O.guide_from_dnd(dnd, names)
+        r_rows, r_order, info = O.msa([s.astype(np.int32) for s in seqs], sim, w, steps, O.msa_params(dna))
+        rows, order, stats = ctx.msa(seqs, sim, w, steps, dna=dna)
+        np.testing.assert_array_equal(order, r_order)
+        for i in range(n):
+            np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
