@@ -24,6 +24,7 @@ struct FastArgs {
     unsigned int* ovf_n;
     const int32_t* plist;
     const unsigned int* plist_n;
+    int64_t lds_wave;         // cw_fast_wave: LDS bytes of each wave of a workgroup (16-byte multiple)
     int32_t dna, ktup, window, wind_gap, signif, percent, max_aln;
     int32_t limit;            // k-tuple codes: 25^ktup (protein) or 4^ktup (DNA)
     int64_t o_zza, o_zzb;     // per position: the previous position with its k-tuple (sequence 1, 2)
@@ -34,7 +35,7 @@ struct FastArgs {
 };
 
 hipError_t cwf_launch(int blocks, hipStream_t st, const FastArgs& a);
-hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a);
+hipError_t cwf_launch_wave(int blocks, int waves, size_t lds_bytes, hipStream_t st, const FastArgs& a);
 
 // LDS bytes cw_fast_wave needs for sequences of up to L residues and acc_len
 // (even) fragment slots: the fragments (6 bytes each), the diagonal counts and list (int16), the k-tuple

@@ -316,11 +316,12 @@ __device__ __forceinline__ int wave_shr1(int old, int src) { return __builtin_am
 // a value every lane holds alike, in a scalar register
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
-extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
+extern "C" __global__ void __launch_bounds__(256) cw_fast_wave(FastArgs a)
 {
+    // one or more waves per workgroup, each with its own lds_wave bytes
     extern __shared__ __align__(16) unsigned char lds[];
-    const WaveLds w = wave_lds(a, lds);
-    const int lane = threadIdx.x;
+    const WaveLds w = wave_lds(a, lds + (size_t)(threadIdx.x >> 6) * (size_t)a.lds_wave);
+    const int lane = threadIdx.x & 63;
     const int mr = a.dna ? 3 : 24;
     // a second pass takes the pairs a first one with fewer fragment slots listed
     const int64_t nwork = a.plist ? (int64_t)*a.plist_n : a.npairs;
@@ -681,12 +682,12 @@ extern "C" __global__ void __launch_bounds__(64) cw_fast_wave(FastArgs a)
     }
 }
 
-hipError_t cwf_launch_wave(int blocks, size_t lds_bytes, hipStream_t st, const FastArgs& a)
+hipError_t cwf_launch_wave(int blocks, int waves, size_t lds_bytes, hipStream_t st, const FastArgs& a)
 {
     hipError_t e = hipFuncSetAttribute((const void*)cw_fast_wave, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds_bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(cw_fast_wave, dim3(blocks), dim3(64), lds_bytes, st, a);
+    hipLaunchKernelGGL(cw_fast_wave, dim3(blocks), dim3(64 * waves), lds_bytes, st, a);
     return hipGetLastError();
 }
 

@@ -13,6 +13,22 @@
 #include "cw_fast_kernels.h"
 #include "cwhip_internal.h"
 
+// cw_fast_wave with `per_wave` LDS bytes per pair: as many waves per CU as
+// LDS holds (at most 16), grouped into workgroups of up to four so that the
+// waves of a CU are spread over its four SIMDs (with one-wave workgroups,
+// three waves per CU ran only 8% faster than two: they shared SIMDs).  Each
+// wave takes pairs from the counter on its own.
+static hipError_t launch_waves(cwhip_ctx* ctx, FastArgs& a, size_t per_wave, int64_t np)
+{
+    const size_t lw = (per_wave + 15) / 16 * 16;
+    const int per_cu = std::max<int>(1, std::min<int>(16, (int)((160 * 1024) / lw)));
+    const int wpg = std::min(per_cu, 4);
+    const int64_t groups = (int64_t)std::max(1, ctx->num_cu) * (per_cu / wpg);
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((np + wpg - 1) / wpg, groups));
+    a.lds_wave = (int64_t)lw;
+    return cwf_launch_wave((int)blocks, wpg, (size_t)wpg * lw, ctx->stream, a);
+}
+
 extern "C" {
 
 int cwhip_fast_params_default(int dna, cwhip_fast_params* fp)
@@ -111,6 +127,7 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
     a.ovf_n = nullptr;
     a.plist = nullptr;
     a.plist_n = nullptr;
+    a.lds_wave = 0;
     a.next = ctx->f_ctr.p;
     a.wL = lmax;
     a.wD = 2 * lmax + 2;
@@ -128,16 +145,12 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
             const int64_t c = (int64_t)((third - fixed) / 6) & ~(int64_t)1;
             if (3 * c >= 2 * full) cap1 = c;
         }
-        const int ncu = std::max(1, ctx->num_cu);
         if (cap1 < full) {
             HIPCHK(ctx->f_scratch.ensure((size_t)np));
             a.acc_len = cap1;
             a.ovf_list = ctx->f_scratch.p;
             a.ovf_n = ctx->f_ctr.p + 1;
-            const size_t l1 = cwf_wave_lds(lmax, limit, cap1, fp->ktup);
-            const int per_cu = std::max<int>(1, (int)((160 * 1024) / l1));
-            HIPCHK(cwf_launch_wave((int)std::min<int64_t>(np, (int64_t)ncu * std::min(per_cu, 16)), l1,
-                                   ctx->stream, a));
+            HIPCHK(launch_waves(ctx, a, cwf_wave_lds(lmax, limit, cap1, fp->ktup), np));
             HIPCHK(hipMemsetAsync(ctx->f_ctr.p, 0, sizeof(unsigned int), ctx->stream));
             a.acc_len = full;
             a.ovf_list = nullptr;
@@ -145,9 +158,7 @@ int cwhip_fast_pairwise_range(cwhip_ctx* ctx, const cwhip_fast_params* fp, int i
             a.plist = ctx->f_scratch.p;
             a.plist_n = ctx->f_ctr.p + 1;
         }
-        const int per_cu = std::max<int>(1, (int)((160 * 1024) / wlds));
-        const int64_t blocks = std::min<int64_t>(np, (int64_t)ncu * std::min(per_cu, 16));
-        HIPCHK(cwf_launch_wave((int)blocks, wlds, ctx->stream, a));
+        HIPCHK(launch_waves(ctx, a, wlds, np));
     } else {
         HIPCHK(cwf_launch((int)(lanes / 256), ctx->stream, a));
     }
