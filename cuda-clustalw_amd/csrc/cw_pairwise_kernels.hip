@@ -584,8 +584,8 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         nq = 0;
         wave_fence();
     };
-    if (lane == 0) {
-        int* e = stack;
+    {
+        int* e = stack;                                  // every lane stores the same values
         e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
     }
     sp = 1;
@@ -685,8 +685,8 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         midj = uni(midj);
         type = uni(type);
         wave_fence();
-        if (sp + 2 > stack_cap) { if (lane == 0) atomicOr(status, 2); break; }
-        if (lane == 0) {
+        if (sp + 2 > stack_cap) { atomicOr(status, 2); break; }
+        {
             int* c0 = stack + 6 * sp;
             int* c1 = stack + 6 * (sp + 1);
             if (type == 1) {
@@ -704,7 +704,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
     count += wave_sum(lcount);
     nodes += wave_sum(lnodes);
     cells += wave_sum(lcells);
-    if (wave_sum(lerr) != 0 && lane == 0) atomicOr(status, 2);
+    if (wave_sum(lerr) != 0) atomicOr(status, 2);
     *nodes_out = nodes;
     *cells_out = cells;
     return count;
@@ -721,17 +721,21 @@ __device__ void load_seq(lu8* dst, const uint8_t* src, int n, int pad, int scale
     }
 }
 
-__device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* wlds)
+// Out of line, and every store and atomic below is made by all 64 lanes (the
+// same value from each): a lane-subset block just before a return lets the
+// structurizer give lanes 1..63 a loop of their own in the caller.  In the
+// persistent loop that compiled to lanes 1..63 taking work item 0 and entering
+// the next pair without lane 0 after an empty sequence (the DP then reads
+// Myers-Miller stack entries that only lane 0 would have written).
+__device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab,
+                                                  lu8* wlds)
 {
-    const int lane = lane_id();
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
     const int n = uni(A.lens[si]), m = uni(A.lens[sj]);
     if (n == 0 || m == 0) {                         // FullPairwiseAlign.cpp:113-118
-        if (lane == 0) {
-            A.out_count[p] = -1;
-            if (A.out_info)
-                for (int k = 0; k < 8; ++k) A.out_info[8 * p + k] = 0;
-        }
+        A.out_count[p] = -1;
+        if (A.out_info)
+            for (int k = 0; k < 8; ++k) A.out_info[8 * p + k] = 0;
         return;
     }
     const int pad = CW_SEQ_PAD;
@@ -802,15 +806,13 @@ __device__ void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab, lu8* w
                          g2E, g3H, g3E, smallF, smallR, CW_SMALL_COLS, colcap, stack, CW_STACK_CAP, tq,
                          A.status, &nodes, &mmcells);
     }
-    if (lane == 0) {
-        A.out_count[p] = count;
-        if (A.out_info) {
-            int* info = A.out_info + 8 * p;
-            info[0] = maxScore; info[1] = se1; info[2] = se2; info[3] = sb1; info[4] = sb2;
-            info[5] = status; info[6] = nodes; info[7] = mmcells;
-        }
-        if (status) atomicOr(A.status, status);
+    A.out_count[p] = count;
+    if (A.out_info) {
+        int* info = A.out_info + 8 * p;
+        info[0] = maxScore; info[1] = se1; info[2] = se2; info[3] = sb1; info[4] = sb2;
+        info[5] = status; info[6] = nodes; info[7] = mmcells;
     }
+    if (status) atomicOr(A.status, status);
 }
 
 __device__ void load_table(const DevArgs& A, CW_LDS int* tab)
@@ -848,9 +850,9 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
     lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
     for (;;) {
-        int item = 0;
-        if (lane_id() == 0) item = (int)atomicAdd(A.queue, 1u);
-        item = uni(__shfl(item, 0, WAVE));
+        // every lane takes part; lane 0 adds 1 and its old value is the item
+        const unsigned got = atomicAdd(A.queue, lane_id() == 0 ? 1u : 0u);
+        const int item = uni((int)got);
         if ((int64_t)item >= A.pair_end) break;
         const int64_t q = A.order ? A.order[item] : item;
         do_pair(A, q, slot, tab, wlds);

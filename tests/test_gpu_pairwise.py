@@ -102,10 +102,11 @@ def test_ties_homopolymers_and_small_alphabets(ctx):
     run_and_compare(ctx, codes, offs, lens, all_pairs(len(seqs)))
 
 
-@pytest.mark.skip(reason="faulted the GPU once (illegal memory access) in a build that used DPP "
-                         "wave reductions; that code was taken out and no cause was found by reading, "
-                         "so this case stays off the GPU until one is shown")
 def test_empty_gaps_unrelated_and_identical(ctx):
+    # empty sequences (distance 1.0, FullPairwiseAlign.cpp:113-118), a gap-only
+    # sequence and '-' codes inside sequences (excluded from len1/len2,
+    # :100-108, :119-127), a one-residue sequence; the empty pairs fall in the
+    # last bin of the batched queue, after pairs that run the full DP
     rng = np.random.default_rng(11)
     base = rng.choice(FULL, 400).astype(np.uint8)
     gapped = base.copy()
