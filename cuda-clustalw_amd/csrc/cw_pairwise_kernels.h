@@ -27,7 +27,7 @@ struct DevArgs {
     int32_t lcap;              // LDS bytes per sequence buffer (with padding)
     int32_t wave_lds;          // LDS bytes per wave
     int32_t rowcap;            // ints per scratch row array (>= maxlen + 1)
-    int32_t pad_;
+    int32_t maxlen;            // longest sequence the LDS buffers, scratch rows and go_by_len cover
     int64_t scratch_stride;    // ints per wave slot (8 * rowcap)
     int32_t* scratch;          // per-wave DP boundary rows
     int32_t* out_count;        // per pair identity count (-1: empty sequence)
@@ -38,7 +38,14 @@ struct DevArgs {
     int32_t* order_out;        // batched queue: the binned work order the persistent kernel pops
     int32_t rmax_cls[CW_NCLASS];   // stripe rows per lane by tuning class
     int32_t colcap_cls[CW_NCLASS]; // Myers-Miller nodes with N <= this use the column path (<= CW_SMALL_COLS)
+    int32_t nseq;              // sequences behind lens/offs: a pair naming another is refused
+    int32_t pad_;
 };
+
+// Status bits of DevArgs::status: 1 reverse pass found no start cell, 2
+// Myers-Miller stack or tiny-node error, 4 a pair that names a sequence outside
+// 0..nseq-1 or longer than maxlen (its count is set to -2 and it is not run).
+#define CW_ST_BADPAIR 4
 
 // Tuning class of a pair: by the longer sequence.
 static inline __host__ __device__ int cw_class_of(int n, int m)

@@ -731,7 +731,15 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
                                                   lu8* wlds)
 {
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
-    const int n = uni(A.lens[si]), m = uni(A.lens[sj]);
+    const bool in_set = (unsigned)si < (unsigned)A.nseq && (unsigned)sj < (unsigned)A.nseq;
+    const int n = in_set ? uni(A.lens[si]) : -1, m = in_set ? uni(A.lens[sj]) : -1;
+    if (n < 0 || m < 0 || n > A.maxlen || m > A.maxlen) {   // outside what the buffers cover
+        A.out_count[p] = -2;
+        if (A.out_info)
+            for (int k = 0; k < 8; ++k) A.out_info[8 * p + k] = 0;
+        atomicOr(A.status, CW_ST_BADPAIR);
+        return;
+    }
     if (n == 0 || m == 0) {                         // FullPairwiseAlign.cpp:113-118
         A.out_count[p] = -1;
         if (A.out_info)
@@ -899,7 +907,10 @@ extern "C" __global__ void __launch_bounds__(256) cw_bin_reset(DevArgs A)
 
 __device__ __forceinline__ int pair_bin(const DevArgs& A, int64_t p)
 {
-    return cw_bin_of(A.lens[A.pairs[2 * p]], A.lens[A.pairs[2 * p + 1]]);
+    const int si = A.pairs[2 * p], sj = A.pairs[2 * p + 1];
+    if ((unsigned)si >= (unsigned)A.nseq || (unsigned)sj >= (unsigned)A.nseq) return 0;  // refused by do_pair
+    const int n = A.lens[si], m = A.lens[sj];
+    return cw_bin_of(n < 0 ? 0 : n > 0xffff ? 0xffff : n, m < 0 ? 0 : m > 0xffff ? 0xffff : m);
 }
 
 extern "C" __global__ void __launch_bounds__(256) cw_bin_count(DevArgs A)

@@ -295,6 +295,8 @@ static void fill_args(cwhip_ctx* ctx, DevArgs& a)
     a.lcap = cw_lcap(ctx->maxlen);
     a.wave_lds = cw_wave_lds_bytes(a.lcap);
     a.rowcap = ((ctx->maxlen + 2) + 63) & ~63;
+    a.maxlen = ctx->maxlen;
+    a.nseq = ctx->nseq;
     a.scratch_stride = 8 * (int64_t)a.rowcap;
     a.out_count = ctx->d_count.p;
     a.out_info = ctx->d_info.p;

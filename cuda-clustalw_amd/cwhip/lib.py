@@ -27,6 +27,7 @@ EXPORTS = [
     "cwhip_fast_pairwise_range", "cwhip_fast_last_ms", "cwhip_nj_tree", "cwhip_server_create",
     "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
     "cwhip_server_text", "cwhip_server_times", "cwhip_server_release", "cwhip_server_stats", "cwhip_guide_from_tree", "cwhip_clustal_text",
+    "cwhip_pairwise_noalloc",
 ]
 
 PRF_W = 35
@@ -59,6 +60,15 @@ class PairParams(C.Structure):
                 ("gap_pos1", C.c_int32), ("gap_pos2", C.c_int32), ("max_len", C.c_int32),
                 ("gap_open_by_len", C.POINTER(C.c_int32))]
 
+
+class PairDevParams(C.Structure):
+    """cwhip_pair_dev_params: the scoring in device memory."""
+    _fields_ = [("matrix", C.c_void_p), ("gap_open_by_len", C.c_void_p), ("gap_extend", C.c_int32),
+                ("gap_pos1", C.c_int32), ("gap_pos2", C.c_int32), ("max_len", C.c_int32)]
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20
+NOALLOC_MAX_PAIRS = 1 << 22
 
 TUNE_CLASSES = 6
 TUNE_CLASS_EDGES = (128, 256, 512, 1024, 2048)   # tuning class = first edge >= the longer sequence
@@ -158,6 +168,8 @@ def load() -> C.CDLL:
     L.cwhip_server_times.argtypes = [vp, C.c_int64, P(np.float64, flags="C")]
     L.cwhip_server_release.argtypes = [vp, C.c_int64]
     L.cwhip_server_stats.argtypes = [vp, P(np.float64, flags="C")]
+    L.cwhip_pairwise_noalloc.argtypes = [C.POINTER(PairDevParams), vp, vp, vp, C.c_int, vp,
+                                         C.c_int64, vp, vp, vp]
     for name in EXPORTS:
         if name not in ("cwhip_destroy", "cwhip_last_error", "cwhip_range_npairs", "cwhip_distance",
                         "cwhip_server_destroy", "cwhip_fast_last_ms"):

@@ -198,6 +198,49 @@ int cwhip_autotune(cwhip_ctx *ctx, const cwhip_pair_params *p, const int32_t *pa
 int cwhip_set_graph(cwhip_ctx *ctx, int enable);
 
 
+/* ---- the pairwise stage without device allocation ----------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same batched kernels as cwhip_pairwise_pairs with
+ * CWHIP_KERNEL_BATCHED (queue reset, length binning, the persistent kernel)
+ * run on the caller's stream over caller memory; their work space (queue
+ * counter, bins, the binned order and each wave's DP rows) is one file-scope
+ * device array of CWHIP_NOALLOC_SCRATCH_BYTES that every GPU the library runs
+ * on holds until the process ends.  The call allocates nothing, does not
+ * synchronise and returns once the work is enqueued.
+ *
+ * Every pointer is device memory of the current device (hipSetDevice): the
+ * sequences (codes[offsets[s] .. + lens[s]], s < nseq), npairs pairs of 0-based
+ * indices, the scoring (matrix NUMRES x NUMRES, gap_open_by_len[0..max_len]),
+ * out_count (npairs identities, -1 when either sequence is empty, as
+ * cwhip_pairwise_pairs) and out_status (one int: 0 when every pair ran; bit 4
+ * when a pair names a sequence outside 0..nseq-1 or longer than max_len; such a
+ * pair gets count -2 and is not aligned).  The distance of a pair is
+ * cwhip_distance(count, gap-free len1, gap-free len2).
+ *
+ * Limits: npairs <= CWHIP_NOALLOC_MAX_PAIRS, max_len <= CWHIP_MAX_LEN, and at
+ * least one workgroup's DP rows must fit beside the pair order (4 waves x 8 rows
+ * of max_len + 2 ints; the number of resident waves is cut to what fits).
+ * Returns CWHIP_E_ARG (null pointer, negative size), CWHIP_E_TOOLONG
+ * (max_len), CWHIP_E_NOMEM (npairs, or rows that do not fit), CWHIP_E_HIP.
+ * All calls on one GPU share its array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+#define CWHIP_NOALLOC_SCRATCH_BYTES (128ll << 20)
+#define CWHIP_NOALLOC_MAX_PAIRS (1ll << 22)
+typedef struct {
+    const int32_t *matrix;            /* device, NUMRES x NUMRES */
+    const int32_t *gap_open_by_len;   /* device, max_len + 1 entries */
+    int32_t gap_extend;
+    int32_t gap_pos1;
+    int32_t gap_pos2;
+    int32_t max_len;
+} cwhip_pair_dev_params;
+
+int cwhip_pairwise_noalloc(const cwhip_pair_dev_params *p, const uint8_t *codes,
+                           const int64_t *offsets, const int32_t *lens, int nseq,
+                           const int32_t *pairs, int64_t npairs, int32_t *out_count,
+                           int32_t *out_status, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Progressive (profile) alignment                                          */
 /* ------------------------------------------------------------------------ */
