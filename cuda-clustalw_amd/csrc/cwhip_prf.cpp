@@ -405,11 +405,27 @@ int get_matrix(const short* tab, const char* order, bool neg, int scale, int mat
     return max_res + 2;
 }
 
-// SubMatrix::getProfileAlignMatrix (SubMatrix.cpp:403-677), default matrix
-// names ("gonnet" series, "iub").
+// a caller's profile_matrix callback is not assumed to be reentrant (the
+// reference's SubMatrix keeps state in members)
+std::mutex g_matrix_cb_mu;
+
+// SubMatrix::getProfileAlignMatrix (SubMatrix.cpp:403-677): the caller's
+// callback when there is one, else the default matrix names ("gonnet" series,
+// "iub").  Returns maxRes (<= 0: the step fails).
 int profile_matrix(const cwhip_msa_params& p, double pcid, int minLen, int mat[NUMRES][NUMRES],
                    float* scale_out, int* mat_avg)
 {
+    if (p.profile_matrix) {
+        std::lock_guard<std::mutex> lk(g_matrix_cb_mu);
+        float sc = 1.0f;                 // MyersMillerProfileAlign.cpp:249-251
+        int32_t avg = 0;
+        for (int i = 0; i < NUMRES; ++i)
+            for (int j = 0; j < NUMRES; ++j) mat[i][j] = 0;
+        const int r = p.profile_matrix(p.profile_matrix_user, pcid, minLen, &mat[0][0], &sc, &avg);
+        *scale_out = sc;
+        *mat_avg = avg;
+        return r;
+    }
     float scale;
     float int_scale = 100;
     int r;
@@ -426,7 +442,7 @@ int profile_matrix(const cwhip_msa_params& p, double pcid, int minLen, int mat[N
         const short* tab;
         scale = 0.75f;
         scale /= 2.0;
-        if (p.neg_matrix) {
+        if (p.neg_matrix || !p.distance_tree) {
             tab = cw_gonnet250;
         } else if (pcid > 35.0) {
             tab = cw_gonnet80;
@@ -509,10 +525,11 @@ void side_spec(const cwhip_msa_params& p, const Seqs& seqs, const Step& st, Side
     for (int k = s.first; k < s.last; ++k) s.sum2 += st.w[k];
 }
 
-// Returns false when a group is empty (profileAlign returns 0, :96-99).
+// Returns 1 when the step runs, 0 when a group is empty (profileAlign returns
+// 0, :96-99), -1 when its profile matrix fails (profileAlign returns -1).
 // rescnt[s]: residues (codes other than GAP1/GAP2) of sequence s, which gaps
 // inserted by earlier steps do not change.
-bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector<int>& rescnt,
+int prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector<int>& rescnt,
                   const double* sim, const std::vector<int>& weights, std::vector<int>& group,
                   const std::vector<int>& aligned, Step& st)
 {
@@ -524,7 +541,7 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector
         if (group[i + 1] == 1) ++n1;
         else if (group[i + 1] == 2) ++n2;
     }
-    if (n1 == 0 || n2 == 0) return false;
+    if (n1 == 0 || n2 == 0) return 0;
     if (n2 > n1) {                                       // :106-120
         for (int i = 0; i < nseq; ++i) {
             if (group[i + 1] == 1) group[i + 1] = 2;
@@ -565,7 +582,7 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector
     int mat[NUMRES][NUMRES];
     float scale = 1.0f;
     int matAvg = 0;
-    profile_matrix(p, pcid, minLen, mat, &scale, &matAvg);
+    if (profile_matrix(p, pcid, minLen, mat, &scale, &matAvg) <= 0) return -1;   // :258-263
     st.mat.assign(&mat[0][0], &mat[0][0] + NUMRES * NUMRES);
     int gapcoef, lencoef;                                 // :254-296
     if (p.dna) {
@@ -599,7 +616,7 @@ bool prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector
     st.job.L1 = L1;
     st.job.L2 = L2;
     st.job.displ = st.displ.data();
-    return true;
+    return 1;
 }
 
 // The residue-specific gap penalty of each code (calcResidueSpecificPen,
@@ -843,9 +860,13 @@ int cwhip_profile_align(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     std::vector<int> w(weights, weights + nseq);
     Step st;
     for (int k = 0; k < 4; ++k) info[k] = 0;
-    const bool run = prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st);
+    const int run = prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st);
     for (int i = 0; i <= nseq; ++i) group[i] = g[i];          // profileAlign mutates group
-    if (!run) return CWHIP_OK;                                 // an empty group: score 0 (:96-99)
+    if (run < 0) {                                             // the matrix failed (:258-263)
+        info[0] = -1;
+        return CWHIP_OK;
+    }
+    if (run == 0) return CWHIP_OK;                             // an empty group: score 0 (:96-99)
     if (rows_out && stride < st.L1 + st.L2 + 2) return fail(ctx, CWHIP_E_ARG, "row stride too small");
     BatchLayout lo;
     const int rc = run_built_batch(ctx, *p, seqs, {&st}, true, lo);
@@ -878,7 +899,9 @@ int cwhip_step_profiles(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     std::vector<int> w(weights, weights + nseq);
     Step st;
     len12[0] = len12[1] = -1;
-    if (!prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st)) return CWHIP_OK;
+    const int run = prepare_step(*p, seqs, residue_counts(seqs), sim, w, g, al, st);
+    if (run < 0) return fail(ctx, CWHIP_E_MATRIX, "the profile matrix callback failed");
+    if (run == 0) return CWHIP_OK;
     len12[0] = st.L1;
     len12[1] = st.L2;
     const int64_t n1 = (int64_t)(st.L1 + 2) * W, n2 = (int64_t)(st.L2 + 2) * W;
@@ -912,6 +935,9 @@ int cwhip_msa_params_default(int dna, cwhip_msa_params* p)
     p->no_weights = 0;
     p->divergence_cutoff = 30;
     p->hyd_residues = "GPSNDQEKR";
+    p->distance_tree = 1;
+    p->profile_matrix = nullptr;
+    p->profile_matrix_user = nullptr;
     return CWHIP_OK;
 }
 
@@ -984,17 +1010,19 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
     for (int lv = 0; lv < nlev; ++lv) {
         const size_t nst = by_level[lv].size();
         std::vector<Step> sts(nst);
-        std::vector<char> ok(nst, 0);
+        std::vector<signed char> ok(nst, 0);
         // the steps of a level touch disjoint sequences: build their profiles
         // (and later apply their edit scripts) on several host threads; a level
         // of few steps builds them in turn, each on all threads
         auto prep = [&](size_t k) {
             std::vector<int> grp = groups[by_level[lv][k]];
-            ok[k] = prepare_step(P, seqs, rescnt, sim, weights, grp, all_aligned, sts[k]) ? 1 : 0;
+            ok[k] = (signed char)prepare_step(P, seqs, rescnt, sim, weights, grp, all_aligned, sts[k]);
         };
         if (nst >= 4) parallel_for(nst, prep);
         else
             for (size_t k = 0; k < nst; ++k) prep(k);
+        for (size_t k = 0; k < nst; ++k)              // MSA.cpp:147-152: a failed step ends the run
+            if (ok[k] < 0) return fail(ctx, CWHIP_E_MATRIX, "the profile matrix callback failed");
         std::vector<Step*> live;
         for (size_t k = 0; k < nst; ++k)
             if (ok[k]) live.push_back(&sts[k]);
@@ -1055,7 +1083,9 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         }
         aligned[iseq] = 1;
         Step st;
-        if (prepare_step(P, seqs, rescnt, sim, weights, group, aligned, st)) {
+        const int run = prepare_step(P, seqs, rescnt, sim, weights, group, aligned, st);
+        if (run < 0) return fail(ctx, CWHIP_E_MATRIX, "the profile matrix callback failed");
+        if (run > 0) {
             BatchLayout lo;
             const int rc = run_built_batch(ctx, P, seqs, {&st}, true, lo);
             if (rc != CWHIP_OK) return rc;

@@ -133,11 +133,13 @@ int run_job(cwhip_ctx* ctx, Job& j)
     j.dnd.resize(len);
     std::vector<int32_t> w(n), steps((size_t)std::max(n - 1, 1) * (n + 1));
     std::vector<double> sim((size_t)(n + 1) * (n + 1));
-    int32_t nsteps = 0;
-    rc = cwhip_guide_from_tree(n, j.dnd.c_str(), nm.data(), w.data(), sim.data(), steps.data(), &nsteps);
+    int32_t nsteps = 0, dtree = 1;
+    rc = cwhip_guide_from_tree(n, j.dnd.c_str(), nm.data(), w.data(), sim.data(), steps.data(), &nsteps,
+                               &dtree);
     if (rc) return rc;
     cwhip_msa_params mp;
     cwhip_msa_params_default(j.dna, &mp);
+    mp.distance_tree = dtree;
     std::vector<int32_t> order(n);
     int32_t alen = 0;
     rc = cwhip_msa(ctx, &mp, n, j.codes.data(), j.offs.data(), j.lens.data(), sim.data(), w.data(),

@@ -637,7 +637,7 @@ int cwhip_nj_tree(int nseq, const double* dist, const char* const* names, char* 
 }
 
 int cwhip_guide_from_tree(int nseq, const char* dnd, const char* const* names, int32_t* weights,
-                          double* sim, int32_t* steps, int32_t* nsteps)
+                          double* sim, int32_t* steps, int32_t* nsteps, int32_t* distance_tree)
 {
     if (nseq < 2 || !dnd || !names || !weights || !sim || !steps || !nsteps) return CWHIP_E_ARG;
     TreeReader T;
@@ -651,6 +651,7 @@ int cwhip_guide_from_tree(int nseq, const char* dnd, const char* const* names, i
     T.create_tree(seqTree, -1);
     if ((int)T.lptr.size() != nseq) return CWHIP_E_ARG;
     if (!T.distance_tree && !T.rooted) return CWHIP_E_ARG;
+    if (distance_tree) *distance_tree = T.distance_tree ? 1 : 0;
     const int root = T.rooted ? seqTree : T.re_root(seqTree, nseq + 1);
     for (int leafp : T.lptr)                                 // orderNodes
         for (int p = leafp; p >= 0; p = T.nodes[p].parent) T.nodes[p].order++;

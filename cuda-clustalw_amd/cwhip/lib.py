@@ -46,7 +46,32 @@ class MsaParams(C.Structure):
                 ("use_end_gaps", C.c_int32), ("end_gap_penalties", C.c_int32),
                 ("transition_weight", C.c_float), ("neg_matrix", C.c_int32),
                 ("no_weights", C.c_int32), ("divergence_cutoff", C.c_int32),
-                ("hyd_residues", C.c_char_p)]
+                ("hyd_residues", C.c_char_p), ("distance_tree", C.c_int32),
+                ("profile_matrix", C.c_void_p), ("profile_matrix_user", C.c_void_p)]
+
+
+# cwhip_profile_matrix_fn: (user, pcid, min_len, int32 matrix[32*32], float* scale, int32* mat_avg) -> maxRes
+PROFILE_MATRIX_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                                C.POINTER(C.c_float), C.POINTER(C.c_int32))
+
+
+def set_profile_matrix(params, fn):
+    """Install a Python profile-matrix function fn(pcid, min_len) -> (maxRes,
+    int32[32,32] matrix, scale, mat_avg) as params' cwhip_profile_matrix_fn;
+    the ctypes thunk is kept on params."""
+    def thunk(user, pcid, min_len, mat, scale, avg):
+        r, m, sc, a = fn(pcid, min_len)
+        m = np.ascontiguousarray(m, np.int32).reshape(-1)
+        for k in range(32 * 32):
+            mat[k] = int(m[k])
+        scale[0] = sc
+        avg[0] = a
+        return int(r)
+    cb = PROFILE_MATRIX_FN(thunk)
+    params._matrix_cb = cb
+    params.profile_matrix = C.cast(cb, C.c_void_p)
+    params.profile_matrix_user = None
+    return params
 
 
 class FastParams(C.Structure):
@@ -152,7 +177,7 @@ def load() -> C.CDLL:
                                 C.POINTER(C.c_int64)]
     L.cwhip_guide_from_tree.argtypes = [C.c_int, C.c_char_p, vp, P(np.int32, flags="C"),
                                         P(np.float64, flags="C"), P(np.int32, flags="C"),
-                                        C.POINTER(C.c_int32)]
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cwhip_clustal_text.argtypes = [C.c_int, vp, P(np.uint8, flags="C"), C.c_int64,
                                      P(np.int32, flags="C"), P(np.int32, flags="C"), C.c_int,
                                      vp, C.c_int64, C.POINTER(C.c_int64)]
@@ -400,8 +425,9 @@ def nj_tree(dist_full, names) -> str:
     return buf.value.decode()
 
 
-def guide_from_tree(dnd: str, names):
-    """(weights int32[n], sim float64[n+1, n+1], steps int32[nsteps, n+1])."""
+def guide_from_tree(dnd: str, names, with_distance_tree: bool = False):
+    """(weights int32[n], sim float64[n+1, n+1], steps int32[nsteps, n+1]), and
+    UserParameters::distanceTree as readTree leaves it when with_distance_tree."""
     L = load()
     n = len(names)
     w = np.zeros(max(n, 1), np.int32)
@@ -409,11 +435,13 @@ def guide_from_tree(dnd: str, names):
     steps = np.zeros(max(n - 1, 1) * (n + 1), np.int32)
     ns = C.c_int32()
     arr = _names(names)
+    dt = C.c_int32()
     rc = L.cwhip_guide_from_tree(n, dnd.encode(), C.cast(arr, C.c_void_p), w, sim, steps,
-                                 C.byref(ns))
+                                 C.byref(ns), C.byref(dt))
     if rc:
         raise CwhipError("cwhip_guide_from_tree failed: %d" % rc)
-    return w[:n].copy(), sim.reshape(n + 1, n + 1), steps[:ns.value * (n + 1)].reshape(-1, n + 1)
+    out = (w[:n].copy(), sim.reshape(n + 1, n + 1), steps[:ns.value * (n + 1)].reshape(-1, n + 1))
+    return out + (dt.value,) if with_distance_tree else out
 
 
 def clustal_text(names, rows, order, dna=False) -> str:

@@ -21,8 +21,13 @@ def distance_matrix(n: int, pairs: np.ndarray, dist: np.ndarray) -> np.ndarray:
     return D
 
 
-def align(ctx: lib.Context, names, seqs, dna: bool, timings: dict | None = None):
-    """(dnd_text, aln_text, aligned rows, 1-based output order) of one run."""
+def align(ctx: lib.Context, names, seqs, dna: bool, timings: dict | None = None,
+          guide_dnd: str | None = None):
+    """(dnd_text, aln_text, aligned rows, 1-based output order) of one run.
+    guide_dnd: a guide tree to use instead of the NJ tree of the distances
+    (Clustal::doAlignUseOldTree, Clustal.cpp:780-900); a tree without branch
+    lengths leaves UserParameters::distanceTree false (tree/Tree.cpp:564), which
+    fixes the profile matrix of every step (SubMatrix.cpp:565)."""
     n = len(seqs)
     t0 = time.perf_counter()
     lens = np.array([len(s) for s in seqs], np.int32)
@@ -31,13 +36,18 @@ def align(ctx: lib.Context, names, seqs, dna: bool, timings: dict | None = None)
         offs[1:] = np.cumsum(lens[:-1])
     codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if n else np.zeros(0, np.uint8)
     ctx.load_sequences(codes, offs, lens)
-    params = lib.pair_params(dna=dna, max_len=max(int(lens.max()) if n else 1, 1))
-    pairs, cnt, dist = ctx.pairwise_range(params, 0, n, 0, n)
-    t1 = time.perf_counter()
-    dnd = lib.nj_tree(distance_matrix(n, pairs, dist), names)
-    w, sim, steps = lib.guide_from_tree(dnd, names)
+    if guide_dnd is None:
+        params = lib.pair_params(dna=dna, max_len=max(int(lens.max()) if n else 1, 1))
+        pairs, cnt, dist = ctx.pairwise_range(params, 0, n, 0, n)
+        t1 = time.perf_counter()
+        dnd = lib.nj_tree(distance_matrix(n, pairs, dist), names)
+    else:
+        t1 = time.perf_counter()
+        dnd = guide_dnd
+    w, sim, steps, dtree = lib.guide_from_tree(dnd, names, with_distance_tree=True)
     t2 = time.perf_counter()
-    rows, order, stats = ctx.msa(seqs, sim, w, steps, dna=dna)
+    rows, order, stats = ctx.msa(seqs, sim, w, steps, params=lib.msa_params(dna, distance_tree=dtree),
+                                 dna=dna)
     t3 = time.perf_counter()
     aln = lib.clustal_text(names, rows, order, dna=dna)
     t4 = time.perf_counter()

@@ -7,7 +7,9 @@
  * (Alignment::getSeqWeight), the similarity matrix the caller reads pcid from
  * (:124-137), the group and aligned vectors and the multiple-alignment
  * UserParameters to cwhip_profile_align, and writes the new rows back with
- * the maximum lengths updated as :206 and :446-493 do.
+ * the maximum lengths updated as :206 and :446-493 do.  The profile matrix of
+ * each step comes from the reference's own SubMatrix, so every matrix series
+ * and the distance-tree rule behave as in the CPU path.
  */
 #include "HipMyersMillerProfileAlign.h"
 
@@ -19,6 +21,26 @@
 
 namespace clustalw
 {
+
+// SubMatrix::getProfileAlignMatrix of the reference (SubMatrix.cpp:403-677),
+// called by the library for every step: whatever matrix series the user chose
+// (gonnet, blosum, pam, id, a user matrix or series; iub, clustalw or a user
+// DNA matrix) and the distance-tree rule come out as the CPU path has them.
+static int refProfileAlignMatrix(void*, double pcid, int32_t minLen, int32_t* out, float* scale,
+                                 int32_t* matAvg)
+{
+    int matrix[NUMRES][NUMRES];
+    PrfScaleValues scaleVals;
+    scaleVals.scale = 1.0;                 // MyersMillerProfileAlign.cpp:249-251
+    scaleVals.intScale = 100.0;
+    int avg = 0;
+    const int maxRes = subMatrix->getProfileAlignMatrix(matrix, pcid, minLen, scaleVals, avg);
+    for (int i = 0; i < NUMRES; ++i)
+        for (int j = 0; j < NUMRES; ++j) out[i * NUMRES + j] = matrix[i][j];
+    *scale = scaleVals.scale;
+    *matAvg = avg;
+    return maxRes;
+}
 
 HipMyersMillerProfileAlign::HipMyersMillerProfileAlign() : ctx(0), simOwner(0), simSize(0)
 {
@@ -40,16 +62,6 @@ int HipMyersMillerProfileAlign::profileAlign(Alignment* alnPtr, DistMatrix* dist
 {
     const int numSeq = alnPtr->getNumSeqs();
     if (numSeq <= 0) return 0;
-    // the library carries the default series only: gonnet (matrixNum 3) and iub
-    // (DNAMatrixNum 1), SubMatrix.cpp:109-111
-    if ((!userParameters->getDNAFlag() && subMatrix->getMatrixNum() != 3) ||
-        (userParameters->getDNAFlag() && subMatrix->getDNAMatrixNum() != 1))
-    {
-        std::cerr << "The GPU profile alignment supports the default matrices only "
-                  << "(gonnet series, iub).\nNeed to terminate program.\n";
-        exit(1);
-    }
-
     cwhip_msa_params p;
     cwhip_msa_params_default(userParameters->getDNAFlag() ? 1 : 0, &p);
     p.gap_open = userParameters->getGapOpen();
@@ -66,6 +78,9 @@ int HipMyersMillerProfileAlign::profileAlign(Alignment* alnPtr, DistMatrix* dist
     p.divergence_cutoff = userParameters->getDivergenceCutoff();
     std::string hyd = userParameters->getHydResidues();
     p.hyd_residues = hyd.c_str();
+    p.distance_tree = userParameters->getDistanceTree() ? 1 : 0;
+    p.profile_matrix = &refProfileAlignMatrix;
+    p.profile_matrix_user = 0;
 
     // the rows and weights of every sequence (codes are < 256 by construction)
     std::vector<int64_t> offs(numSeq);

@@ -42,7 +42,9 @@ enum {
     CWHIP_E_TOOLONG = -4,   /* a sequence exceeds CWHIP_MAX_LEN */
     CWHIP_E_KERNEL = -5,    /* a kernel reported an internal inconsistency */
     CWHIP_E_NOMEM = -6,     /* device allocation failed, or a job larger than the HBM budget */
-    CWHIP_E_TIMEOUT = -7    /* cwhip_server_wait: the job did not finish in time */
+    CWHIP_E_TIMEOUT = -7,   /* cwhip_server_wait: the job did not finish in time */
+    CWHIP_E_MATRIX = -8     /* the profile matrix of a step could not be set up (the caller's
+                               cwhip_profile_matrix_fn returned <= 0) */
 };
 
 #define CWHIP_MAX_LEN 16000  /* longest sequence the kernels accept */
@@ -268,6 +270,16 @@ typedef struct {
  * len1).  end_gap_penalties is UserParameters::endGapPenalties. */
 int cwhip_profile_dp(cwhip_ctx *ctx, cwhip_prf_pair *pairs, int npairs, int end_gap_penalties);
 
+/* Optional replacement of the built-in matrix series (gonnet for protein, iub
+ * for DNA) of a profileAlign step: fills matrix (NUMRES x NUMRES, row-major),
+ * *scale and *mat_avg the way SubMatrix::getProfileAlignMatrix(matrix, pcid,
+ * minLen, scaleValues, matAvg) does (substitutionMatrix/SubMatrix.cpp:403-677)
+ * and returns its maxRes.  A return of 0 or less fails the step, as the
+ * `return -1` of MyersMillerProfileAlign.cpp:258-263 does.  The library calls
+ * it from one thread at a time (host worker threads, serialised). */
+typedef int (*cwhip_profile_matrix_fn)(void *user, double pcid, int32_t min_len, int32_t *matrix,
+                                       float *scale, int32_t *mat_avg);
+
 /* The UserParameters the progressive stage reads (general/UserParameters.cpp:
  * 33-175, setDNAParams/setProtParams :501-523). */
 typedef struct {
@@ -281,9 +293,17 @@ typedef struct {
     int32_t no_weights;
     int32_t divergence_cutoff;
     const char *hyd_residues;         /* hydResidues, e.g. "GPSNDQEKR" */
+    int32_t distance_tree;            /* UserParameters::getDistanceTree: 0 after a guide tree
+                                         without branch lengths (tree/Tree.cpp:564) or the
+                                         profile-to-profile path (multipleAlign/MSA.cpp:572);
+                                         then every step takes the series' fixed matrix
+                                         (gon250, SubMatrix.cpp:565) */
+    cwhip_profile_matrix_fn profile_matrix;   /* NULL: the built-in series */
+    void *profile_matrix_user;                /* passed to profile_matrix */
 } cwhip_msa_params;
 
-/* The reference defaults for protein (dna = 0) or DNA (dna = 1). */
+/* The reference defaults for protein (dna = 0) or DNA (dna = 1); distance_tree 1,
+ * no profile_matrix callback. */
 int cwhip_msa_params_default(int dna, cwhip_msa_params *p);
 
 /* MSA::multiSeqAlign with iStart = 0 (multipleAlign/MSA.cpp:32-347) driving
@@ -296,7 +316,9 @@ int cwhip_msa_params_default(int dna, cwhip_msa_params *p);
  * each (element 0 unused; 1 / 2 = group, 0 = not in the step) as
  * Tree::createSets makes them.  out_order[k] (1-based) is the k-th sequence of
  * the output order (outputOrder ALIGNED); *aln_len is the alignment length.
- * The aligned rows stay in the context: fetch them with cwhip_msa_rows. */
+ * The aligned rows stay in the context: fetch them with cwhip_msa_rows.
+ * Returns CWHIP_E_MATRIX when a step's profile matrix fails (MSA.cpp:147-152
+ * returns -1 then). */
 int cwhip_msa(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t *codes,
               const int64_t *offsets, const int32_t *lens, const double *sim,
               const int32_t *seq_weight, const int32_t *steps, int nsteps, int32_t *out_order,
@@ -310,7 +332,8 @@ int cwhip_msa(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t
  * is written to rows_out + s*stride (stride >= info[2] + info[3] + 2).  As in
  * the reference, group is modified (members with aligned[s] == 0 dropped, the
  * groups swapped when group 2 is larger).  info: [0] profileAlign's return
- * value (score / 100; 0 when a group is empty and nothing was aligned),
+ * value (score / 100; 0 when a group is empty and nothing was aligned; -1,
+ * with nothing aligned, when the profile matrix fails, :258-263),
  * [1] the new alignment length, [2] prfLength1, [3] prfLength2. */
 int cwhip_profile_align(cwhip_ctx *ctx, const cwhip_msa_params *p, int nseq, const uint8_t *codes,
                         const int64_t *offsets, const int32_t *lens, const double *sim,
@@ -359,9 +382,12 @@ int cwhip_nj_tree(int nseq, const double *dist, const char *const *names, char *
  * (tree/TreeInterface.cpp:209-276, tree/Tree.cpp): from .dnd text to the
  * sequence weights (nseq), the similarity matrix ((nseq+1)^2, row-major,
  * 1-based) and the alignment steps (room for (nseq-1) x (nseq+1) ints; element
- * 0 of each step unused).  *nsteps gets the number of steps. */
+ * 0 of each step unused).  *nsteps gets the number of steps; *distance_tree
+ * (may be NULL) gets UserParameters::distanceTree as readTree leaves it: 1, or
+ * 0 when a leaf has no branch length (tree/Tree.cpp:145, :561-565), for
+ * cwhip_msa_params.distance_tree. */
 int cwhip_guide_from_tree(int nseq, const char *dnd, const char *const *names, int32_t *weights,
-                          double *sim, int32_t *steps, int32_t *nsteps);
+                          double *sim, int32_t *steps, int32_t *nsteps, int32_t *distance_tree);
 
 /* AlignmentOutput::clustalOut (alignment/AlignmentOutput.cpp:1573-2116): the
  * CLUSTAL text of an alignment.  rows: sequence s (0-based) at rows + s*stride,

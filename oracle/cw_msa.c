@@ -126,6 +126,21 @@ int cwo_profile_matrix(int dna, double pcid, int min_len, float transition_weigh
     return r;
 }
 
+/* the matrix of one profileAlign step (MyersMillerProfileAlign.cpp:249-258):
+ * the caller's series when it gives one, else the default names */
+static int step_matrix(const cwo_msa_params *pm, double pcid, int min_len, int mat[NUMRES][NUMRES],
+                       float *scale, int *mat_avg)
+{
+    if (pm->profile_matrix) {
+        *scale = 1.0f;
+        *mat_avg = 0;
+        memset(mat, 0, sizeof(int) * NUMRES * NUMRES);
+        return pm->profile_matrix(pm->profile_matrix_user, pcid, min_len, &mat[0][0], scale, mat_avg);
+    }
+    return cwo_profile_matrix(pm->dna, pcid, min_len, pm->transition_weight, pm->neg_matrix,
+                              pm->distance_tree, mat, scale, mat_avg);
+}
+
 /* ---------------------------------------------------------------------- */
 /* Profile builders.  `rows` are the profile's sequences (0-based columns). */
 typedef struct {
@@ -877,8 +892,7 @@ static int profile_align(const cwo_msa_params *pm, int nseq, int **seqs, int *le
     }
     len2 = (int)(len2 / (float)nseqs2);
     minLen = len1 < len2 ? len1 : len2;
-    int maxRes = cwo_profile_matrix(pm->dna, pcid, minLen, pm->transition_weight,
-                                    pm->neg_matrix, 1, mat, &scale, &matAvg);
+    int maxRes = step_matrix(pm, pcid, minLen, mat, &scale, &matAvg);
     if (maxRes == 0 || maxRes == -1) return -1;
     if (pm->dna) {
         gapcoef1 = gapcoef2 = (int)(100.0 * pm->gap_open * scale);
@@ -1073,8 +1087,13 @@ int cwo_step_profiles(const cwo_msa_params *pm, int nseq, int *const *seqs, cons
     l2 = (int)(l2 / (float)n2);
     int minLen = l1 < l2 ? l1 : l2, matAvg = 0, mat[NUMRES][NUMRES];
     float scale;
-    cwo_profile_matrix(pm->dna, pcid, minLen, pm->transition_weight, pm->neg_matrix, 1, mat,
-                       &scale, &matAvg);
+    if (step_matrix(pm, pcid, minLen, mat, &scale, &matAvg) <= 0) {
+        for (i = 0; i < nseq; i++) free(rows[i]);
+        free(rows);
+        free(w);
+        free(group);
+        return -1;
+    }
     int gapcoef, lencoef;
     if (pm->dna) {
         gapcoef = (int)(100.0 * pm->gap_open * scale);
@@ -1258,4 +1277,7 @@ void cwo_msa_params_default(int dna, cwo_msa_params *p)
     p->no_weights = 0;
     p->divergence_cutoff = 30;
     p->hyd_residues = "GPSNDQEKR";
+    p->distance_tree = 1;
+    p->profile_matrix = NULL;
+    p->profile_matrix_user = NULL;
 }

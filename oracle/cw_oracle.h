@@ -97,6 +97,12 @@ typedef struct {
     int no_weights;
     int divergence_cutoff;
     const char *hyd_residues;
+    int distance_tree;               /* UserParameters::getDistanceTree (SubMatrix.cpp:490,534,565) */
+    /* optional stand-in for SubMatrix::getProfileAlignMatrix with another
+     * matrix series: fills mat, *scale, *mat_avg and returns maxRes */
+    int (*profile_matrix)(void *user, double pcid, int min_len, int *mat, float *scale,
+                          int *mat_avg);
+    void *profile_matrix_user;
 } cwo_msa_params;
 
 typedef struct {
@@ -136,7 +142,8 @@ int cwo_prog_diff_trace(const int *prf1, int len1, const int *prf2, int len2,
                         int *trace, int trace_cap, int *ntrace);
 
 /* The two profiles profileAlign would build for one step (malloc'd; free
- * with cwo_free).  Returns 0 when a group is empty, 1 otherwise. */
+ * with cwo_free).  Returns 0 when a group is empty, -1 when the profile
+ * matrix fails, 1 otherwise. */
 int cwo_step_profiles(const cwo_msa_params *pm, int nseq, int *const *seqs, const int *lens,
                       const double *sim, const int *weights, const int *group,
                       const int *aligned, int **prf1, int *len1, int **prf2, int *len2);

@@ -32,7 +32,30 @@ class MsaParams(C.Structure):
                 ("use_end_gaps", C.c_int), ("end_gap_penalties", C.c_int),
                 ("transition_weight", C.c_float), ("neg_matrix", C.c_int),
                 ("no_weights", C.c_int), ("divergence_cutoff", C.c_int),
-                ("hyd_residues", C.c_char_p)]
+                ("hyd_residues", C.c_char_p), ("distance_tree", C.c_int),
+                ("profile_matrix", C.c_void_p), ("profile_matrix_user", C.c_void_p)]
+
+
+PROFILE_MATRIX_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(C.c_float), C.POINTER(C.c_int))
+
+
+def set_profile_matrix(params, fn):
+    """Use fn(pcid, min_len) -> (maxRes, int32[32,32], scale, mat_avg) as the
+    params' matrix series (a stand-in for SubMatrix::getProfileAlignMatrix)."""
+    def thunk(user, pcid, min_len, mat, scale, avg):
+        r, m, sc, a = fn(pcid, min_len)
+        m = np.ascontiguousarray(m, np.int32).reshape(-1)
+        for k in range(32 * 32):
+            mat[k] = int(m[k])
+        scale[0] = sc
+        avg[0] = a
+        return int(r)
+    cb = PROFILE_MATRIX_FN(thunk)
+    params._matrix_cb = cb
+    params.profile_matrix = C.cast(cb, C.c_void_p)
+    params.profile_matrix_user = None
+    return params
 
 
 class DpStats(C.Structure):
@@ -312,6 +335,8 @@ def step_profiles(seqs, sim, weights, group, aligned, params: MsaParams):
                                     C.byref(p1), C.byref(l1), C.byref(p2), C.byref(l2))
         if r == 0:
             return None
+        if r < 0:
+            raise RuntimeError("the profile matrix failed")
         a = np.ctypeslib.as_array(p1, shape=((l1.value + 2) * PRF_STRIDE,)).copy()
         b = np.ctypeslib.as_array(p2, shape=((l2.value + 2) * PRF_STRIDE,)).copy()
         lib().cwo_free(C.cast(p1, C.c_void_p))

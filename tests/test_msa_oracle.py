@@ -207,3 +207,52 @@ def test_prog_diff_score_equals_rescored_group_profiles(setname):
     for p1, l1, p2, l2 in random_profile_pairs(setname, 8, 3):
         score, displ, st = O.prog_diff(p1, l1, p2, l2)
         assert _rescore(p1, l1, p2, l2, displ) == score
+
+
+def divergent_set(seed):
+    """Two seeded families, one close and one divergent, with their NJ guide
+    tree: (seqs int32, sim, weights, steps).  Its steps span every matrix of
+    the gonnet series, unlike the fixtures (all above 90% identity)."""
+    from cwhip import seqgen
+    a = seqgen.family(seed * 13 + 5, 12, 220, branch_mean=0.8, name_prefix="da")
+    b = seqgen.family(seed * 13 + 6, 8, 180, branch_mean=2.0, name_prefix="db")
+    seqs, names = a.seqs + b.seqs, a.names + b.names
+    n = len(seqs)
+    codes = np.concatenate(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    pairs = O.range_pairs(n, 0, n, 0, n)
+    _, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring())
+    dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    return [s.astype(np.int32) for s in seqs], sim, w, steps
+
+
+def test_oracle_distance_tree_false_fixes_the_matrix():
+    """distanceTree false: every step takes gon250 (SubMatrix.cpp:565);
+    with it true the series follows pcid/minLen and the rows differ."""
+    seqs, sim, w, steps = divergent_set(0)
+    rows0, _, info0 = O.msa(seqs, sim, w, steps, O.msa_params(False, distance_tree=0))
+    rows1, _, info1 = O.msa(seqs, sim, w, steps, O.msa_params(False))
+    g250 = O.profile_matrix(False, 90.0, 500, distance_tree=0)[3]
+    ran = [i for i in info0 if i.nseqs1 > 0]            # steps whose groups were not empty
+    assert ran and {i.mat_avg for i in ran} == {g250}
+    assert len({i.mat_avg for i in info1 if i.nseqs1 > 0}) > 1
+    assert any(not np.array_equal(a, b) for a, b in zip(rows0, rows1))
+
+
+def test_oracle_matrix_callback_plumbing():
+    """A caller series that returns the built-in choice gives the default
+    alignment; one that fails ends multiSeqAlign with -1 (MSA.cpp:147-152)."""
+    names, codes, offs, lens, dna, dnd = load("100_97")
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    rows, order, _ = O.msa(_seqs(codes, offs, lens), sim, w, steps, O.msa_params(dna))
+    p = O.set_profile_matrix(O.msa_params(dna), lambda pcid, ml: O.profile_matrix(False, pcid, ml))
+    rows2, order2, _ = O.msa(_seqs(codes, offs, lens), sim, w, steps, p)
+    np.testing.assert_array_equal(order, order2)
+    for a, b in zip(rows, rows2):
+        np.testing.assert_array_equal(a, b)
+    bad = O.set_profile_matrix(O.msa_params(dna), lambda pcid, ml: (0, np.zeros((32, 32)), 1.0, 0))
+    with pytest.raises(RuntimeError):
+        O.msa(_seqs(codes, offs, lens), sim, w, steps, bad)

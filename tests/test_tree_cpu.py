@@ -75,3 +75,25 @@ def test_clustal_text_reproduces_reference_aln(setname):
     reference file byte for byte (name column, line length, conservation)."""
     names, rows, order, dna = _aln_rows(setname)
     assert cwhip.clustal_text(names, rows, order, dna=dna) == text(setname + ".aln")
+
+
+def test_guide_from_tree_distance_tree_flag():
+    """readTree leaves distanceTree true for a tree with branch lengths and false
+    for a rooted one without (Tree.cpp:145, :561-565); without lengths every
+    weight is INT_SCALE/n (Tree.cpp:177-182) and the steps equal the oracle's."""
+    names, codes, offs, lens, dna, dnd = load("100_97")
+    assert cwhip.guide_from_tree(dnd, names, with_distance_tree=True)[3] == 1
+
+    def build(lo, hi):
+        if hi - lo == 1:
+            return names[lo]
+        mid = (lo + hi) // 2
+        return "(%s,%s)" % (build(lo, mid), build(mid, hi))
+    tree = build(0, len(names)) + ";\n"
+    w, sim, steps, dtree = cwhip.guide_from_tree(tree, names, with_distance_tree=True)
+    wo, simo, stepso = O.guide_from_dnd(tree, names)
+    assert dtree == 0
+    assert (w == 1000 // len(names)).all()
+    np.testing.assert_array_equal(w, wo)
+    np.testing.assert_array_equal(sim.view(np.uint64), simo.view(np.uint64))
+    np.testing.assert_array_equal(steps, stepso)
