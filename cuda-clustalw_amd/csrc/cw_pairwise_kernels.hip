@@ -101,7 +101,12 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                                        int* outE, int want_out, Track& T)
 {
     const int lane = lane_id();
-    const int ncols = S.ncols;
+    // wave-uniform values in scalar registers, so the step loop's control
+    // (chunk turns, output flushes, loop bounds) runs on the scalar unit
+    const int ncols = uni(S.ncols);
+    P = uni(P);
+    first = uni(first);
+    want_out = uni(want_out);
     const int k1 = DIRECT ? P : P - WAVE * (R - 1);    // lanes < k1 hold R rows
     const bool dummy = (!DIRECT) && (R > 1) && (lane >= k1);
     const bool lane_on = DIRECT ? (lane < P) : true;
@@ -121,10 +126,10 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         tj[r] = BIG;
     }
     int dprev = left_h(S, firstrow - 1);
-    const int ge = S.ge, goe = S.goe, thr = S.thr;
+    const int ge = uni(S.ge), goe = uni(S.goe), thr = uni(S.thr);
     lcu8* const tab = S.tab;
     lcu8* const cs = S.cs;
-    const int cd = S.cd;
+    const int cd = uni(S.cd);
 
     // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
     auto chunk = [&](int ci, int& h, int& e) {
@@ -149,13 +154,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     int caddr = S.cb + cd * (1 - lane);
     int b4 = cs[caddr];                                 // 4*code of this lane's column at t=1
 
-    auto step = [&](int t, auto masked) {
-        if (((t - 1) & 63) == 0 && t > 1) {
-            const int ci = (t - 1) >> 6;
-            cinH = nxtH;
-            cinE = nxtE;
-            if (ci + 1 < nchunks) chunk(ci + 1, nxtH, nxtE);
-        }
+    // masked: some lane may be off its valid columns; flush: this step may end
+    // an output chunk (every 64th output column, and the last one)
+    auto step = [&](int t, auto masked, auto flush) {
         const int nH = shl1(cinH, cinH);                // chunk rotation for the next step
         const int nE = shl1(cinE, cinE);
         const int inHv = shr1(cinH, outHv);             // lane 0: chunk element; others: lane above
@@ -205,7 +206,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
             coutH = shl1(outHv, coutH);
             coutE = shl1(outEv, coutE);
             const int oc = t - 63;
-            if (oc >= 1 && (((oc & 63) == 0) || oc == ncols)) {
+            if (decltype(flush)::value && oc >= 1 && (((oc & 63) == 0) || oc == ncols)) {
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
@@ -216,18 +217,34 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         }
     };
 
-    if (DIRECT) {
-        const int tend = ncols + P - 1;
-        for (int t = 1; t <= tend; ++t) step(t, Tag<true>{});
-    } else {
-        // ramp-up (lanes still entering), steady state (every lane on a valid
-        // column: no exec masking), ramp-down
-        const int tend = ncols + 63;
-        const int up_end = ncols < 63 ? ncols + 63 : 63;
-        int t = 1;
-        for (; t <= up_end; ++t) step(t, Tag<true>{});
-        for (; t <= ncols; ++t) step(t, Tag<false>{});
-        for (; t <= tend; ++t) step(t, Tag<true>{});
+    // Steps run in blocks of 64: block b covers t = 64b+1 .. 64b+64, the
+    // lane-0 input chunk turns at its start.  A block in which every lane is on
+    // a valid column (t in [64, ncols]) runs without exec masking; a full one
+    // flushes its output chunk only after its step 64b+63 (output column 64b).
+    const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
+    for (int b = 0; 64 * b + 1 <= tend; ++b) {
+        const int t0 = 64 * b + 1;
+        if (b > 0) {
+            cinH = nxtH;
+            cinE = nxtE;
+            if (b + 1 < nchunks) chunk(b + 1, nxtH, nxtE);
+        }
+        const int t1 = t0 + 63 < tend ? t0 + 63 : tend;
+        if (DIRECT || t0 < 64 || t1 > ncols) {
+#pragma unroll 1
+            for (int t = t0; t <= t1; ++t) step(t, Tag<true>{}, Tag<true>{});
+        } else if (t1 == t0 + 63) {
+#pragma unroll 1
+            for (int k = 0; k < 62; k += 2) {
+                step(t0 + k, Tag<false>{}, Tag<false>{});
+                step(t0 + k + 1, Tag<false>{}, Tag<false>{});
+            }
+            step(t0 + 62, Tag<false>{}, Tag<true>{});
+            step(t0 + 63, Tag<false>{}, Tag<false>{});
+        } else {
+#pragma unroll 1
+            for (int t = t0; t <= t1; ++t) step(t, Tag<false>{}, Tag<true>{});
+        }
     }
 
     // per-stripe reduction of the tracked cell
