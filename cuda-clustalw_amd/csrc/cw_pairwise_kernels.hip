@@ -230,9 +230,15 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
             if (b + 1 < nchunks) chunk(b + 1, nxtH, nxtE);
         }
         const int t1 = t0 + 63 < tend ? t0 + 63 : tend;
-        if (DIRECT || t0 < 64 || t1 > ncols) {
+        if (DIRECT || t0 < 64) {
 #pragma unroll 1
             for (int t = t0; t <= t1; ++t) step(t, Tag<true>{}, Tag<true>{});
+        } else if (t1 > ncols) {       // the steps up to ncols still have every lane on
+            int t = t0;
+#pragma unroll 1
+            for (; t <= ncols; ++t) step(t, Tag<false>{}, Tag<true>{});
+#pragma unroll 1
+            for (; t <= t1; ++t) step(t, Tag<true>{}, Tag<true>{});
         } else if (t1 == t0 + 63) {
 #pragma unroll 1
             for (int k = 0; k < 62; k += 2) {
@@ -422,7 +428,7 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
 // The end penalties of every node are 0 or the gap-open penalty, so a node
 // packs into {A | B << 16, M | N << 5 | tb? << 30 | te? << 31}.
 
-constexpr int kTinyRows = 8;
+constexpr int kTinyRows = 16;
 constexpr int kTinyCols = 16;
 constexpr int kTinyStack = 6;
 
@@ -875,8 +881,10 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
     lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
     for (;;) {
-        // every lane takes part; lane 0 adds 1 and its old value is the item
-        const unsigned got = atomicAdd(A.queue, lane_id() == 0 ? 1u : 0u);
+        // one atomic per pop (lane 0's); every lane reaches the broadcast, where
+        // lane 0 is the first active lane
+        unsigned got = 0;
+        if (lane_id() == 0) got = atomicAdd(A.queue, 1u);
         const int item = uni((int)got);
         if ((int64_t)item >= A.pair_end) break;
         const int64_t q = A.order ? A.order[item] : item;
