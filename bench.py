@@ -250,6 +250,18 @@ def ktuple_stage(cpu: bool):
     index = {(int(a), int(b)): k for k, (a, b) in enumerate(pairs)}
     got = np.array([dist[index[(int(pick[a]), int(pick[b]))]] for a, b in po])
     out["identical_sample"] = bool(np.array_equal(got.view(np.uint64), do.view(np.uint64)))
+    # the same run at ~500 residues (1000 sequences, 499,500 pairs), where more
+    # waves fit a CU's LDS than at ~1523
+    fam5 = seqgen.workload("mid498")
+    c5, o5, l5 = fam5.packed()
+    ctx = cwhip.Context(0)
+    ctx.load_sequences(c5, o5, l5)
+    ctx.fast_pairwise_range(0, 8, 0, 8)
+    p5, d5 = ctx.fast_pairwise_range(0, len(l5), 0, len(l5))
+    ms5 = ctx.fast_last_ms()
+    ctx.close()
+    out["mid498"] = {"sequences": len(l5), "pairs": int(len(d5)), "length_mean": round(float(l5.mean()), 1),
+                     "kernel_ms": round(ms5, 1), "pairs_per_s": round(len(d5) / (ms5 * 1e-3), 1)}
     if cpu:
         head = [np.asarray(s, np.int32) for s in fam.seqs[:180]]
         t = time.perf_counter()
@@ -319,7 +331,7 @@ def serving_stage(rate: float, count: int, seed: int = 20260105, slots: int = 4)
             "distance_jobs_p50_p99_s": [pct(kinds_a == 0, 50), pct(kinds_a == 0, 99)],
             "alignment_jobs_p50_p99_s": [pct(kinds_a == 1, 50), pct(kinds_a == 1, 99)],
             "throughput_gcups": round(cells / window / 1e9, 2), "window_s": round(float(window), 3),
-            "max_concurrent_jobs": st["max_running"], "peak_footprint_bytes": st["peak_bytes"],
+            "max_concurrent_jobs": st["max_running"], "peak_device_bytes": st["peak_bytes"], "budget_bytes": st["budget"],
             "slots": st["slots"], "spot_check_identical_to_direct_path": bool(same)}
 
 

@@ -69,6 +69,7 @@ int cwhip_create(int device, cwhip_ctx** out)
     e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) { delete ctx; return CWHIP_E_HIP; }
     ctx->num_cu = prop.multiProcessorCount;
+    ctx->bind_account();
     ctx->tune = default_table(ctx->rmax);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -89,16 +90,8 @@ void cwhip_destroy(cwhip_ctx* ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    ctx->d_codes.release(); ctx->d_offs.release(); ctx->d_lens.release();
-    ctx->d_pairs.release(); ctx->d_order.release(); ctx->d_count.release();
-    ctx->d_info.release(); ctx->d_matrix.release(); ctx->d_go.release();
-    ctx->d_status.release(); ctx->d_scratch.release(); ctx->d_queue.release();
-    ctx->d_bins.release();
-    ctx->p_pool.release(); ctx->p_out.release(); ctx->p_tiles.release(); ctx->p_probs.release();
-    ctx->h_prf_in.release(); ctx->h_prf_nodes.release();
-    ctx->p_rows.release(); ctx->p_sides.release(); ctx->h_rows.release();
-    ctx->f_scratch.release(); ctx->f_pairs.release(); ctx->f_dist.release(); ctx->f_ctr.release();
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->release_buffers();
+    ctx->h_prf_in.release(); ctx->h_prf_nodes.release(); ctx->h_rows.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);

@@ -14,15 +14,26 @@
 #include "cwhip_internal.h"
 
 // cw_fast_wave with `per_wave` LDS bytes per pair: as many waves per CU as
-// LDS holds (at most 16), grouped into workgroups of up to four so that the
-// waves of a CU are spread over its four SIMDs (with one-wave workgroups,
-// three waves per CU ran only 8% faster than two: they shared SIMDs).  Each
-// wave takes pairs from the counter on its own.
+// LDS holds (at most 16), grouped into workgroups of two to four waves so that
+// the waves of a CU are spread over its four SIMDs (with one-wave workgroups,
+// three waves per CU ran only 8% faster than two: they shared SIMDs).  The
+// group size is the one that leaves the most waves resident (7 fit: two groups
+// of 3, not one of 4); only when a single wave fits is it a group of one.
+// Each wave takes pairs from the counter on its own.
+static int waves_per_group(int per_cu)
+{
+    if (per_cu < 2) return 1;
+    int best = 2;
+    for (int w = 3; w <= 4; ++w)
+        if (w * (per_cu / w) >= best * (per_cu / best)) best = w;
+    return std::min(best, per_cu);
+}
+
 static hipError_t launch_waves(cwhip_ctx* ctx, FastArgs& a, size_t per_wave, int64_t np)
 {
     const size_t lw = (per_wave + 15) / 16 * 16;
     const int per_cu = std::max<int>(1, std::min<int>(16, (int)((160 * 1024) / lw)));
-    const int wpg = std::min(per_cu, 4);
+    const int wpg = waves_per_group(per_cu);
     const int64_t groups = (int64_t)std::max(1, ctx->num_cu) * (per_cu / wpg);
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((np + wpg - 1) / wpg, groups));
     a.lds_wave = (int64_t)lw;

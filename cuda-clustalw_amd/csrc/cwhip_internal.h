@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -17,19 +18,59 @@
 // rather than at every batch (a device free synchronises the device).
 inline size_t grown(size_t have, size_t count) { return have ? std::max(count, have + have / 2) : count; }
 
+// Device bytes a context holds, and an optional cap on them (a server sets
+// the cap to the reservation a job was admitted with).  `total`/`peak` point
+// at counters shared by several contexts (a server's slots), or are null.
+struct MemAccount {
+    int64_t bytes = 0;
+    int64_t limit = 0;                         // 0: no cap
+    std::atomic<int64_t>* total = nullptr;
+    std::atomic<int64_t>* peak = nullptr;
+    void add(int64_t d) {
+        bytes += d;
+        if (total) {
+            const int64_t now = total->fetch_add(d) + d;
+            if (peak) {
+                int64_t p = peak->load();
+                while (now > p && !peak->compare_exchange_weak(p, now)) {}
+            }
+        }
+    }
+};
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    MemAccount* acct = nullptr;
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
-        const size_t want = grown(n, count);
-        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        size_t want = grown(n, count);
+        if (acct && acct->limit) {
+            // within the cap: grow by exactly what is asked when the usual
+            // half-again step would not fit, and fail when that does not fit
+            const int64_t rest = acct->bytes - (int64_t)(n * sizeof(T));
+            if (rest + (int64_t)(std::max<size_t>(want, 1) * sizeof(T)) > acct->limit) want = count;
+            if (rest + (int64_t)(std::max<size_t>(want, 1) * sizeof(T)) > acct->limit) return hipErrorOutOfMemory;
+        }
+        release();
         hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
-        if (e == hipSuccess) n = want;
+        if (e == hipSuccess) {
+            n = want;
+            if (acct) acct->add((int64_t)(std::max<size_t>(want, 1) * sizeof(T)));
+        } else {
+            p = nullptr;
+        }
         return e;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void release() {
+        if (p) {
+            (void)hipFree(p);
+            if (acct) acct->add(-(int64_t)(std::max<size_t>(n, 1) * sizeof(T)));
+        }
+        p = nullptr;
+        n = 0;
+    }
 };
 
 // Page-locked host staging (hipHostMalloc), so copies to and from the device
@@ -95,6 +136,33 @@ struct cwhip_ctx {
     double prf_cells = 0.0, prf_entries = 0.0;
     std::vector<std::vector<uint8_t>> msa_rows;   // rows of the last cwhip_msa
     double msa_stats[6] = {0, 0, 0, 0, 0, 0};
+    MemAccount mem;                               // device bytes of the buffers above
+
+    // every device buffer of the context, for the accounting and release_buffers
+    template <typename F>
+    void for_each_buf(F f)
+    {
+        f(d_codes); f(d_offs); f(d_lens); f(d_pairs); f(d_order); f(d_count); f(d_info);
+        f(d_matrix); f(d_go); f(d_status); f(d_scratch); f(d_queue); f(d_bins);
+        f(p_pool); f(p_out); f(p_tiles); f(p_probs); f(f_scratch); f(f_pairs); f(f_dist);
+        f(f_ctr); f(p_rows); f(p_sides);
+    }
+    void bind_account() { for_each_buf([this](auto& b) { b.acct = &mem; }); }
+    // frees every device buffer (a later call allocates again); the loaded
+    // sequences and a staged pair list are gone with them
+    void release_buffers()
+    {
+        for_each_buf([](auto& b) { b.release(); });
+        nseq = 0;
+        npairs = 0;
+        h_pairs.clear();
+        if (gexec) {
+            (void)hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+        }
+        gkey.clear();
+        glast.clear();
+    }
 };
 
 static inline int fail(cwhip_ctx* c, int code, const std::string& msg)

@@ -1,18 +1,26 @@
 // cwhip_serve.cpp -- a request loop over the library for distance-matrix and
 // full-alignment jobs of different sizes arriving over time.
 //
-// One device context per slot, each with its own stream and worker thread.  A
-// job is admitted (first fit in arrival order) as soon as a slot is free and
-// its estimated device footprint fits in what the HBM budget has left, so
-// jobs arriving while others run start at once and their kernels overlap on
-// the device; a job whose footprint alone exceeds the budget is refused at
-// submission.  A job's sequences are uploaded once to its slot and stay
-// resident for all of its stages.  Every result is computed by the same entry
-// points a caller uses directly (cwhip_pairwise_range, cwhip_nj_tree,
-// cwhip_guide_from_tree, cwhip_msa, cwhip_clustal_text), so it is the same
-// alone and in any mix.
+// One device context per slot, each with its own stream and worker thread.
+// Admission is by device memory, counted as it really is: an idle slot is
+// charged the device bytes its context still holds from earlier jobs (buffers
+// are kept for reuse), a running slot the reservation its job was admitted
+// with, and the context of a running job cannot allocate beyond that
+// reservation (its buffers refuse to grow past it).  The sum of the charges
+// never exceeds the HBM budget, so neither do the bytes actually held.  Jobs
+// start in arrival order: the oldest pending job starts as soon as a slot is
+// free and its reservation fits (idle slots' buffers are released if that
+// makes it fit); a later job never overtakes it, so a large job cannot starve.
+// A job whose estimate turns out too small is run again with twice the
+// reservation (up to the budget).  Jobs on different slots run on different
+// streams, so their kernels overlap on the device.  A job's sequences are
+// uploaded once to its slot and stay resident for all of its stages.  Every
+// result is computed by the same entry points a caller uses directly
+// (cwhip_pairwise_range, cwhip_nj_tree, cwhip_guide_from_tree, cwhip_msa,
+// cwhip_clustal_text), so it is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -24,7 +32,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/cwhip.h"
+#include "cwhip_internal.h"
 
 namespace {
 
@@ -37,7 +45,8 @@ struct Job {
     std::vector<int64_t> offs;
     std::vector<int32_t> lens;
     std::vector<std::string> names;
-    int64_t footprint = 0;
+    int64_t footprint = 0;            // reservation (bytes); doubled when a run needs more
+    int tries = 0;
     double cells = 0;
     // results
     std::vector<float> dist;
@@ -70,11 +79,17 @@ int64_t footprint(const Job& j, int num_cu)
 
 }  // namespace
 
+struct Slot {
+    cwhip_ctx* ctx = nullptr;
+    Job* job = nullptr;               // assigned job (running), or null (idle)
+    int64_t charge = 0;               // idle: bytes its context holds; running: the reservation
+};
+
 struct cwhip_server {
     int device = 0;
-    int64_t budget = 0, in_use = 0, peak = 0;
+    int64_t budget = 0, peak = 0;
     int num_cu = 0, running = 0, max_running = 0;
-    std::vector<cwhip_ctx*> slots;
+    std::vector<Slot> slots;
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -83,8 +98,15 @@ struct cwhip_server {
     int64_t next_id = 1;
     bool stop = false;
     double done = 0, cells = 0;
+    std::atomic<int64_t> real_total{0}, real_peak{0};   // device bytes the slots' contexts hold
     Clock::time_point t0;
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    int64_t charged() const
+    {
+        int64_t c = 0;
+        for (const Slot& sl : slots) c += sl.charge;
+        return c;
+    }
 };
 
 namespace {
@@ -162,44 +184,101 @@ int run_job(cwhip_ctx* ctx, Job& j)
     return CWHIP_OK;
 }
 
+// Starts pending jobs in arrival order while the oldest one fits (called with
+// the server's lock held).  The slot for a job is the idle one whose held
+// buffers need the least growth; when the job does not fit the budget with
+// what idle slots hold, their buffers are released, largest first.
+void schedule(cwhip_server* srv)
+{
+    while (!srv->stop && !srv->pending.empty()) {
+        Job* j = srv->pending.front();
+        int best = -1;
+        int64_t best_growth = 0;
+        for (int s = 0; s < (int)srv->slots.size(); ++s) {
+            const Slot& sl = srv->slots[s];
+            if (sl.job) continue;
+            const int64_t growth = std::max<int64_t>(0, j->footprint - sl.charge);
+            if (best < 0 || growth < best_growth) {
+                best = s;
+                best_growth = growth;
+            }
+        }
+        if (best < 0) return;                               // no idle slot
+        if (srv->charged() + best_growth > srv->budget) {
+            // release idle slots' buffers (the chosen slot's last) until it fits
+            std::vector<int> idle;
+            for (int s = 0; s < (int)srv->slots.size(); ++s)
+                if (!srv->slots[s].job && s != best && srv->slots[s].charge > 0) idle.push_back(s);
+            std::sort(idle.begin(), idle.end(),
+                      [&](int a, int b) { return srv->slots[a].charge > srv->slots[b].charge; });
+            idle.push_back(best);
+            int64_t held_by_idle = 0;
+            for (int s : idle) held_by_idle += srv->slots[s].charge;
+            const int64_t running = srv->charged() - held_by_idle;
+            if (running + j->footprint > srv->budget) return;   // wait for running jobs
+            for (int s : idle) {
+                if (srv->charged() + std::max<int64_t>(0, j->footprint - srv->slots[best].charge) <= srv->budget)
+                    break;
+                Slot& sl = srv->slots[s];
+                (void)hipSetDevice(srv->device);
+                sl.ctx->release_buffers();
+                sl.charge = sl.ctx->mem.bytes;
+            }
+        }
+        Slot& sl = srv->slots[best];
+        srv->pending.pop_front();
+        sl.job = j;
+        sl.charge = std::max(sl.charge, j->footprint);
+        sl.ctx->mem.limit = sl.charge;
+        srv->peak = std::max(srv->peak, srv->charged());
+        ++srv->running;
+        srv->max_running = std::max(srv->max_running, srv->running);
+        j->state = 1;
+        if (j->tries == 0) j->t_start = srv->now();
+    }
+}
+
 void worker(cwhip_server* srv, int slot)
 {
-    cwhip_ctx* ctx = srv->slots[slot];
     for (;;) {
         Job* j = nullptr;
+        cwhip_ctx* ctx = nullptr;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_work.wait(lk, [&] {
-                if (srv->stop) return true;
-                for (Job* q : srv->pending)
-                    if (srv->in_use + q->footprint <= srv->budget) return true;
-                return false;
-            });
+            srv->cv_work.wait(lk, [&] { return srv->stop || srv->slots[slot].job != nullptr; });
             if (srv->stop) return;
-            for (auto it = srv->pending.begin(); it != srv->pending.end(); ++it)
-                if (srv->in_use + (*it)->footprint <= srv->budget) {
-                    j = *it;
-                    srv->pending.erase(it);
-                    break;
-                }
-            srv->in_use += j->footprint;
-            srv->peak = std::max(srv->peak, srv->in_use);
-            ++srv->running;
-            srv->max_running = std::max(srv->max_running, srv->running);
-            j->state = 1;
-            j->t_start = srv->now();
+            j = srv->slots[slot].job;
+            ctx = srv->slots[slot].ctx;
         }
         const int rc = run_job(ctx, *j);
         {
             std::lock_guard<std::mutex> lk(srv->mu);
-            j->rc = rc;
-            if (rc) j->err = cwhip_last_error(ctx);
-            j->state = rc ? 3 : 2;
-            j->t_end = srv->now();
-            srv->in_use -= j->footprint;
+            Slot& sl = srv->slots[slot];
+            sl.job = nullptr;
             --srv->running;
-            srv->done += 1;
-            srv->cells += j->cells;
+            if (rc == CWHIP_E_NOMEM && j->footprint < srv->budget) {
+                // the estimate was short: start again from empty buffers with
+                // twice the reservation, ahead of every other pending job
+                (void)hipSetDevice(srv->device);
+                ctx->release_buffers();
+                ctx->mem.limit = 0;
+                sl.charge = ctx->mem.bytes;
+                j->footprint = std::min(srv->budget, 2 * j->footprint);
+                ++j->tries;
+                j->cells = 0;
+                j->state = 0;
+                srv->pending.push_front(j);
+            } else {
+                ctx->mem.limit = 0;
+                sl.charge = ctx->mem.bytes;                // what the context keeps
+                j->rc = rc;
+                if (rc) j->err = cwhip_last_error(ctx);
+                j->state = rc ? 3 : 2;
+                j->t_end = srv->now();
+                srv->done += 1;
+                srv->cells += j->cells;
+            }
+            schedule(srv);
         }
         srv->cv_done.notify_all();
         srv->cv_work.notify_all();
@@ -234,11 +313,15 @@ int cwhip_server_create(int device, int slots, int64_t hbm_budget, cwhip_server*
         cwhip_ctx* c = nullptr;
         const int rc = cwhip_create(device, &c);
         if (rc) {
-            for (cwhip_ctx* x : srv->slots) cwhip_destroy(x);
+            for (Slot& x : srv->slots) cwhip_destroy(x.ctx);
             delete srv;
             return rc;
         }
-        srv->slots.push_back(c);
+        c->mem.total = &srv->real_total;
+        c->mem.peak = &srv->real_peak;
+        Slot sl;
+        sl.ctx = c;
+        srv->slots.push_back(sl);
     }
     for (int s = 0; s < slots; ++s) srv->workers.emplace_back(worker, srv, s);
     *out = srv;
@@ -254,7 +337,7 @@ void cwhip_server_destroy(cwhip_server* srv)
     }
     srv->cv_work.notify_all();
     for (auto& t : srv->workers) t.join();
-    for (cwhip_ctx* c : srv->slots) cwhip_destroy(c);
+    for (Slot& sl : srv->slots) cwhip_destroy(sl.ctx);
     delete srv;
 }
 
@@ -291,8 +374,9 @@ int cwhip_server_submit(cwhip_server* srv, int kind, int dna, int nseq, const ui
         *job = j->id;
         srv->pending.push_back(j.get());
         srv->jobs[j->id] = std::move(j);
+        schedule(srv);
     }
-    srv->cv_work.notify_one();
+    srv->cv_work.notify_all();
     return CWHIP_OK;
 }
 
@@ -362,7 +446,7 @@ int cwhip_server_stats(cwhip_server* srv, double* out6)
     std::lock_guard<std::mutex> lk(srv->mu);
     out6[0] = srv->done;
     out6[1] = srv->cells;
-    out6[2] = (double)srv->peak;
+    out6[2] = (double)std::max<int64_t>(srv->real_peak.load(), 0);
     out6[3] = srv->max_running;
     out6[4] = (double)srv->budget;
     out6[5] = (double)srv->slots.size();

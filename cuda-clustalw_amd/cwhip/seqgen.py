@@ -212,6 +212,7 @@ def workload(name: str, seed: int | None = None) -> Family:
     large1523 : 1000 seqs, root 1523     (499,500 pairs)
     uneven    : 2000 seqs, lengths lognormal(ln 400, 0.8) clipped [50, 3000]
     prog1002  : 1000 seqs, root 1002     (progressive-alignment workload)
+    mid498    : 1000 seqs, root 498      (k-tuple distances at ~500 residues)
     """
     if name == "small97":
         return family(20260101 if seed is None else seed, 100, 97)
@@ -223,4 +224,6 @@ def workload(name: str, seed: int | None = None) -> Family:
         return family(s, 2000, 3000, target_lens=tl)
     if name == "prog1002":
         return family(20260104 if seed is None else seed, 1000, 1002)
+    if name == "mid498":
+        return family(20260106 if seed is None else seed, 1000, 498)
     raise KeyError(name)

@@ -404,10 +404,15 @@ int cwhip_clustal_text(int nseq, const char *const *names, const uint8_t *rows, 
 /* No counterpart in the reference, which runs one alignment per process; the
  * jobs run the entry points above (the reference operations they replace are
  * listed there).  A server owns `slots` device contexts, each with its own
- * stream and worker thread.  A submitted job is admitted, first fit in arrival
- * order, as soon as a slot is free and its estimated device footprint fits in
- * what hbm_budget bytes have left; admitted jobs run concurrently and their
- * kernels overlap on the device.  Results do not depend on the mix. */
+ * stream and worker thread.  Jobs start in arrival order (none overtakes an
+ * older one) as soon as a slot is free and the job's device reservation fits
+ * in hbm_budget beside what the other slots hold: a running slot its job's
+ * reservation, an idle slot the buffers it keeps from earlier jobs (released
+ * when a job needs the room).  A running job's context cannot allocate beyond
+ * its reservation; a job that needs more is run again with twice the
+ * reservation.  So the device bytes the server holds never exceed hbm_budget.
+ * Admitted jobs run concurrently and their kernels overlap on the device.
+ * Results do not depend on the mix. */
 
 typedef struct cwhip_server cwhip_server;
 
@@ -436,8 +441,9 @@ int cwhip_server_text(cwhip_server *srv, int64_t job, int which, char *out, int6
  * [3] conventional pairwise cells + Myers-Miller cell updates of the job */
 int cwhip_server_times(cwhip_server *srv, int64_t job, double *t4);
 int cwhip_server_release(cwhip_server *srv, int64_t job);
-/* [0] jobs finished, [1] cells, [2] peak admitted footprint (bytes), [3] most jobs
- * running at once, [4] budget (bytes), [5] slots */
+/* [0] jobs finished, [1] cells, [2] peak device bytes held by the slots'
+ * contexts (counted at every allocation and release), [3] most jobs running at
+ * once, [4] budget (bytes), [5] slots */
 int cwhip_server_stats(cwhip_server *srv, double *out6);
 
 #ifdef __cplusplus

@@ -117,3 +117,38 @@ def test_serving_all_slots_busy():
                 assert dnd == ref[1] and aln == ref[2]
     finally:
         ctx.close()
+
+
+def test_serving_real_device_bytes_within_budget_and_in_order():
+    """A budget of 512 MiB, narrower than what four slots keep from earlier jobs:
+    the device bytes the slots' contexts actually hold never exceed it (idle
+    slots' buffers are released when the next job needs the room, and a running
+    job cannot grow past its reservation); jobs start in arrival order, so none
+    is overtaken; every result equals the same job on the direct path."""
+    reqs = _requests(12, seed=21)
+    srv = cwhip.Server(0, slots=4, hbm_budget=512 << 20)
+    try:
+        jobs = [srv.submit(kind, fam.seqs, fam.names) for kind, fam in reqs]
+        got = []
+        for (kind, fam), j in zip(reqs, jobs):
+            assert srv.wait(j, 300)
+            got.append(_collect(srv, j, kind))
+        starts = [srv.times(j)[1] for j in jobs]
+        assert all(a <= b for a, b in zip(starts, starts[1:]))
+        st = srv.stats()
+        assert 0 < st["peak_bytes"] <= 512 << 20
+    finally:
+        srv.close()
+    ctx = cwhip.Context(0)
+    try:
+        for (kind, fam), ref in zip(reqs, got):
+            codes, offs, lens = fam.packed()
+            ctx.load_sequences(codes, offs, lens)
+            params = cwhip.pair_params(max_len=int(lens.max()))
+            pairs, cnt, dist = ctx.pairwise_range(params, 0, len(lens), 0, len(lens))
+            np.testing.assert_array_equal(dist.view(np.uint32), ref[0].view(np.uint32))
+            if kind == cwhip.JOB_ALIGNMENT:
+                dnd, aln, rows, order = pipeline.align(ctx, fam.names, fam.seqs, False)
+                assert dnd == ref[1] and aln == ref[2]
+    finally:
+        ctx.close()
