@@ -81,3 +81,72 @@ def test_distances_in_unit_range_on_a_family():
     p, d = O.fast_pairwise(seqs)
     assert len(d) == 40 * 39 // 2
     assert d.min() >= 0.0 and d.max() <= 1.0
+
+
+# --- chains of k-tuple blocks (FastPairwiseAlign.cpp:171-403) -----------------
+# Blocks of distinct letters in fillers that the other sequence lacks: the only
+# k-tuple matches are the blocks' own, each block on one diagonal.  A block of
+# b letters is a chain of b matches that adds ktup, then 1 per step: score b.
+# A block B2 after B1 in both sequences on another flagged diagonal chains
+# from B1's best fragment at the window-gap price: b1 - wind_gap + ktup at its
+# first match, then 1 per step, so b1 + b2 - wind_gap; on the same diagonal
+# there is no price (b1 + b2).  Only the top diagonals and those within
+# `window` of them are searched at all.
+
+B1 = np.arange(0, 8, dtype=np.int32)          # 8 distinct letters
+B2 = np.arange(8, 13, dtype=np.int32)         # 5 more
+F1, F2 = 13, 14                                # filler of sequence 1 / 2 only
+
+
+def _two_blocks(gap1, gap2, lead1=10, lead2=10, tail=10):
+    s1 = np.concatenate([np.full(lead1, F1), B1, np.full(gap1, F1), B2, np.full(tail, F1)]).astype(np.int32)
+    s2 = np.concatenate([np.full(lead2, F2), B1, np.full(gap2, F2), B2, np.full(tail, F2)]).astype(np.int32)
+    return s1, s2
+
+
+def _score(s1, s2, **kw):
+    p, d = O.fast_pairwise([s1, s2], percent=0, **kw)
+    return round((1.0 - d[0]) * 100.0, 6)      # percent off: d = (100 - score) / 100
+
+
+def test_single_block_scores_its_length():
+    s1 = np.concatenate([np.full(20, F1), B1, np.full(15, F1)]).astype(np.int32)
+    s2 = np.concatenate([np.full(5, F2), B1, np.full(30, F2)]).astype(np.int32)
+    assert _score(s1, s2) == 8
+    for ktup in (1, 2):
+        assert _score(s1, s2, ktup=ktup) == 8
+    # percent on: score / min length x 100
+    p, d = O.fast_pairwise([s1, s2])
+    assert d[0] == (100.0 - (8.0 / 43.0) * 100.0) / 100.0
+
+
+def test_two_blocks_chain_with_the_window_gap():
+    # B2 three diagonals off B1: chained, charged wind_gap once
+    s1, s2 = _two_blocks(4, 7)
+    assert _score(s1, s2) == 8 + 5 - 3
+    assert _score(s1, s2, wind_gap=1) == 8 + 5 - 1
+    assert _score(s1, s2, wind_gap=6) == 8                      # chaining would cost more
+    # the same diagonal: the chain goes on at no price across the filler
+    s1, s2 = _two_blocks(4, 4)
+    assert _score(s1, s2) == 8 + 5
+    # B2 before B1 in sequence 2: no chain, the longer block alone
+    s1 = np.concatenate([np.full(10, F1), B1, np.full(4, F1), B2, np.full(10, F1)]).astype(np.int32)
+    s2 = np.concatenate([np.full(10, F2), B2, np.full(4, F2), B1, np.full(10, F2)]).astype(np.int32)
+    assert _score(s1, s2) == 8
+
+
+def test_only_top_diagonals_and_their_window_are_searched():
+    # desQuickSort leaves the last of the tl1 diagonals in place (its top
+    # partition ends at tl1 - 1, :584-640) and the top-SIGNIF loop starts at
+    # that unsorted last one (:236-251): signif 2 flags the one best diagonal,
+    # signif 1 only the corner diagonal l1 + l2 - 1, which no block is on
+    s1, s2 = _two_blocks(4, 7)                     # B2 three diagonals off B1
+    assert _score(s1, s2, signif=1) == 0
+    assert _score(s1, s2, signif=2) == 10          # inside window 5 of B1's diagonal
+    assert _score(s1, s2, signif=2, window=3) == 10
+    assert _score(s1, s2, signif=2, window=2) == 8  # outside: B2 adds nothing
+    s1, s2 = _two_blocks(4, 16)                    # twelve diagonals off
+    assert _score(s1, s2, signif=2) == 8
+    assert _score(s1, s2, signif=3) == 10          # B2's diagonal is a top one itself
+    assert _score(s1, s2, signif=2, window=12) == 10
+    assert _score(s1, s2, signif=2, window=11) == 8

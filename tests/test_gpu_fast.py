@@ -176,3 +176,21 @@ def test_fast_random_cases(ctx):
     for seed in range(300):
         seqs, dna, kw, max_aln = _random_case(seed)
         _check(ctx, seqs, dna, max_aln=max_aln, **kw)
+
+
+def test_fast_block_chains(ctx):
+    """The closed-form block layouts of tests/test_fast_oracle.py (one block,
+    two chained blocks, the same diagonal, crossing blocks, top diagonals and
+    their window) on the device, against the oracle, under the parameters that
+    change their scores."""
+    from test_fast_oracle import B1, B2, F1, F2, _two_blocks
+    single = (np.concatenate([np.full(20, F1), B1, np.full(15, F1)]).astype(np.uint8),
+              np.concatenate([np.full(5, F2), B1, np.full(30, F2)]).astype(np.uint8))
+    cross = (np.concatenate([np.full(10, F1), B1, np.full(4, F1), B2, np.full(10, F1)]).astype(np.uint8),
+             np.concatenate([np.full(10, F2), B2, np.full(4, F2), B1, np.full(10, F2)]).astype(np.uint8))
+    layouts = [single, cross] + [tuple(x.astype(np.uint8) for x in _two_blocks(4, g)) for g in (4, 7, 16)]
+    for s1, s2 in layouts:
+        for kw in (dict(), dict(wind_gap=1), dict(wind_gap=6), dict(signif=1), dict(signif=2),
+                   dict(signif=2, window=2), dict(signif=3), dict(signif=2, window=12), dict(ktup=2),
+                   dict(percent=0)):
+            _check(ctx, [s1, s2], **kw)

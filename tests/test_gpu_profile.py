@@ -182,3 +182,50 @@ def test_gpu_msa_random_families(ctx):
         np.testing.assert_array_equal(order, r_order)
         for i in range(n):
             np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
+
+
+def test_delayed_pass_is_one_profile_align_step(ctx):
+    """MSA::multiSeqAlign's second pass (MSA.cpp:187-335) for one divergent
+    sequence X: the first pass aligns the others, then X is aligned to all of
+    them in one profileAlign step with the weights recomputed as tree weight x
+    identity with X, normalised to INT_SCALE_FACTOR (:266-296).  So the final
+    alignment equals one cwhip_profile_align step of X against the first-pass
+    rows (the final rows without X and without the columns only X fills), with
+    those weights, and X comes last in the output order."""
+    fam = seqgen.family(4711, 9, 180, branch_mean=0.15, name_prefix="d")
+    rng = np.random.default_rng(4711)
+    x = rng.choice([0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 17, 18, 20, 21, 23], 170).astype(np.uint8)
+    seqs, names = fam.seqs + [x], fam.names + ["xout"]
+    n = len(seqs)
+    codes = np.concatenate(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    pairs = O.range_pairs(n, 0, n, 0, n)
+    _, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring())
+    dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    maxid = [max(int(sim[i, j]) for j in range(1, n + 1) if j != i) for i in range(1, n + 1)]
+    assert maxid[-1] <= 30 and min(maxid[:-1]) > 30         # X alone is delayed
+    rows, order, stats = ctx.msa(seqs, sim, w, steps)
+    r_rows, r_order, _ = O.msa([s.astype(np.int32) for s in seqs], sim, w, steps, O.msa_params(False))
+    np.testing.assert_array_equal(order, r_order)
+    assert order[-1] == n
+    S = np.array(rows[:-1])
+    first_pass = S[:, (S < 30).any(axis=0)]
+    wt = np.array(w, np.int64).copy()
+    wt[:-1] = (w[:-1].astype(np.float64) * sim[1:n, n]).astype(np.int64)
+    total = int(wt[:-1].sum())
+    if total == 0:
+        wt[:] = 1
+        total = n
+    wt[:-1] = np.maximum((wt[:-1] * 1000) // total, 1)
+    group = np.zeros(n + 1, np.int32)
+    group[1:n] = 1
+    group[n] = 2
+    from cwhip.lib import msa_params, profile_align
+    info, g, new = profile_align(ctx, [r for r in first_pass] + [x], sim, wt.astype(np.int32), group,
+                                 np.ones(n + 1, np.int32), msa_params(False))
+    assert info[0] >= 0
+    for i in range(n):
+        np.testing.assert_array_equal(new[i], rows[i])

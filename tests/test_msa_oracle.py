@@ -8,6 +8,8 @@ order.  One wrong tie (progDiff midpoint, base case, progAdd/progDel order),
 one wrong float rounding in a profile builder, or one wrong gap coefficient
 moves a gap and fails the row comparison.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -256,3 +258,47 @@ def test_oracle_matrix_callback_plumbing():
     bad = O.set_profile_matrix(O.msa_params(dna), lambda pcid, ml: (0, np.zeros((32, 32)), 1.0, 0))
     with pytest.raises(RuntimeError):
         O.msa(_seqs(codes, offs, lens), sim, w, steps, bad)
+
+
+MATRICES_H = "/root/reference/CPU/clustalw-2.0.11/src/substitutionMatrix/matrices.h"
+
+
+def _tables(text, pattern):
+    """name -> list of ints of every `short name[]={...};` / `name[25*25] = {...};` block."""
+    import re
+    out = {}
+    for m in re.finditer(pattern, text, re.S):
+        body = re.sub(r"/\*.*?\*/", "", m.group(2), flags=re.S)
+        out[m.group(1)] = [int(x) for x in re.findall(r"-?\d+", body)]
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(MATRICES_H), reason="reference sources not present")
+def test_matrix_tables_equal_the_reference_text():
+    """The Gonnet series and the two DNA tables that the library and the oracle
+    compile (cuda-clustalw_amd/csrc/cw_matrices.inc) against the reference's
+    matrices.h, read here as text: lower triangles in the reference's letter
+    order (SubMatrix.cpp:179-180), re-indexed by residue code."""
+    ref = _tables(open(MATRICES_H).read(), r"short\s+(\w+)\s*\[\]\s*=\s*\{(.*?)\};")
+    inc_path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "cuda-clustalw_amd", "csrc", "cw_matrices.inc")
+    inc = _tables(open(inc_path).read(), r"static const short\s+(\w+)\s*\[25 \* 25\]\s*=\s*\{(.*?)\};")
+    codes = "ABCDEFGHIKLMNOPQRSTUVWXYZ"
+    aa, dna = "ABCDEFGHIKLMNPQRSTVWXYZ", "ABCDGHKMNRSTUVWXY"
+    pairs = [("gon80mt", "cw_gonnet80", aa), ("gon120mt", "cw_gonnet120", aa),
+             ("gon160mt", "cw_gonnet160", aa), ("gon250mt", "cw_gonnet250", aa),
+             ("gon350mt", "cw_gonnet350", aa), ("swgapdnamt", "cw_iub_dna", dna),
+             ("clustalvdnamt", "cw_clustalv_dna", dna)]
+    for rname, iname, order in pairs:
+        tri = ref[rname]
+        n = len(order)
+        assert len(tri) == n * (n + 1) // 2, rname
+        full = np.zeros((25, 25), np.int64)
+        k = 0
+        for i in range(n):
+            for j in range(i + 1):
+                a, b = codes.index(order[i]), codes.index(order[j])
+                full[a, b] = full[b, a] = tri[k]
+                k += 1
+        got = np.array(inc[iname], np.int64).reshape(25, 25)
+        np.testing.assert_array_equal(got, full, err_msg=rname)
