@@ -417,6 +417,14 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
     }
 }
 
+template <int C>
+__device__ __attribute__((noinline)) void halves_cols_call(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B,
+                                                           int M, int N, int tb, int te, int go, int ge,
+                                                           int* fH, int* fE, int* rH, int* rE)
+{
+    mm_halves_cols<C>(tab, s1, s2, uni(A), uni(B), uni(M), uni(N), uni(tb), uni(te), uni(go), uni(ge), fH,
+                      fE, rH, rE);
+}
 
 // ---- tiny subtrees, one lane each ---------------------------------------------
 // A node with 2 <= M <= kTinyRows and N <= kTinyCols, and every M == 1 leaf, is
@@ -658,10 +666,17 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         if (N <= colcap) {
             fH = smallF; fE = smallF + (small_cap + 1);
             rH = smallR; rE = smallR + (small_cap + 1);
-            if (N <= 32) mm_halves_cols<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
-            else if (N <= 64) mm_halves_cols<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
-            else if (N <= 128) mm_halves_cols<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
-            else mm_halves_cols<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+            // C = ceil(N / 32) columns per lane: every lane of a half in use
+            switch ((N + 31) >> 5) {
+            case 1: halves_cols_call<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 2: halves_cols_call<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 3: halves_cols_call<3>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 4: halves_cols_call<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 5: halves_cols_call<5>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 6: halves_cols_call<6>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 7: halves_cols_call<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            default: halves_cols_call<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            }
         } else {
             Track T{0, 0};
             Sweep F = base;
