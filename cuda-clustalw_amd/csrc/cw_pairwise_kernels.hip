@@ -66,6 +66,11 @@ __device__ __forceinline__ void wave_fence() {
 #define CW_LDS __attribute__((address_space(3)))
 typedef CW_LDS const uint8_t lcu8;
 typedef CW_LDS uint8_t lu8;
+// Global address space for the boundary rows in HBM: global (not flat)
+// loads and stores, which an LDS wait (lgkmcnt) does not wait for.
+#define CW_GLB __attribute__((address_space(1)))
+typedef CW_GLB int gint;
+typedef CW_GLB const int gcint;
 
 struct Sweep {
     int nrows, ncols;
@@ -138,8 +143,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
             h = S.th0 + col * S.ths;
             e = h + S.teo;
         } else if (col <= ncols) {
-            h = inH[col];
-            e = inE[col];
+            h = ((gcint*)inH)[col];
+            e = ((gcint*)inE)[col];
         } else {
             h = 0;
             e = 0;
@@ -198,8 +203,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
             outHv = hup;
             outEv = eup;
             if (DIRECT && want_out && lane == P - 1) {
-                outH[c] = hup;
-                outE[c] = eup;
+                ((gint*)outH)[c] = hup;
+                ((gint*)outE)[c] = eup;
             }
         }
         if (!DIRECT && want_out) {
@@ -210,8 +215,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
-                    outH[col] = coutH;
-                    outE[col] = coutE;
+                    ((gint*)outH)[col] = coutH;
+                    ((gint*)outE)[col] = coutE;
                 }
             }
         }
@@ -415,6 +420,111 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
             else { rH[jj] = hup[c]; rE[jj] = e[c]; }
         }
     }
+}
+
+// Both halves of a Myers-Miller node on the row path in one sweep, for nodes
+// whose halves have 33..256 rows: lanes 0..31 sweep the forward half (rows
+// 1..midi from corner (A,B), end penalty tb), lanes 32..63 the reverse half
+// (rows M..midi+1 from corner (A+M,B+N), end penalty te), each half a single
+// stripe of 32 lanes with R rows per lane (the row path's recurrence and
+// dummy first register), so a node costs one ramp of 31 steps instead of two
+// sweeps with 63-step ramps, at twice the rows per lane.  Lanes 0 and 32 take
+// their half's top row from its formula; lanes 31 and 63 write their half's
+// final row (H and E at each column, indexed by the half's own column) to
+// fH/fE and rH/rE, the layout the two row-path sweeps leave.
+template <int R>
+__device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B, int M,
+                                                      int N, int tb, int te, int go, int ge, int* fH,
+                                                      int* fE, int* rH, int* rE)
+{
+    A = uni(A); B = uni(B); M = uni(M); N = uni(N);
+    tb = uni(tb); te = uni(te); go = uni(go); ge = uni(ge);
+    const int lane = lane_id();
+    const int grp = lane >> 5, L = lane & 31;
+    const int midi = M / 2;
+    const int P = grp ? M - midi : midi;
+    const int t0v = grp ? te : tb;
+    const int rb = grp ? A + M + 1 : A, rd = grp ? -1 : 1;    // code of half-row i: s1[rb + rd*i]
+    const int cb = grp ? B + N + 1 : B, cd = grp ? -1 : 1;    // 4*code of half-column j: s2[cb + cd*j]
+    const int goe = go + ge;
+    const int k1 = P - 32 * (R - 1);                           // lanes < k1 hold R rows
+    const bool dummy = (R > 1) && (L >= k1);
+    const int d = dummy ? 1 : 0;
+    const int firstrow = 1 + (L < k1 ? L * R : k1 * R + (L - k1) * (R - 1));
+    int sb[R], hL[R], hLg[R], f[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = firstrow + r - d;
+        const int rr = row < 1 ? 1 : (row > P ? P : row);
+        sb[r] = (int)s1[rb + rd * rr] * 128;
+        hL[r] = row == 0 ? 0 : -t0v - row * ge;                // left column H[row][0]
+        f[r] = hL[r] - go;
+        hLg[r] = hL[r] - goe;
+    }
+    int dprev = (firstrow == 1) ? 0 : -t0v - (firstrow - 1) * ge;
+    int outHv = 0, outEv = 0;
+    int caddr = cb + cd * (1 - L);
+    int b4 = s2[caddr];
+    gint* const oH = (gint*)(grp ? rH : fH);
+    gint* const oE = (gint*)(grp ? rE : fE);
+    const bool top = (L == 0), bottom = (L == 31);
+
+    auto step = [&](int t, auto masked) {
+        // lanes 0 and 32: their half's top row H[0][t] = -t0 - t*ge, E = H - go
+        const int topH = -t0v - t * ge;
+        int inHv = shr1(topH, outHv);
+        int inEv = shr1(topH - go, outEv);
+        if (top) {
+            inHv = topH;
+            inEv = topH - go;
+        }
+        const int bcur = b4;
+        caddr += cd;
+        b4 = s2[caddr];
+        const int c = t - L;
+        bool act = true;
+        if constexpr (decltype(masked)::value) act = c >= 1 && c <= N;
+        if (act) {
+            int hup = inHv, eup = inEv, dg = dprev;
+            int hupg = inHv - goe;
+            dprev = inHv;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int sc = score_at(tab, sb[r] + bcur);
+                int e = max(eup - ge, hupg);
+                f[r] = max(f[r] - ge, hLg[r]);
+                int h = max(max(dg + sc, e), f[r]);
+                if (r == 0 && R > 1) {
+                    if (dummy) { h = hup; e = eup; }
+                }
+                dg = hL[r];
+                hL[r] = h;
+                hLg[r] = h - goe;
+                hup = h;
+                hupg = hLg[r];
+                eup = e;
+            }
+            outHv = hup;
+            outEv = eup;
+            if (bottom) {
+                oH[c] = hup;
+                oE[c] = eup;
+            }
+        }
+    };
+    // every lane on a valid column for t in [32, N]
+    const int tend = N + 31;
+    int t = 1;
+    const int up_end = N < 31 ? tend : 31;
+#pragma unroll 1
+    for (; t <= up_end; ++t) step(t, Tag<true>{});
+#pragma unroll 1
+    for (; t + 1 <= N; t += 2) {
+        step(t, Tag<false>{});
+        step(t + 1, Tag<false>{});
+    }
+#pragma unroll 1
+    for (; t <= tend; ++t) step(t, Tag<true>{});
 }
 
 template <int C>
@@ -676,6 +786,18 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             case 6: halves_cols_call<6>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             case 7: halves_cols_call<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             default: halves_cols_call<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            }
+        } else if (midi >= 33 && M - midi <= 32 * CW_RMAX) {
+            fH = g2H; fE = g2E;
+            rH = g3H; rE = g3E;
+            switch ((M - midi + 31) >> 5) {               // rows per lane of the taller half
+            case 2: dual_halves<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 3: dual_halves<3>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 4: dual_halves<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 5: dual_halves<5>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 6: dual_halves<6>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 7: dual_halves<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            default: dual_halves<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             }
         } else {
             Track T{0, 0};
