@@ -423,15 +423,16 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
 }
 
 // Both halves of a Myers-Miller node on the row path in one sweep, for nodes
-// whose halves have 33..256 rows: lanes 0..31 sweep the forward half (rows
+// whose halves have at most 256 rows: lanes 0..31 sweep the forward half (rows
 // 1..midi from corner (A,B), end penalty tb), lanes 32..63 the reverse half
 // (rows M..midi+1 from corner (A+M,B+N), end penalty te), each half a single
 // stripe of 32 lanes with R rows per lane (the row path's recurrence and
 // dummy first register), so a node costs one ramp of 31 steps instead of two
-// sweeps with 63-step ramps, at twice the rows per lane.  Lanes 0 and 32 take
-// their half's top row from its formula; lanes 31 and 63 write their half's
-// final row (H and E at each column, indexed by the half's own column) to
-// fH/fE and rH/rE, the layout the two row-path sweeps leave.
+// sweeps with 63-step ramps, at twice the rows per lane.  With R == 1 a half of
+// P < 32 rows uses lanes 0..P-1 of its 32.  Lanes 0 and 32 take their half's
+// top row from its formula; the last lane of each half writes the half's final
+// row (H and E at each column, indexed by the half's own column) to fH/fE and
+// rH/rE, the layout the two row-path sweeps leave.
 template <int R>
 __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B, int M,
                                                       int N, int tb, int te, int go, int ge, int* fH,
@@ -449,6 +450,10 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
     const int goe = go + ge;
     const int k1 = P - 32 * (R - 1);                           // lanes < k1 hold R rows
     const bool dummy = (R > 1) && (L >= k1);
+    // R == 1: lanes 0..P-1 hold a row each, the others idle
+    const int used = (R > 1) ? 32 : uni(M - midi);             // lanes of the taller half
+    const bool lane_on = (R > 1) || (L < P);
+    const int lastlane = (R > 1) ? 31 : P - 1;
     const int d = dummy ? 1 : 0;
     const int firstrow = 1 + (L < k1 ? L * R : k1 * R + (L - k1) * (R - 1));
     int sb[R], hL[R], hLg[R], f[R];
@@ -467,7 +472,7 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
     int b4 = s2[caddr];
     gint* const oH = (gint*)(grp ? rH : fH);
     gint* const oE = (gint*)(grp ? rE : fE);
-    const bool top = (L == 0), bottom = (L == 31);
+    const bool top = (L == 0), bottom = (L == lastlane);
 
     auto step = [&](int t, auto masked) {
         // lanes 0 and 32: their half's top row H[0][t] = -t0 - t*ge, E = H - go
@@ -483,7 +488,7 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
         b4 = s2[caddr];
         const int c = t - L;
         bool act = true;
-        if constexpr (decltype(masked)::value) act = c >= 1 && c <= N;
+        if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= N;
         if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
             int hupg = inHv - goe;
@@ -512,16 +517,18 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
             }
         }
     };
-    // every lane on a valid column for t in [32, N]
-    const int tend = N + 31;
+    // every lane on a valid column for t in [32, N] (when all 32 lanes hold rows)
+    const int tend = N + used - 1;
     int t = 1;
-    const int up_end = N < 31 ? tend : 31;
+    if (used == 32) {
+        const int up_end = N < 31 ? tend : 31;
 #pragma unroll 1
-    for (; t <= up_end; ++t) step(t, Tag<true>{});
+        for (; t <= up_end; ++t) step(t, Tag<true>{});
 #pragma unroll 1
-    for (; t + 1 <= N; t += 2) {
-        step(t, Tag<false>{});
-        step(t + 1, Tag<false>{});
+        for (; t + 1 <= N; t += 2) {
+            step(t, Tag<false>{});
+            step(t + 1, Tag<false>{});
+        }
     }
 #pragma unroll 1
     for (; t <= tend; ++t) step(t, Tag<true>{});
@@ -787,10 +794,11 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             case 7: halves_cols_call<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             default: halves_cols_call<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             }
-        } else if (midi >= 33 && M - midi <= 32 * CW_RMAX) {
+        } else if (M - midi <= 32 * CW_RMAX) {
             fH = g2H; fE = g2E;
             rH = g3H; rE = g3E;
             switch ((M - midi + 31) >> 5) {               // rows per lane of the taller half
+            case 1: dual_halves<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             case 2: dual_halves<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             case 3: dual_halves<3>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             case 4: dual_halves<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
