@@ -47,13 +47,26 @@ __device__ __forceinline__ int shl1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
 }
 
+// Wave reductions with DPP (all 64 lanes active): within quads, half-rows and
+// rows of 16, then row_bcast:15 / :31 carry the rows into row 3, so lane 63
+// holds the result, which is read into a scalar register.  Six VALU steps,
+// no LDS traffic.
+template <typename Op>
+__device__ __forceinline__ int wave_reduce(int x, Op op)
+{
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false));  // row_half_mirror
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false));  // row_mirror
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));  // row_bcast:15 -> rows 1, 3
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));  // row_bcast:31 -> rows 2, 3
+    return __builtin_amdgcn_readlane(x, 63);
+}
 __device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
-    return v;
+    return wave_reduce(v, [](int a, int b) { return a > b ? a : b; });
 }
 __device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
-    return v;
+    return wave_reduce(v, [](int a, int b) { return a < b ? a : b; });
 }
 
 // make LDS/global writes of this wave visible to its later reads
@@ -709,8 +722,7 @@ __device__ __attribute__((noinline)) void pw_tiny_subtree(lcu8* tab, lcu8* s1, l
 }
 
 __device__ __forceinline__ int wave_sum(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+    return wave_reduce(v, [](int a, int b) { return a + b; });
 }
 
 // FullPairwiseAlign::diff (:418-612) restated as a depth-first walk; returns the
