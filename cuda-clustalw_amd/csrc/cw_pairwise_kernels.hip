@@ -46,6 +46,10 @@ __device__ __forceinline__ int shr1(int old, int src) {
 __device__ __forceinline__ int shl1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
 }
+// lane l receives src[l+1]; lane 63 receives 0 (no copy of an old value)
+__device__ __forceinline__ int shl1z(int src) {
+    return __builtin_amdgcn_mov_dpp(src, 0x130, 0xf, 0xf, true);
+}
 
 // Wave reductions with DPP (all 64 lanes active): within quads, half-rows and
 // rows of 16, then row_bcast:15 / :31 carry the rows into row 3, so lane 63
@@ -175,8 +179,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     // masked: some lane may be off its valid columns; flush: this step may end
     // an output chunk (every 64th output column, and the last one)
     auto step = [&](int t, auto masked, auto flush) {
-        const int nH = shl1(cinH, cinH);                // chunk rotation for the next step
-        const int nE = shl1(cinE, cinE);
+        const int nH = shl1z(cinH);                     // chunk rotation for the next step (lane 63's
+        const int nE = shl1z(cinE);                     // element is past the chunk's last use)
         const int inHv = shr1(cinH, outHv);             // lane 0: chunk element; others: lane above
         const int inEv = shr1(cinE, outEv);
         cinH = nH;

@@ -73,14 +73,24 @@ __device__ __forceinline__ int max3(int a, int b, int c) { return max(max(a, b),
 #define CWP_LDS __attribute__((address_space(3)))
 typedef CWP_LDS int lint;
 
+// Global address space: these arrays are read with global (not flat) loads,
+// which LDS waits (lgkmcnt) do not wait for.
+#define CWP_GLB __attribute__((address_space(1)))
+typedef CWP_GLB const int gcint;
+typedef CWP_GLB const int4 gcint4;
+typedef CWP_GLB int gint;
+
+// An int4 read through a global-address-space pointer.
+__device__ __forceinline__ int4 ld4(gcint4* q) { return make_int4(q->x, q->y, q->z, q->w); }
+
 // One profile pair as the device sees it.
 struct Prob {
     int L1, L2, ldS;
-    const int4* row;   // {g1, l1, m1, g1+l1}[I], I = 0..L1+1
-    const int4* col;   // {g2, l2, m2, g2+l2}[J], J = 0..L2+1
-    const int* pl1;    // pl1[x] = sum_{y<x} l1[y]
-    const int* pl2;
-    const int* S;      // S[J*ldS + I]
+    gcint4* row;       // {g1, l1, m1, g1+l1}[I], I = 0..L1+1
+    gcint4* col;       // {g2, l2, m2, g2+l2}[J], J = 0..L2+1
+    gcint* pl1;        // pl1[x] = sum_{y<x} l1[y]
+    gcint* pl2;
+    gcint* S;          // S[J*ldS + I]
 };
 
 __device__ __forceinline__ int open1(const Prob& p, int I, int J) { return p.row[I].z & (p.row[I].x + p.col[J].x); }
@@ -121,7 +131,7 @@ __device__ __forceinline__ int left_h(const PSweep& W, int i) {
     if (i <= 0) return 0;
     if (MODE == MODE_FWD) return W.lt0 + i * W.ltl;
     const int I = W.ib + W.id * i;
-    const int4 ce = W.p.col[W.jedge];
+    const int4 ce = ld4(&W.p.col[W.jedge]);
     const int g = W.go2 ? -(ce.z & (W.p.row[I].x + ce.x)) : 0;
     return g - (ce.z & (W.p.pl1[W.ib] - W.p.pl1[I]));
 }
@@ -177,8 +187,8 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
         const int i = firstrow + r - d;
         const int ii = i < 1 ? 1 : (i > W.nrows ? W.nrows : i);
         const int I = W.ib + W.id * ii;
-        const int4 rf = p.row[MODE == MODE_FWD ? I : I - 1];
-        const int4 re = p.row[I];
+        const int4 rf = ld4(&p.row[MODE == MODE_FWD ? I : I - 1]);
+        const int4 re = ld4(&p.row[I]);
         gF[r] = rf.x;
         mF[r] = rf.z;
         gE[r] = (MODE == MODE_FWD) ? re.w : re.x;
@@ -194,8 +204,8 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
         if (!DIRECT && sy.wait) wait_chunks(sy, ci + 1);
         const int col = 64 * ci + 1 + lane;
         if (col <= ncols) {
-            h = inH[col];
-            e = first ? NEG : inE[col];
+            h = ((gcint*)inH)[col];
+            e = first ? NEG : ((gcint*)inE)[col];
         } else {
             h = 0;
             e = NEG;
@@ -217,9 +227,10 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
     int nsv[R];
     auto prefetch = [&](int c) {
         const int J = colJ(c);
-        nrecF = p.col[J];
-        nrecE = (MODE == MODE_FWD) ? nrecF : p.col[J - 1];
-        const int* Sc = p.S + (int64_t)J * ldS;
+        nrecF = ld4(&p.col[J]);
+        if constexpr (MODE == MODE_FWD) nrecE = nrecF;
+        else nrecE = ld4(&p.col[J - 1]);
+        gcint* Sc = p.S + (int64_t)J * ldS;
 #pragma unroll
         for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
     };
@@ -276,8 +287,8 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
             outHv = hup;
             outEv = eup;
             if (DIRECT && lane == P - 1) {
-                outH[c] = hup;
-                outE[c] = eup;
+                ((gint*)outH)[c] = hup;
+                ((gint*)outE)[c] = eup;
             }
         }
         if (!DIRECT) {
@@ -288,8 +299,8 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
-                    outH[col] = coutH;
-                    outE[col] = coutE;
+                    ((gint*)outH)[col] = coutH;
+                    ((gint*)outE)[col] = coutE;
                 }
                 if (sy.pub) {
                     wave_fence();
@@ -426,7 +437,7 @@ __device__ void fill_top(const Prob& p, int A, int B, int M, int N, int* h0)
         const int t0 = -open1(p, A, B + 1), tl = -ext1(p, A, B + 1);
         for (int c = 1 + lane; c <= N; c += WAVE) h0[c] = t0 + c * tl;
     } else {
-        const int4 re = p.row[A + M];
+        const int4 re = ld4(&p.row[A + M]);
         for (int c = 1 + lane; c <= N; c += WAVE) {
             const int J = B + N - c + 1;
             h0[c] = -(re.z & (re.x + p.col[J].x)) - (re.z & (p.pl2[B + N + 1] - p.pl2[J]));
@@ -546,7 +557,7 @@ __device__ __attribute__((noinline)) void do_mid(const Prob& p, int* nd, int is_
     // HH[0]: left column of forward row midi; RR[N]: left column of reverse row M-midi
     const int HH0 = (go1 ? -open2(p, A + 1, B) : 0) - midi * ext2(p, A + 1, B);
     const int Ie = A + midi + 1;
-    const int4 ce = p.col[B + N];
+    const int4 ce = ld4(&p.col[B + N]);
     const int RRN = (go2 ? -(ce.z & (p.row[Ie].x + ce.x)) : 0) - (ce.z & (p.pl1[A + M + 1] - p.pl1[Ie]));
     const int* fH = lv.fH;
     const int* fE = lv.fE;
@@ -1070,12 +1081,12 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
 
     // copy in, 16 loads in flight per lane before the LDS stores
     for (int x = lane; x < M0 + 2; x += WAVE) {
-        const int4 r = p.row[A0 + x];
+        const int4 r = ld4(&p.row[A0 + x]);
         lrow[4 * x] = r.x; lrow[4 * x + 1] = r.y; lrow[4 * x + 2] = r.z; lrow[4 * x + 3] = r.w;
         lpl1[x] = p.pl1[A0 + x];
     }
     for (int x = lane; x < N0 + 2; x += WAVE) {
-        const int4 c = p.col[B0 + x];
+        const int4 c = ld4(&p.col[B0 + x]);
         lcol[4 * x] = c.x; lcol[4 * x + 1] = c.y; lcol[4 * x + 2] = c.z; lcol[4 * x + 3] = c.w;
         lpl2[x] = p.pl2[B0 + x];
     }
@@ -1173,11 +1184,11 @@ __device__ static Prob make_prob(const PrfArgs& a, const PrfProb& pp)
     p.L1 = pp.L1;
     p.L2 = pp.L2;
     p.ldS = pp.ldS;
-    p.row = (const int4*)(a.pool + pp.row);
-    p.col = (const int4*)(a.pool + pp.col);
-    p.pl1 = a.pool + pp.pl1;
-    p.pl2 = a.pool + pp.pl2;
-    p.S = a.pool + pp.S;
+    p.row = (gcint4*)(a.pool + pp.row);
+    p.col = (gcint4*)(a.pool + pp.col);
+    p.pl1 = (gcint*)(a.pool + pp.pl1);
+    p.pl2 = (gcint*)(a.pool + pp.pl2);
+    p.S = (gcint*)(a.pool + pp.S);
     return p;
 }
 
