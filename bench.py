@@ -31,6 +31,7 @@ METRIC = "pairwise GCUPS (1000x1523), batched vs plain; serving p50/p99 latency"
 PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # CUs x SIMDs x lanes x clock (Tops/s)
 PEAK_HBM_GBS = 8000.0
 OPS_PER_CELL = 10                                 # int32 ops per Gotoh cell update
+PROFILE_OPS_PER_CELL = 10 + 2 * 27                # with prfScore's 27 multiply-adds
 
 
 def ref_cells(lens, pairs, info):
@@ -41,6 +42,25 @@ def ref_cells(lens, pairs, info):
     ms, se1, se2, sb1, sb2 = (info[:, k].astype(np.int64) for k in range(5))
     rev = np.where(ms > 0, (se1 - sb1) * se2 + (se2 - sb2 + 1), 0)
     return n * m, rev, info[:, 7].astype(np.int64)
+
+
+def host_cpu():
+    """The host CPU the baseline ran on: model name, logical CPUs in the
+    machine and CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
 
 
 def cpu_baseline(codes, offs, lens, pairs, seconds, threads=None):
@@ -65,7 +85,7 @@ def cpu_baseline(codes, offs, lens, pairs, seconds, threads=None):
     dt = time.perf_counter() - t
     cells = (lens[sample[:, 0]].astype(np.int64) * lens[sample[:, 1]]).sum()
     return {"value": round(float(cells) / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
-            "kind": "port",
+            "kind": "port", **host_cpu(),
             "sample": "every %dth pair of the workload (%d pairs, %.1f s, %d OpenMP threads)"
                       % (stride, len(sample), dt, threads)}
 
@@ -193,12 +213,26 @@ def progressive_stage(cpu: bool):
     ops = st["cells"] * OPS_PER_CELL + st["prf_entries"] * 54
     dev_s = st["device_ms"] * 1e-3
     ach = ops / dev_s / 1e12
+    # the per-cell form: every Myers-Miller cell update charged 10 + 2 x 27 ops,
+    # as if prfScore were recomputed at each update (what the reference does)
+    ach64 = st["cells"] * PROFILE_OPS_PER_CELL / dev_s / 1e12
+    traffic, tsrc = None, None
+    tfile = os.path.join(HERE, "profiles", "pmc_traffic_prog1002.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        traffic, tsrc = int(t["traffic_bytes_per_msa"]), t["summary"]
     out["roofline"] = {"bound": "valu", "achieved": round(ach, 4), "peak": PEAK_INT32_TOPS,
                        "unit": "Tops/s (int32)", "frac": round(ach / PEAK_INT32_TOPS, 5),
-                       "traffic": None,
+                       "achieved_64_ops_per_cell": round(ach64, 4),
+                       "frac_64_ops_per_cell": round(ach64 / PEAK_INT32_TOPS, 5),
+                       "traffic": traffic, "traffic_source": tsrc,
+                       "hbm_gbs": round(traffic / dev_s / 1e9, 1) if traffic else None,
                        "note": "device time of the profile kernels (cw_prf_cols, cw_prf_gapcols, "
                                "cw_prf_score, cw_prf_mm, cw_prf_local) summed over the "
-                               "dependency-level batches"}
+                               "dependency-level batches; achieved = 10 ops per Myers-Miller "
+                               "cell update + 54 per prfScore entry (each computed once); "
+                               "achieved_64_ops_per_cell = 64 ops per cell update; traffic = "
+                               "FETCH_SIZE + WRITE_SIZE of those kernels per cwhip_msa"}
     if cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import oracle as O  # checker/baseline only
@@ -209,7 +243,7 @@ def progressive_stage(cpu: bool):
         same = bool(np.array_equal(order2, r_order) and
                     all(np.array_equal(rows2[i], r_rows[i].astype(np.uint8)) for i in range(len(seqs))))
         out["cpu_baseline"] = {"value": round(st["cells"] / dt, 1), "unit": "Myers-Miller cell updates/s",
-                               "cores": 1, "kind": "port", "seconds": round(dt, 2),
+                               "cores": 1, "kind": "port", "seconds": round(dt, 2), **host_cpu(),
                                "sample": "the whole multiSeqAlign stage (999 profile alignments), 1 thread",
                                "identical_rows": same}
         out["speedup_vs_cpu"] = round(dt / el, 2)
@@ -268,7 +302,7 @@ def ktuple_stage(cpu: bool):
         _, dh = O.fast_pairwise(head, max_aln=2 * int(lens.max()))
         dt = time.perf_counter() - t
         out["cpu_baseline"] = {"value": round(len(dh) / dt, 1), "unit": "pairs/s", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", **host_cpu(),
                                "sample": "the first 180 sequences' %d pairs, %.1f s, one thread" % (len(dh), dt)}
         out["speedup_vs_cpu"] = round(out["pairs_per_s"] / out["cpu_baseline"]["value"], 1)
     return out
@@ -353,6 +387,29 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+VALU_PROBE_OPS = ("add_max", "add", "max", "add3", "max3", "mov_dpp", "subrev", "dp_mix")
+
+
+def valu_probe(trips, waves):
+    """The int32 VALU calibration kernels (libcwcal.so), one launch per op
+    kind and waves-per-CU figure: a known count of wave instructions, timed
+    with HIP events. Their SQ_INSTS_VALU readings calibrate that counter, and
+    the fastest of them is the measured issue ceiling the DP kernels are
+    compared with."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(HERE, "cuda-clustalw_amd", "libcwcal.so"))
+    lib.cwcal_valu.argtypes = [C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 3
+    for op, name in enumerate(VALU_PROBE_OPS):
+        for w in waves:
+            ms, instrs, ops = C.c_double(), C.c_double(), C.c_double()
+            rc = lib.cwcal_valu(op, trips, w, C.byref(ms), C.byref(instrs), C.byref(ops))
+            if rc != 0:
+                raise RuntimeError("cwcal_valu failed (%d)" % rc)
+            print(json.dumps({"valu_probe": {"op": name, "trips": trips, "waves_per_cu": w,
+                                             "ms": round(ms.value, 4), "loop_wave_instrs": instrs.value,
+                                             "lane_ops_per_s": ops.value / (ms.value * 1e-3)}}), flush=True)
+
+
 def serving_all(args):
     """The serving stream at the stated rate, then the same mix at higher rates."""
     out = {"serving": serving_stage(args.serve_rate, args.serve_count)}
@@ -389,9 +446,16 @@ def main():
     ap.add_argument("--max-pairs", type=int, default=0, help="debug: cap the pair count")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--valu-probe", type=int, default=0,
+                    help="only run the int32 VALU calibration kernel for this many loop trips")
+    ap.add_argument("--valu-probe-waves", default="4,8,16,32",
+                    help="waves per CU of the VALU probe launches (comma-separated multiples of 4)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.valu_probe:
+        valu_probe(args.valu_probe, [int(w) for w in args.valu_probe_waves.split(",")])
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only (the pairwise stage would shard by pair lists, "
@@ -462,14 +526,26 @@ def main():
                     "wall_s": round(wall_b, 3)},
     }
     achieved = upd_cells * OPS_PER_CELL / (kms * 1e-3) / 1e12
-    traffic, tsrc = None, None
+    traffic, tsrc, issue = None, None, None
     tfile = os.path.join(HERE, "profiles", "pmc_traffic_%s.json" % args.workload)
     if os.path.exists(tfile) and not args.max_pairs:
         t = json.load(open(tfile))
         traffic, tsrc = int(t["traffic_bytes"]), t["summary"]
+        if "valu_instr_per_cycle_per_simd" in t:
+            # SQ_INSTS_VALU of the same launch, against the calibration probe
+            # that issues the forward loop's opcode mix at the same occupancy
+            issue = {"wave_instr_per_cycle_per_simd": t["valu_instr_per_cycle_per_simd"],
+                     "clock_ghz": t["clock_ghz"],
+                     "lane_slots_per_cell_update": round(t["valu_wave_instrs"] * 64 / upd_cells, 2),
+                     "probe_dp_mix_16_waves": t["probe_dp_mix_instr_per_cycle_per_simd"]["16"],
+                     "frac_of_probe": round(t["valu_instr_per_cycle_per_simd"] /
+                                            t["probe_dp_mix_instr_per_cycle_per_simd"]["16"], 3),
+                     "source": t["calibration"]}
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT32_TOPS,
                        "unit": "Tops/s (int32)", "frac": round(achieved / PEAK_INT32_TOPS, 4),
                        "traffic": traffic, "traffic_source": tsrc,
+                       "hbm_gbs": round(traffic / (kms * 1e-3) / 1e9, 1) if traffic else None,
+                       "valu_issue": issue,
                        "note": "integer DP: %d int32 ops per reference cell update; "
                                "the HBM roof (%.0f GB/s) is not the bound" % (OPS_PER_CELL, PEAK_HBM_GBS)}
 

@@ -29,6 +29,18 @@ def test_library_exports_header_symbols():
     assert set(names) == set(cl.EXPORTS)
 
 
+def test_calibration_library_loads():
+    """The VALU calibration kernels bench.py --valu-probe times live in their
+    own library, and without a device the call reports an error, not a
+    number."""
+    lib = C.CDLL(os.path.join(ROOT, "cuda-clustalw_amd", "libcwcal.so"))
+    lib.cwcal_valu.argtypes = [C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 3
+    ms, instrs, ops = C.c_double(), C.c_double(), C.c_double()
+    assert lib.cwcal_valu(99, 10, 4, C.byref(ms), C.byref(instrs), C.byref(ops)) == -1
+    assert lib.cwcal_valu(0, 10, 6, C.byref(ms), C.byref(instrs), C.byref(ops)) == -1
+    assert lib.cwcal_valu(0, 10, 4, C.byref(ms), C.byref(instrs), C.byref(ops)) < 0
+
+
 def test_binding_return_types():
     """Entry points that return something other than an int status keep their
     ctypes return type (a float read back as int reads 0)."""
