@@ -56,6 +56,10 @@ __device__ __forceinline__ int shr1(int old, int src) {
 __device__ __forceinline__ int shl1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, 0x130, 0xf, 0xf, false);
 }
+// lane l receives src[l+1]; lane 63 receives 0 (one DPP move, no copy of old)
+__device__ __forceinline__ int shl1z(int src) {
+    return __builtin_amdgcn_mov_dpp(src, 0x130, 0xf, 0xf, true);
+}
 __device__ __forceinline__ int wave_max(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
     return v;
@@ -124,6 +128,7 @@ struct PSweep {
     //   reverse: (go2 ? -open2(I(i), jedge) : 0) - (m2[jedge] & (pl1[ib] - pl1[I(i)]))
     int lt0, ltl;
     int go2, jedge;
+    int* err;          // set to 1 on a violated precondition (reported, not trapped)
 };
 
 template <int MODE>
@@ -137,6 +142,7 @@ __device__ __forceinline__ int left_h(const PSweep& W, int i) {
 }
 
 template <bool V> struct Tag { static constexpr bool value = V; };
+template <int V> struct Par { static constexpr int value = V; };
 
 // Two waves sharing one half sweep: stripe j reads the bottom row stripe j-1
 // writes, 64 columns at a time.  `pub` (LDS) counts the chunks a stripe has
@@ -166,38 +172,65 @@ __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 // R-1 rows behind a pass-through row).  The top boundary row comes from inH
 // (inE, or -inf on the first stripe); lane 63's rows leave through outH/outE.
 // DIRECT: R == 1, P < 64, lane P-1 writes its row straight to out.
+//
+// Rows and columns of a half sweep run the same way: MODE_FWD scores row
+// I = ib + i and column J = jb + c, MODE_REV I = ib - i and J = jb - c.  The
+// row gap mask m1 is all ones on every row a half sweep reads (forward rows
+// A+1 .. A+M/2, reverse records A+M/2 .. A+M-1: never record 0 or L1), so the
+// recurrences leave it out (a stripe that finds otherwise sets *W.err); the
+// column mask m2 can be 0 only at the node's last column c = ncols, so the
+// blocks in which every lane stays left of that column leave it out too.
+//
+// At step t lane L works on column c = t - L.  Its column record and its R
+// scores S(I_r, J) are loaded two steps ahead into the buffers of parity t & 1.
+// MODE_REV also needs col[J-1], the record of column c+1: the one the lane
+// loaded for step t+1.
 template <int R, int MODE, bool DIRECT>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
                                         const int* inH, const int* inE, int* outH, int* outE,
                                         StripeSync sy)
 {
+    constexpr int DIR = (MODE == MODE_FWD) ? 1 : -1;
     const int lane = lane_id();
-    const int ncols = W.ncols;
+    // wave-uniform values in scalar registers, so the control of the step
+    // loops (chunk turns, flushes, bounds) runs on the scalar unit
+    const int ncols = uni(W.ncols);
+    const int nrows = uni(W.nrows);
+    P = uni(P);
+    first = uni(first);
+    row0 = uni(row0);
+    const int ib = uni(W.ib), jb = uni(W.jb);
     const int k1 = DIRECT ? P : P - WAVE * (R - 1);
     const bool dummy = (!DIRECT) && (R > 1) && (lane >= k1);
     const bool lane_on = DIRECT ? (lane < P) : true;
     const int d = dummy ? 1 : 0;
     const int firstrow = row0 + 1 + (lane < k1 ? lane * R : k1 * R + (lane - k1) * (R - 1));
     const Prob& p = W.p;
-    const int ldS = p.ldS;
+    const unsigned ldS = (unsigned)uni(p.ldS);
 
-    int gF[R], mF[R], gE[R], lE[R], hL[R], f[R], sI[R];
+    // per row: -gap-open terms, LENCOL, the left column H and F
+    int ngF[R], ngE[R], lE[R], hL[R], f[R];
+    int mall = -1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = firstrow + r - d;
-        const int ii = i < 1 ? 1 : (i > W.nrows ? W.nrows : i);
-        const int I = W.ib + W.id * ii;
+        const int ii = i < 1 ? 1 : (i > nrows ? nrows : i);
+        const int I = ib + DIR * ii;
         const int4 rf = ld4(&p.row[MODE == MODE_FWD ? I : I - 1]);
         const int4 re = ld4(&p.row[I]);
-        gF[r] = rf.x;
-        mF[r] = rf.z;
-        gE[r] = (MODE == MODE_FWD) ? re.w : re.x;
+        ngF[r] = -rf.x;
+        mall &= rf.z;
+        ngE[r] = (MODE == MODE_FWD) ? -re.w : -re.x;
         lE[r] = re.y;
-        hL[r] = left_h<MODE>(W, i < W.nrows ? i : W.nrows);   // idle lanes: any in-range row
+        hL[r] = left_h<MODE>(W, i < nrows ? i : nrows);   // idle lanes: any in-range row
         f[r] = NEG;
-        sI[r] = I;
     }
-    int dprev = left_h<MODE>(W, firstrow - 1 < W.nrows ? firstrow - 1 : W.nrows);
+    if (!__all(mall == -1)) *W.err = 1;                  // same value from every lane
+    int dprev = left_h<MODE>(W, firstrow - 1 < nrows ? firstrow - 1 : nrows);
+    // S row of the lane's first row (rows r follow at I0 + DIR*r; a multi-row
+    // stripe's rows, its pass-through row included, are all in 0 .. nrows)
+    const int i0 = DIRECT ? (firstrow > nrows ? nrows : firstrow) : firstrow - d;
+    const unsigned sI0 = (unsigned)(ib + DIR * i0);
 
     // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
     auto chunk = [&](int ci, int& h, int& e) {
@@ -218,61 +251,62 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 
     int outHv = 0, outEv = NEG, coutH = 0, coutE = 0;
 
-    // column records and scores of this lane's column, one step ahead
-    auto colJ = [&](int c) {
-        const int cc = c < 1 ? 1 : (c > ncols ? ncols : c);
-        return W.jb + W.jd * cc;
-    };
-    int4 nrecF, nrecE;
-    int nsv[R];
-    auto prefetch = [&](int c) {
-        const int J = colJ(c);
-        nrecF = ld4(&p.col[J]);
-        if constexpr (MODE == MODE_FWD) nrecE = nrecF;
-        else nrecE = ld4(&p.col[J - 1]);
-        gcint* Sc = p.S + (int64_t)J * ldS;
+    // the column record and scores of step t into rec/sv (columns clamped to
+    // 0 .. ncols+1, which keeps J and every S index inside the pair's arrays)
+    int4 rec[2];
+    int sv[2][R];
+    auto load = [&](int t, int4& rc, int (&sc)[R]) {
+        int c = t - lane;
+        c = c < 0 ? 0 : (c > ncols + 1 ? ncols + 1 : c);
+        const int J = jb + DIR * c;
+        rc = ld4(&p.col[J]);
+        gcint* Sc = p.S + (__umul24((unsigned)J, ldS) + sI0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) nsv[r] = Sc[sI[r]];
+        for (int r = 0; r < R; ++r) sc[r] = Sc[DIR * r];
     };
-    prefetch(1 - lane);
+    load(1, rec[1], sv[1]);
+    load(2, rec[0], sv[0]);
 
-    auto step = [&](int t, auto masked) {
-        if (((t - 1) & 63) == 0 && t > 1) {
-            const int ci = (t - 1) >> 6;
-            cinH = nxtH;
-            cinE = nxtE;
-            if (ci + 1 < nchunks) chunk(ci + 1, nxtH, nxtE);
-        }
-        const int nH = shl1(cinH, cinH);                // chunk rotation for the next step
-        const int nE = shl1(cinE, cinE);
+    // par: t & 1.  masked: some lane may be off its columns, or on the last
+    // one (exec masking and the column mask).  flush: this step may end an
+    // output chunk (every 64th output column, and the last one).
+    auto step = [&](int t, auto par, auto masked, auto flush) {
+        constexpr int Q = decltype(par)::value;
+        constexpr bool MK = decltype(masked)::value;
+        const int nH = shl1z(cinH);                     // chunk rotation for the next step (lane 63's
+        const int nE = shl1z(cinE);                     // element is past the chunk's last use)
         const int inHv = shr1(cinH, outHv);             // lane 0: chunk element; others: lane above
         const int inEv = shr1(cinE, outEv);
         cinH = nH;
         cinE = nE;
-        const int4 cF = nrecF, cE = nrecE;
-        int s[R];
+        const int4 cF = rec[Q];
+        const int4 cE = (MODE == MODE_FWD) ? rec[Q] : rec[Q ^ 1];
+        int sc[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = nsv[r];
-        prefetch(t + 1 - lane);
+        for (int r = 0; r < R; ++r) sc[r] = sv[Q][r];
+        load(t + 2, rec[Q], sv[Q]);
         const int c = t - lane;
         bool act = true;
-        if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= ncols;
+        if constexpr (MK) act = lane_on && c >= 1 && c <= ncols;
         if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
             dprev = inHv;
+            const int nFw = -cF.w, nFx = -cF.x, nEx = -cE.x;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 int h, e;
-                if (MODE == MODE_FWD) {
-                    const int fn = max(hL[r] - ((gF[r] + cF.w) & mF[r]), f[r] - (cF.y & mF[r]));
-                    const int en = max(hup - ((gE[r] + cE.x) & cE.z), eup - (lE[r] & cE.z));
-                    h = max3(dg + s[r], fn, en);
+                if constexpr (MODE == MODE_FWD) {
+                    const int fn = max(hL[r] + ngF[r] + nFw, f[r] - cF.y);
+                    const int en = MK ? max(hup - ((cE.x - ngE[r]) & cE.z), eup - (lE[r] & cE.z))
+                                      : max(hup + ngE[r] + nEx, eup - lE[r]);
+                    h = max3(dg + sc[r], fn, en);
                     f[r] = fn;
                     e = en;
                 } else {
-                    const int g = max(hL[r], f[r]) - (cF.y & mF[r]);
-                    const int k = max(hup, eup) - (lE[r] & cE.z);
-                    h = max3(dg + s[r], g - ((gF[r] + cF.x) & mF[r]), k - ((gE[r] + cE.x) & cE.z));
+                    const int g = max(hL[r], f[r]) - cF.y;
+                    const int k = MK ? max(hup, eup) - (lE[r] & cE.z) : max(hup, eup) - lE[r];
+                    const int ke = MK ? k - ((cE.x - ngE[r]) & cE.z) : k + ngE[r] + nEx;
+                    h = max3(dg + sc[r], g + ngF[r] + nFx, ke);
                     f[r] = g;
                     e = k;
                 }
@@ -295,7 +329,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
             coutH = shl1(outHv, coutH);
             coutE = shl1(outEv, coutE);
             const int oc = t - 63;
-            if (oc >= 1 && (((oc & 63) == 0) || oc == ncols)) {
+            if (decltype(flush)::value && oc >= 1 && (((oc & 63) == 0) || oc == ncols)) {
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
@@ -310,16 +344,37 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
         }
     };
 
-    if (DIRECT) {
-        const int tend = ncols + P - 1;
-        for (int t = 1; t <= tend; ++t) step(t, Tag<true>{});
-    } else {
-        const int tend = ncols + 63;
-        const int up_end = ncols < 63 ? ncols + 63 : 63;
-        int t = 1;
-        for (; t <= up_end; ++t) step(t, Tag<true>{});
-        for (; t <= ncols; ++t) step(t, Tag<false>{});
-        for (; t <= tend; ++t) step(t, Tag<true>{});
+    // Steps run in blocks of 64: block b covers t = 64b+1 .. 64b+64 (t odd at
+    // its start, so steps alternate buffer 1, 0), the lane-0 input chunk turns
+    // at its start.  A block in which every lane stays on columns 1 .. ncols-1
+    // (t in [64, ncols-1]) runs without exec masking or column mask and flushes
+    // its output chunk only after its step 64b+63 (output column 64b).
+    const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
+    for (int b = 0; 64 * b + 1 <= tend; ++b) {
+        const int t0 = 64 * b + 1;
+        if (b > 0) {
+            cinH = nxtH;
+            cinE = nxtE;
+            if (b + 1 < nchunks) chunk(b + 1, nxtH, nxtE);
+        }
+        const int t1 = t0 + 63 < tend ? t0 + 63 : tend;
+        if (!DIRECT && t0 >= 64 && t0 + 63 < ncols) {
+#pragma unroll 1
+            for (int k = 0; k < 62; k += 2) {
+                step(t0 + k, Par<1>{}, Tag<false>{}, Tag<false>{});
+                step(t0 + k + 1, Par<0>{}, Tag<false>{}, Tag<false>{});
+            }
+            step(t0 + 62, Par<1>{}, Tag<false>{}, Tag<true>{});
+            step(t0 + 63, Par<0>{}, Tag<false>{}, Tag<false>{});
+        } else {
+            int t = t0;
+#pragma unroll 1
+            for (; t + 1 <= t1; t += 2) {
+                step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
+                step(t + 1, Par<0>{}, Tag<true>{}, Tag<true>{});
+            }
+            if (t <= t1) step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
+        }
     }
 }
 
@@ -520,6 +575,7 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
         F.ltl = -ext2(p, A + 1, B);
         F.go2 = 0; F.jedge = 0;
+        F.err = err;
         if (S > 1) {
             if (m == 0) fill_top<MODE_FWD>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_FWD>(F, rmax, m, S, regions, rowcap, lv.fH, lv.fE, prog, err);
@@ -534,6 +590,7 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
         Rv.lt0 = 0; Rv.ltl = 0;
         Rv.go2 = go2; Rv.jedge = B + N;
+        Rv.err = err;
         if (S > 1) {
             if (m == 0) fill_top<MODE_REV>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_REV>(Rv, rmax, m, S, regions, rowcap, lv.rH, lv.rK, prog, err);
