@@ -72,6 +72,21 @@ __device__ __forceinline__ void wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
 __device__ __forceinline__ int max3(int a, int b, int c) { return max(max(a, b), c); }
+// a + b + c as one v_add3_u32 (the compiler's reassociation would otherwise
+// split it into two adds)
+__device__ __forceinline__ int add3(int a, int b, int c) {
+    int r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// -x, computed no earlier than `after` exists: keeps the scheduler from
+// hoisting the negation of a prefetched value (and with it the wait for the
+// load) above the step that uses it
+__device__ __forceinline__ int neg_after(int x, int after) {
+    int r;
+    asm("v_sub_u32 %0, 0, %1" : "=v"(r) : "v"(x), "v"(after));
+    return r;
+}
 
 // LDS address space: reads through these compile to ds_read_* rather than flat loads
 #define CWP_LDS __attribute__((address_space(3)))
@@ -128,7 +143,7 @@ struct PSweep {
     //   reverse: (go2 ? -open2(I(i), jedge) : 0) - (m2[jedge] & (pl1[ib] - pl1[I(i)]))
     int lt0, ltl;
     int go2, jedge;
-    int* err;          // set to 1 on a violated precondition (reported, not trapped)
+    lint* err;         // set to 1 on a violated precondition (reported, not trapped)
 };
 
 template <int MODE>
@@ -150,18 +165,18 @@ template <int V> struct Par { static constexpr int value = V; };
 // chunk.  A wait that outlasts ~2^24 sleeps sets *err and goes on, so a
 // mistake shows up as a reported error, not a hang.
 struct StripeSync {
-    const int* wait;
-    int* pub;
-    int* err;
+    const int* wait;   // generic pointers, tested against null; the counters
+    int* pub;          // and the error flag are in LDS (offset 0 is a valid
+    int* err;          // LDS address) and are accessed as such
 };
 
 __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 {
     int spins = 0;
-    while (uni(*(const volatile int*)sy.wait) < need) {
+    while (uni(*(const volatile lint*)sy.wait) < need) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1 << 24)) {
-            *sy.err = 1;                                 // same value from every lane
+            *(lint*)sy.err = 1;                          // same value from every lane
             break;
         }
     }
@@ -182,9 +197,9 @@ __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 // blocks in which every lane stays left of that column leave it out too.
 //
 // At step t lane L works on column c = t - L.  Its column record and its R
-// scores S(I_r, J) are loaded two steps ahead into the buffers of parity t & 1.
-// MODE_REV also needs col[J-1], the record of column c+1: the one the lane
-// loaded for step t+1.
+// scores S(I_r, J) are loaded four steps ahead into buffer t & 3, which step
+// t-4 refilled after its own use.  MODE_REV also needs col[J-1], the record
+// of column c+1: the one the lane loaded for step t+1.
 template <int R, int MODE, bool DIRECT>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
                                         const int* inH, const int* inE, int* outH, int* outE,
@@ -251,23 +266,38 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 
     int outHv = 0, outEv = NEG, coutH = 0, coutE = 0;
 
-    // the column record and scores of step t into rec/sv (columns clamped to
-    // 0 .. ncols+1, which keeps J and every S index inside the pair's arrays)
-    int4 rec[2];
-    int sv[2][R];
-    auto load = [&](int t, int4& rc, int (&sc)[R]) {
-        int c = t - lane;
-        c = c < 0 ? 0 : (c > ncols + 1 ? ncols + 1 : c);
-        const int J = jb + DIR * c;
-        rc = ld4(&p.col[J]);
-        gcint* Sc = p.S + (__umul24((unsigned)J, ldS) + sI0);
+    // The column record and scores of step t go into rec/sv[t & 3], read
+    // through pointers cp/sp[t & 3] to the record and to S(I0, J).  Columns
+    // are clamped to 0 .. ncols+1, which keeps J and every S index inside the
+    // pair's arrays.  Within a block whose lanes all stay on columns
+    // 1 .. ncols-1 and that starts at t0 with t0 + 66 <= ncols, no load is
+    // clamped (t+4 - lane <= ncols+1), so a pointer moves on by four columns
+    // per use.
+    int4 rec[4];
+    int sv[4][R];
+    gcint4* cp[4];
+    gcint* sp[4];
+    auto load = [&](int t, auto par, auto from_scratch) {
+        constexpr int Q = decltype(par)::value;
+        if constexpr (decltype(from_scratch)::value) {
+            const int c = min(max(t - lane, 0), ncols + 1);
+            const int J = jb + DIR * c;
+            cp[Q] = p.col + J;
+            sp[Q] = p.S + (__umul24((unsigned)J, ldS) + sI0);
+        } else {
+            cp[Q] += 4 * DIR;
+            sp[Q] += 4 * DIR * (int)ldS;
+        }
+        rec[Q] = ld4(cp[Q]);
 #pragma unroll
-        for (int r = 0; r < R; ++r) sc[r] = Sc[DIR * r];
+        for (int r = 0; r < R; ++r) sv[Q][r] = sp[Q][DIR * r];
     };
-    load(1, rec[1], sv[1]);
-    load(2, rec[0], sv[0]);
+    load(1, Par<1>{}, Tag<true>{});
+    load(2, Par<2>{}, Tag<true>{});
+    load(3, Par<3>{}, Tag<true>{});
+    load(4, Par<0>{}, Tag<true>{});
 
-    // par: t & 1.  masked: some lane may be off its columns, or on the last
+    // par: t & 3.  masked: some lane may be off its columns, or on the last
     // one (exec masking and the column mask).  flush: this step may end an
     // output chunk (every 64th output column, and the last one).
     auto step = [&](int t, auto par, auto masked, auto flush) {
@@ -280,33 +310,31 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
         cinH = nH;
         cinE = nE;
         const int4 cF = rec[Q];
-        const int4 cE = (MODE == MODE_FWD) ? rec[Q] : rec[Q ^ 1];
-        int sc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) sc[r] = sv[Q][r];
-        load(t + 2, rec[Q], sv[Q]);
+        const int4 cE = (MODE == MODE_FWD) ? rec[Q] : rec[(Q + 1) & 3];
+        const int* const sc = sv[Q];
         const int c = t - lane;
         bool act = true;
         if constexpr (MK) act = lane_on && c >= 1 && c <= ncols;
         if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
             dprev = inHv;
-            const int nFw = -cF.w, nFx = -cF.x, nEx = -cE.x;
+            const int nFw = neg_after(cF.w, inHv), nFx = neg_after(cF.x, inHv);
+            const int nEx = neg_after(cE.x, inHv);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 int h, e;
                 if constexpr (MODE == MODE_FWD) {
-                    const int fn = max(hL[r] + ngF[r] + nFw, f[r] - cF.y);
+                    const int fn = max(add3(hL[r], ngF[r], nFw), f[r] - cF.y);
                     const int en = MK ? max(hup - ((cE.x - ngE[r]) & cE.z), eup - (lE[r] & cE.z))
-                                      : max(hup + ngE[r] + nEx, eup - lE[r]);
+                                      : max(add3(hup, ngE[r], nEx), eup - lE[r]);
                     h = max3(dg + sc[r], fn, en);
                     f[r] = fn;
                     e = en;
                 } else {
                     const int g = max(hL[r], f[r]) - cF.y;
                     const int k = MK ? max(hup, eup) - (lE[r] & cE.z) : max(hup, eup) - lE[r];
-                    const int ke = MK ? k - ((cE.x - ngE[r]) & cE.z) : k + ngE[r] + nEx;
-                    h = max3(dg + sc[r], g + ngF[r] + nFx, ke);
+                    const int ke = MK ? k - ((cE.x - ngE[r]) & cE.z) : add3(k, ngE[r], nEx);
+                    h = max3(dg + sc[r], add3(g, ngF[r], nFx), ke);
                     f[r] = g;
                     e = k;
                 }
@@ -325,6 +353,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                 ((gint*)outE)[c] = eup;
             }
         }
+        load(t + 4, par, Tag<MK>{});                   // buffer Q is free again
         if (!DIRECT) {
             coutH = shl1(outHv, coutH);
             coutE = shl1(outEv, coutE);
@@ -338,16 +367,17 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                 }
                 if (sy.pub) {
                     wave_fence();
-                    *(volatile int*)sy.pub = (oc + 63) >> 6;    // same value from every lane
+                    *(volatile lint*)sy.pub = (oc + 63) >> 6;   // same value from every lane
                 }
             }
         }
     };
 
-    // Steps run in blocks of 64: block b covers t = 64b+1 .. 64b+64 (t odd at
-    // its start, so steps alternate buffer 1, 0), the lane-0 input chunk turns
-    // at its start.  A block in which every lane stays on columns 1 .. ncols-1
-    // (t in [64, ncols-1]) runs without exec masking or column mask and flushes
+    // Steps run in blocks of 64: block b covers t = 64b+1 .. 64b+64 (t = 1
+    // mod 4 at its start, so steps take buffers 1, 2, 3, 0 in turn), the
+    // lane-0 input chunk turns at its start.  A block in which every lane
+    // stays on columns 1 .. ncols-1 and no load is clamped (t0 >= 64,
+    // t0 + 66 <= ncols) runs without exec masking or column mask and flushes
     // its output chunk only after its step 64b+63 (output column 64b).
     const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
     for (int b = 0; 64 * b + 1 <= tend; ++b) {
@@ -358,32 +388,43 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
             if (b + 1 < nchunks) chunk(b + 1, nxtH, nxtE);
         }
         const int t1 = t0 + 63 < tend ? t0 + 63 : tend;
-        if (!DIRECT && t0 >= 64 && t0 + 63 < ncols) {
+        if (!DIRECT && t0 >= 64 && t0 + 66 <= ncols) {
 #pragma unroll 1
-            for (int k = 0; k < 62; k += 2) {
+            for (int k = 0; k < 60; k += 4) {
                 step(t0 + k, Par<1>{}, Tag<false>{}, Tag<false>{});
-                step(t0 + k + 1, Par<0>{}, Tag<false>{}, Tag<false>{});
+                step(t0 + k + 1, Par<2>{}, Tag<false>{}, Tag<false>{});
+                step(t0 + k + 2, Par<3>{}, Tag<false>{}, Tag<false>{});
+                step(t0 + k + 3, Par<0>{}, Tag<false>{}, Tag<false>{});
             }
-            step(t0 + 62, Par<1>{}, Tag<false>{}, Tag<true>{});
+            step(t0 + 60, Par<1>{}, Tag<false>{}, Tag<false>{});
+            step(t0 + 61, Par<2>{}, Tag<false>{}, Tag<false>{});
+            step(t0 + 62, Par<3>{}, Tag<false>{}, Tag<true>{});
             step(t0 + 63, Par<0>{}, Tag<false>{}, Tag<false>{});
         } else {
             int t = t0;
 #pragma unroll 1
-            for (; t + 1 <= t1; t += 2) {
+            for (; t + 3 <= t1; t += 4) {
                 step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
-                step(t + 1, Par<0>{}, Tag<true>{}, Tag<true>{});
+                step(t + 1, Par<2>{}, Tag<true>{}, Tag<true>{});
+                step(t + 2, Par<3>{}, Tag<true>{}, Tag<true>{});
+                step(t + 3, Par<0>{}, Tag<true>{}, Tag<true>{});
             }
             if (t <= t1) step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
+            if (t + 1 <= t1) step(t + 1, Par<2>{}, Tag<true>{}, Tag<true>{});
+            if (t + 2 <= t1) step(t + 2, Par<3>{}, Tag<true>{}, Tag<true>{});
         }
     }
 }
 
-template <int R, int MODE>
+// Every stripe runs out of line: its loads' waits are then placed from the
+// stripe's own accesses alone (all global or LDS; a flat access pending in
+// the caller would make every wait for a load a wait for all of them).
+template <int R, int MODE, bool DIRECT = false>
 __device__ __attribute__((noinline)) void pstripe_call(const PSweep W, int row0, int P, int first,
                                                        const int* inH, const int* inE, int* outH,
                                                        int* outE, StripeSync sy)
 {
-    pstripe<R, MODE, false>(W, row0, P, first, inH, inE, outH, outE, sy);
+    pstripe<R, MODE, DIRECT>(W, row0, P, first, inH, inE, outH, outE, sy);
 }
 
 template <int MODE>
@@ -391,9 +432,9 @@ __device__ void run_pstripe(int R, bool direct, const PSweep& W, int row0, int P
                             const int* inH, const int* inE, int* outH, int* outE,
                             StripeSync sy = StripeSync{nullptr, nullptr, nullptr})
 {
-    if (direct) { pstripe<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE, sy); return; }
+    if (direct) { pstripe_call<1, MODE, true>(W, row0, P, first, inH, inE, outH, outE, sy); return; }
     switch (R) {
-    case 1: pstripe<1, MODE, false>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 1: pstripe_call<1, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
     case 2: pstripe_call<2, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
     case 3: pstripe_call<3, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
     case 4: pstripe_call<4, MODE>(W, row0, P, first, inH, inE, outH, outE, sy); break;
@@ -575,7 +616,7 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
         F.ltl = -ext2(p, A + 1, B);
         F.go2 = 0; F.jedge = 0;
-        F.err = err;
+        F.err = (lint*)err;
         if (S > 1) {
             if (m == 0) fill_top<MODE_FWD>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_FWD>(F, rmax, m, S, regions, rowcap, lv.fH, lv.fE, prog, err);
@@ -590,7 +631,7 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
         Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
         Rv.lt0 = 0; Rv.ltl = 0;
         Rv.go2 = go2; Rv.jedge = B + N;
-        Rv.err = err;
+        Rv.err = (lint*)err;
         if (S > 1) {
             if (m == 0) fill_top<MODE_REV>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_REV>(Rv, rmax, m, S, regions, rowcap, lv.rH, lv.rK, prog, err);
