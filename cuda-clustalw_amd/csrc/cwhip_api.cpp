@@ -201,11 +201,15 @@ int cwhip_load_sequences(cwhip_ctx* ctx, const uint8_t* codes, const int64_t* of
     HIPCHK(ctx->d_codes.ensure(total + 64));
     HIPCHK(ctx->d_offs.ensure(nseq + 1));
     HIPCHK(ctx->d_lens.ensure(nseq + 1));
+    HIPCHK(ctx->d_gapfree.ensure(nseq + 1));
     if (total) HIPCHK(hipMemcpyAsync(ctx->d_codes.p, codes, total, hipMemcpyHostToDevice, ctx->stream));
     if (nseq) {
         HIPCHK(hipMemcpyAsync(ctx->d_offs.p, offsets, nseq * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->d_lens.p, lens, nseq * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->d_gapfree.p, ctx->h_gapfree.data(), nseq * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
     }
+    ctx->counts_ready = false;                    // distances of earlier sequences are stale
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CWHIP_OK;
 }
@@ -249,6 +253,7 @@ int cwhip_stage_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t*
     }
     HIPCHK(hipSetDevice(ctx->device));
     ctx->npairs = npairs;
+    ctx->counts_ready = false;
     ctx->h_pairs.assign(pairs, pairs + 2 * npairs);
     ctx->gap1 = p->gap_pos1;
     ctx->gap2 = p->gap_pos2;
@@ -419,6 +424,7 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     HIPCHK(hipMemcpyAsync(&st, ctx->d_status.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (st) return fail(ctx, CWHIP_E_KERNEL, "kernel status " + std::to_string(st));
+    ctx->counts_ready = true;
     return CWHIP_OK;
 }
 
@@ -543,6 +549,7 @@ int cwhip_autotune(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t* pa
         if (rc) return rc;
     } else {
         ctx->npairs = 0;
+        ctx->counts_ready = false;
         ctx->h_pairs.clear();
     }
     return CWHIP_OK;
@@ -591,6 +598,130 @@ int cwhip_pairwise_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32
             out_dist[k] = cnt[k] < 0 ? 1.0f : cwhip_distance(cnt[k], ctx->h_gapfree[si], ctx->h_gapfree[sj]);
     }
     return CWHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-joining on the device.
+
+// the tree text into out (cwhip_nj_tree's sizing rules)
+static int put_text(const std::string& s, char* out, int64_t cap, int64_t* out_len)
+{
+    *out_len = (int64_t)s.size();
+    if (!out || cap < (int64_t)s.size() + 1) return CWHIP_E_ARG;
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    return CWHIP_OK;
+}
+
+// the DistMatrix of the last run, on the host, as FullPairwiseAlign leaves it
+static int host_matrix_of_run(cwhip_ctx* ctx, std::vector<double>& D)
+{
+    const int n = ctx->nseq;
+    std::vector<int32_t> cnt(ctx->npairs);
+    const int rc = cwhip_fetch_counts(ctx, cnt.data(), ctx->npairs);
+    if (rc) return rc;
+    D.assign((size_t)(n + 1) * (n + 1), 0.0);
+    for (int64_t k = 0; k < ctx->npairs; ++k) {
+        const int si = ctx->h_pairs[2 * k], sj = ctx->h_pairs[2 * k + 1];
+        const float d = cnt[k] < 0 ? 1.0f : cwhip_distance(cnt[k], ctx->h_gapfree[si], ctx->h_gapfree[sj]);
+        D[(size_t)(si + 1) * (n + 1) + sj + 1] = D[(size_t)(sj + 1) * (n + 1) + si + 1] = (double)d;
+    }
+    return CWHIP_OK;
+}
+
+// cw_nj over D (built from the run's counts when `from_run`, else uploaded
+// from `dist`); the text from the kernel's treeDesc and branch lengths.
+// Returns 1 when the kernel left the tree to the host.
+static int nj_device(cwhip_ctx* ctx, int n, bool from_run, const double* dist, const char* const* names,
+                     std::string& text)
+{
+    const int64_t ld = n + 1;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(ctx->t_dist.ensure((size_t)ld * ld));
+    HIPCHK(ctx->t_desc.ensure((size_t)ld * ld));
+    HIPCHK(ctx->t_branch.ensure(2 * (size_t)(n + 2)));
+    HIPCHK(ctx->t_status.ensure(1));
+    NjArgs a{};
+    if (from_run) {
+        a.pairs = ctx->d_pairs.p;
+        a.count = ctx->d_count.p;
+        a.gapfree = ctx->d_gapfree.p;
+        a.npairs = ctx->npairs;
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->t_dist.p, dist, (size_t)ld * ld * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+    a.D = ctx->t_dist.p;
+    a.desc = ctx->t_desc.p;
+    a.left = ctx->t_branch.p;
+    a.right = ctx->t_branch.p + (n + 2);
+    a.status = ctx->t_status.p;
+    a.n = n;
+    HIPCHK(cw_launch_nj(a, ctx->stream));
+    std::vector<int32_t> desc((size_t)ld * ld);
+    std::vector<double> br(2 * (size_t)(n + 2));
+    int32_t st = 0;
+    HIPCHK(hipMemcpyAsync(desc.data(), ctx->t_desc.p, desc.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(br.data(), ctx->t_branch.p, br.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(&st, ctx->t_status.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (st) return 1;
+    std::vector<double> left(br.begin(), br.begin() + (n + 2)), right(br.begin() + (n + 2), br.end());
+    text = cw_phylip_text(n, desc, left, right, names);
+    return CWHIP_OK;
+}
+
+int cwhip_nj_tree_from_run(cwhip_ctx* ctx, const char* const* names, char* out, int64_t cap, int64_t* out_len)
+{
+    if (!ctx || !names || !out_len) return fail(ctx, CWHIP_E_ARG, "null argument");
+    const int n = ctx->nseq;
+    for (int i = 0; i < n; ++i)
+        if (!names[i]) return fail(ctx, CWHIP_E_ARG, "null name");
+    if (n < 2) return fail(ctx, CWHIP_E_ARG, "a tree needs two sequences");
+    // the run must have covered every pair of the loaded sequences
+    if (!ctx->counts_ready || ctx->npairs != (int64_t)n * (n - 1) / 2)
+        return fail(ctx, CWHIP_E_ARG, "no finished run over all pairs of the loaded sequences");
+    std::vector<bool> seen((size_t)n * n, false);
+    for (int64_t k = 0; k < ctx->npairs; ++k) {
+        int si = ctx->h_pairs[2 * k], sj = ctx->h_pairs[2 * k + 1];
+        if (si > sj) std::swap(si, sj);
+        if (si == sj || seen[(size_t)si * n + sj])
+            return fail(ctx, CWHIP_E_ARG, "no finished run over all pairs of the loaded sequences");
+        seen[(size_t)si * n + sj] = true;
+    }
+    if (n >= 4 && n <= CW_NJ_MAX_SEQS) {
+        std::string s;
+        const int rc = nj_device(ctx, n, true, nullptr, names, s);
+        if (rc == CWHIP_OK) return put_text(s, out, cap, out_len);
+        if (rc != 1) return rc;
+    }
+    std::vector<double> D;                          // two or three sequences, or past the kernel's size
+    const int rc = host_matrix_of_run(ctx, D);
+    if (rc) return rc;
+    return cwhip_nj_tree(n, D.data(), names, out, cap, out_len);
+}
+
+int cwhip_nj_tree_on_device(cwhip_ctx* ctx, int nseq, const double* dist, const char* const* names, char* out,
+                            int64_t cap, int64_t* out_len)
+{
+    if (!ctx || nseq < 2 || !dist || !names || !out_len) return fail(ctx, CWHIP_E_ARG, "bad tree arguments");
+    for (int i = 0; i < nseq; ++i)
+        if (!names[i]) return fail(ctx, CWHIP_E_ARG, "null name");
+    // the kernel reads D(jj, ii) for D(ii, jj): a matrix that is not symmetric
+    // goes to the host code, which reads it as the reference does
+    bool sym = true;
+    const int64_t ld = nseq + 1;
+    for (int i = 1; i <= nseq && sym; ++i)
+        for (int j = i + 1; j <= nseq; ++j)
+            if (dist[i * ld + j] != dist[j * ld + i]) { sym = false; break; }
+    if (sym && nseq >= 4 && nseq <= CW_NJ_MAX_SEQS) {
+        std::string s;
+        const int rc = nj_device(ctx, nseq, false, dist, names, s);
+        if (rc == CWHIP_OK) return put_text(s, out, cap, out_len);
+        if (rc != 1) return rc;
+    }
+    return cwhip_nj_tree(nseq, dist, names, out, cap, out_len);
 }
 
 int cwhip_pairwise_range(cwhip_ctx* ctx, const cwhip_pair_params* p, int i_start, int i_end,

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/cwhip.h"
+#include "cw_nj_kernels.h"
 #include "cw_pairwise_kernels.h"
 #include "cw_profile_kernels.h"
 
@@ -103,9 +104,10 @@ struct cwhip_ctx {
     std::vector<uint8_t> h_codes;
     DevBuf<uint8_t> d_codes;
     DevBuf<int64_t> d_offs;
-    DevBuf<int32_t> d_lens;
-    // staged work
+    DevBuf<int32_t> d_lens, d_gapfree;
+    // staged work; counts_ready: d_count holds the finished run of the staged pairs
     int64_t npairs = 0;
+    bool counts_ready = false;
     DevBuf<int32_t> d_pairs, d_order, d_count, d_info, d_matrix, d_go, d_status, d_scratch;
     DevBuf<unsigned int> d_queue;
     std::vector<int32_t> h_pairs;
@@ -127,6 +129,9 @@ struct cwhip_ctx {
     DevBuf<double> f_dist;
     DevBuf<unsigned int> f_ctr;
     float fast_ms = 0.f;
+    DevBuf<double> t_dist;                      // neighbour-joining: the (n+1)^2 DistMatrix,
+    DevBuf<int32_t> t_desc, t_status;           // treeDesc and the kernel's status,
+    DevBuf<double> t_branch;                    // leftBranch then rightBranch
     DevBuf<uint8_t> p_rows;                     // group rows of a profile batch
     DevBuf<PrfSide> p_sides;                    // its profiles to build
     PinBuf<uint8_t> h_rows;                     // staging of the rows
@@ -142,10 +147,10 @@ struct cwhip_ctx {
     template <typename F>
     void for_each_buf(F f)
     {
-        f(d_codes); f(d_offs); f(d_lens); f(d_pairs); f(d_order); f(d_count); f(d_info);
+        f(d_codes); f(d_offs); f(d_lens); f(d_gapfree); f(d_pairs); f(d_order); f(d_count); f(d_info);
         f(d_matrix); f(d_go); f(d_status); f(d_scratch); f(d_queue); f(d_bins);
         f(p_pool); f(p_out); f(p_tiles); f(p_probs); f(f_scratch); f(f_pairs); f(f_dist);
-        f(f_ctr); f(p_rows); f(p_sides);
+        f(f_ctr); f(t_dist); f(t_desc); f(t_status); f(t_branch); f(p_rows); f(p_sides);
     }
     void bind_account() { for_each_buf([this](auto& b) { b.acct = &mem; }); }
     // frees every device buffer (a later call allocates again); the loaded
@@ -155,6 +160,7 @@ struct cwhip_ctx {
         for_each_buf([](auto& b) { b.release(); });
         nseq = 0;
         npairs = 0;
+        counts_ready = false;
         h_pairs.clear();
         if (gexec) {
             (void)hipGraphExecDestroy(gexec);
@@ -164,6 +170,12 @@ struct cwhip_ctx {
         glast.clear();
     }
 };
+
+// The PHYLIP text of a neighbour-joining result (ClusterTreeOutput::
+// printPhylipTree and twoWaySplit) from its treeDesc ((n+1)^2, consumed) and
+// branch lengths (n+2 each); names[0 .. n-1].  Defined in cwhip_tree.cpp.
+std::string cw_phylip_text(int n, std::vector<int32_t>& desc, const std::vector<double>& left,
+                           const std::vector<double>& right, const char* const* names);
 
 static inline int fail(cwhip_ctx* c, int code, const std::string& msg)
 {

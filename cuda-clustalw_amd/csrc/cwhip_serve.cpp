@@ -14,7 +14,10 @@
 // A job whose estimate turns out too small is run again with twice the
 // reservation (up to the budget).  Jobs on different slots run on different
 // streams, so their kernels overlap on the device.  A job's sequences are
-// uploaded once to its slot and stay resident for all of its stages.  Every
+// uploaded once to its slot and stay resident for all of its stages; an
+// alignment job's guide tree is joined on the device from the distances its
+// pairwise stage left in the slot's memory (only the tree text comes back
+// before the progressive stage).  Every
 // result is computed by the same entry points a caller uses directly
 // (cwhip_pairwise_range, cwhip_nj_tree, cwhip_guide_from_tree, cwhip_msa,
 // cwhip_clustal_text), so it is the same alone and in any mix.
@@ -61,7 +64,8 @@ struct Job {
 // batched kernel's per-wave scratch (8 rows of max length per wave, at most
 // one wave per pair), and for an alignment the profile-DP pool of its widest
 // level (score matrix, records and scratch of up to nseq/2 pairs of profiles
-// of about 1.25 x the longest sequence).
+// of about 1.25 x the longest sequence) and the device guide tree's DistMatrix
+// and treeDesc (12 bytes per entry, 16 counted).
 int64_t footprint(const Job& j, int num_cu)
 {
     int64_t total = 0, lmax = 0;
@@ -133,23 +137,18 @@ int run_job(cwhip_ctx* ctx, Job& j)
     for (int64_t k = 0; k < np; ++k)
         j.cells += (double)j.lens[pairs[2 * k]] * j.lens[pairs[2 * k + 1]];
     if (j.kind != CWHIP_JOB_ALIGNMENT || n < 2) return CWHIP_OK;
-    // DistMatrix as FullPairwiseAlign leaves it (diagonal 0, both SetAt's)
-    std::vector<double> D((size_t)(n + 1) * (n + 1), 0.0);
-    for (int64_t k = 0; k < np; ++k) {
-        const int a = pairs[2 * k] + 1, b = pairs[2 * k + 1] + 1;
-        D[(size_t)a * (n + 1) + b] = D[(size_t)b * (n + 1) + a] = (double)j.dist[k];
-    }
     std::vector<const char*> nm(n);
     for (int i = 0; i < n; ++i) nm[i] = j.names[i].c_str();
-    // one tree build: room for every name and two branch lengths per node,
-    // a second call only if the text did not fit
+    // the guide tree from the distances still in the slot's device memory;
+    // one tree build: room for every name and two branch lengths per node, a
+    // second call only if the text did not fit
     int64_t len = 0, cap = 80 * (int64_t)(n + 2) + 64;
     for (int i = 0; i < n; ++i) cap += (int64_t)j.names[i].size();
     j.dnd.assign(cap, '\0');
-    rc = cwhip_nj_tree(n, D.data(), nm.data(), &j.dnd[0], cap, &len);
+    rc = cwhip_nj_tree_from_run(ctx, nm.data(), &j.dnd[0], cap, &len);
     if (rc && len + 1 > cap) {
         j.dnd.assign(len + 1, '\0');
-        rc = cwhip_nj_tree(n, D.data(), nm.data(), &j.dnd[0], len + 1, &len);
+        rc = cwhip_nj_tree_from_run(ctx, nm.data(), &j.dnd[0], len + 1, &len);
     }
     if (rc) return rc;
     j.dnd.resize(len);

@@ -320,6 +320,27 @@ int two_way_split(NJ& t, std::string& s, const std::vector<std::string>& names, 
     return startRow;
 }
 
+// ClusterTreeOutput::printPhylipTree (:45-99): the three subtrees of the
+// final trichotomy
+std::string phylip(NJ& t, const std::vector<std::string>& nm)
+{
+    const int n = t.n;
+    std::string s = "(\n";
+    two_way_split(t, s, nm, n - 2, 1);
+    s += ":";
+    put_fixed5(s, t.left[n - 2]);
+    s += ",\n";
+    two_way_split(t, s, nm, n - 2, 2);
+    s += ":";
+    put_fixed5(s, t.left[n - 1]);
+    s += ",\n";
+    two_way_split(t, s, nm, n - 2, 3);
+    s += ":";
+    put_fixed5(s, t.left[n]);
+    s += ");\n";
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // Tree::readTree and what TreeInterface derives from it (tree/Tree.cpp).
 constexpr int MAXNAMES = 150;       // general/clustalw.h:164
@@ -615,20 +636,7 @@ int cwhip_nj_tree(int nseq, const double* dist, const char* const* names, char* 
         t.n = nseq;
         t.d.assign(dist, dist + (size_t)(nseq + 1) * (nseq + 1));
         t.run();
-        // ClusterTreeOutput::printPhylipTree (:45-99)
-        s = "(\n";
-        two_way_split(t, s, nm, nseq - 2, 1);
-        s += ":";
-        put_fixed5(s, t.left[nseq - 2]);
-        s += ",\n";
-        two_way_split(t, s, nm, nseq - 2, 2);
-        s += ":";
-        put_fixed5(s, t.left[nseq - 1]);
-        s += ",\n";
-        two_way_split(t, s, nm, nseq - 2, 3);
-        s += ":";
-        put_fixed5(s, t.left[nseq]);
-        s += ");\n";
+        s = phylip(t, nm);
     }
     *out_len = (int64_t)s.size();
     if (!out || cap < (int64_t)s.size() + 1) return CWHIP_E_ARG;
@@ -873,3 +881,18 @@ int cwhip_clustal_text(int nseq, const char* const* names, const uint8_t* rows, 
 }
 
 }  // extern "C"
+
+std::string cw_phylip_text(int n, std::vector<int32_t>& desc, const std::vector<double>& left,
+                           const std::vector<double>& right, const char* const* names)
+{
+    NJ t;
+    t.n = n;
+    t.desc.swap(desc);
+    t.left = left;
+    t.right = right;
+    std::vector<std::string> nm(n + 1);
+    for (int i = 1; i <= n; ++i) nm[i] = names[i - 1];
+    std::string s = phylip(t, nm);
+    t.desc.swap(desc);
+    return s;
+}

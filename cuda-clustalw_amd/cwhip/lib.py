@@ -27,7 +27,7 @@ EXPORTS = [
     "cwhip_fast_pairwise_range", "cwhip_fast_last_ms", "cwhip_nj_tree", "cwhip_server_create",
     "cwhip_server_destroy", "cwhip_server_submit", "cwhip_server_wait", "cwhip_server_distances",
     "cwhip_server_text", "cwhip_server_times", "cwhip_server_release", "cwhip_server_stats", "cwhip_guide_from_tree", "cwhip_clustal_text",
-    "cwhip_pairwise_noalloc",
+    "cwhip_pairwise_noalloc", "cwhip_nj_tree_from_run", "cwhip_nj_tree_on_device",
 ]
 
 PRF_W = 35
@@ -175,6 +175,9 @@ def load() -> C.CDLL:
     L.cwhip_fast_last_ms.argtypes = [vp]
     L.cwhip_nj_tree.argtypes = [C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
                                 C.POINTER(C.c_int64)]
+    L.cwhip_nj_tree_from_run.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.cwhip_nj_tree_on_device.argtypes = [vp, C.c_int, P(np.float64, flags="C"), vp, vp, C.c_int64,
+                                          C.POINTER(C.c_int64)]
     L.cwhip_guide_from_tree.argtypes = [C.c_int, C.c_char_p, vp, P(np.int32, flags="C"),
                                         P(np.float64, flags="C"), P(np.int32, flags="C"),
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -291,6 +294,18 @@ class Context:
                   "cwhip_pairwise_pairs")
         return cnt[:n], dist[:n]
 
+    def nj_tree_from_run(self, names) -> str:
+        """.dnd text of the NJ guide tree built on the device from the last
+        pairwise run over all pairs of the loaded sequences."""
+        return _tree_text(lambda a, b, c, r: self._L.cwhip_nj_tree_from_run(self._h, a, b, c, r), names,
+                          "cwhip_nj_tree_from_run", self._h)
+
+    def nj_tree_on_device(self, dist_full, names) -> str:
+        """nj_tree's text, the joins run on the device."""
+        d = np.ascontiguousarray(dist_full, dtype=np.float64).reshape(-1)
+        return _tree_text(lambda a, b, c, r: self._L.cwhip_nj_tree_on_device(self._h, len(names), d, a, b, c, r),
+                          names, "cwhip_nj_tree_on_device", self._h)
+
     def pairwise_range(self, params, i_start, i_end, j_start, j_end, kernel=KERNEL_BATCHED):
         n = self._L.cwhip_range_npairs(len(self._lens), i_start, i_end, j_start, j_end)
         n = max(int(n), 0)
@@ -404,25 +419,30 @@ def _names(names):
     return arr
 
 
+def _tree_text(call, names, what, ctx=None) -> str:
+    """Runs call(names_array, buf, cap, len_ref) with room for every name and
+    two branch lengths per node (the tree is built once); a second call only if
+    the text did not fit."""
+    n = len(names)
+    arr = _names(names)
+    ln = C.c_int64()
+    cap = sum(len(x.encode()) for x in names) + 80 * (n + 2) + 64
+    buf = C.create_string_buffer(cap)
+    rc = call(C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), cap, C.byref(ln))
+    if rc and ln.value + 1 > cap:
+        buf = C.create_string_buffer(ln.value + 1)
+        rc = call(C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), ln.value + 1, C.byref(ln))
+    if rc:
+        msg = load().cwhip_last_error(ctx).decode() if ctx else ""
+        raise CwhipError("%s failed: %d %s" % (what, rc, msg))
+    return buf.value.decode()
+
+
 def nj_tree(dist_full, names) -> str:
     """.dnd text of the NJ guide tree of an (n+1)x(n+1) 1-based distance matrix."""
     L = load()
-    n = len(names)
     d = np.ascontiguousarray(dist_full, dtype=np.float64).reshape(-1)
-    arr = _names(names)
-    ln = C.c_int64()
-    # one call with room for every name and two branch lengths per node (the
-    # tree is built once); a second call only if the text did not fit
-    cap = sum(len(x.encode()) for x in names) + 80 * (n + 2) + 64
-    buf = C.create_string_buffer(cap)
-    rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), cap, C.byref(ln))
-    if rc and ln.value + 1 > cap:
-        buf = C.create_string_buffer(ln.value + 1)
-        rc = L.cwhip_nj_tree(n, d, C.cast(arr, C.c_void_p), C.cast(buf, C.c_void_p), ln.value + 1,
-                             C.byref(ln))
-    if rc:
-        raise CwhipError("cwhip_nj_tree failed: %d" % rc)
-    return buf.value.decode()
+    return _tree_text(lambda a, b, c, r: L.cwhip_nj_tree(len(names), d, a, b, c, r), names, "cwhip_nj_tree")
 
 
 def guide_from_tree(dnd: str, names, with_distance_tree: bool = False):

@@ -38,9 +38,10 @@ def align(ctx: lib.Context, names, seqs, dna: bool, timings: dict | None = None,
     ctx.load_sequences(codes, offs, lens)
     if guide_dnd is None:
         params = lib.pair_params(dna=dna, max_len=max(int(lens.max()) if n else 1, 1))
-        pairs, cnt, dist = ctx.pairwise_range(params, 0, n, 0, n)
+        ctx.pairwise_range(params, 0, n, 0, n)
         t1 = time.perf_counter()
-        dnd = lib.nj_tree(distance_matrix(n, pairs, dist), names)
+        # the guide tree from the distances as they stand in device memory
+        dnd = ctx.nj_tree_from_run(names)
     else:
         t1 = time.perf_counter()
         dnd = guide_dnd

@@ -377,6 +377,23 @@ int cwhip_msa_stats(cwhip_ctx *ctx, double *out6);
 int cwhip_nj_tree(int nseq, const double *dist, const char *const *names, char *out,
                   int64_t cap, int64_t *len);
 
+/* The same tree, built on the context's device (tree/NJTree.cpp:161-751 as
+ * above, every double computed by the reference's expression in its order, so
+ * the text is byte-identical to cwhip_nj_tree's):
+ *   cwhip_nj_tree_from_run   over the DistMatrix of the context's last
+ *       finished pairwise run, which must have covered every pair of the
+ *       loaded sequences (cwhip_pairwise_range over 0..nseq x 0..nseq): the
+ *       distances are formed and joined in device memory and never cross to
+ *       the host (only the tree does); nseq and names as loaded;
+ *   cwhip_nj_tree_on_device  over a caller's (nseq+1)^2 DistMatrix, uploaded.
+ * The kernel takes 4 .. CWHIP_NJ_DEVICE_MAX sequences; other sizes (and a
+ * matrix that is not symmetric) run the host code of cwhip_nj_tree.  Sizing
+ * as cwhip_nj_tree.  CWHIP_E_ARG when the context has no such run. */
+#define CWHIP_NJ_DEVICE_MAX 3000
+int cwhip_nj_tree_from_run(cwhip_ctx *ctx, const char *const *names, char *out, int64_t cap, int64_t *len);
+int cwhip_nj_tree_on_device(cwhip_ctx *ctx, int nseq, const double *dist, const char *const *names,
+                            char *out, int64_t cap, int64_t *len);
+
 /* Tree::readTree, calcSeqWeights, calcSimilarities and createSets as
  * TreeInterface::getWeightsAndStepsUseOldGuideTreeNJ calls them
  * (tree/TreeInterface.cpp:209-276, tree/Tree.cpp): from .dnd text to the
