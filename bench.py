@@ -41,7 +41,7 @@ def ref_cells(lens, pairs, info):
     m = lens[pairs[:, 1]].astype(np.int64)
     ms, se1, se2, sb1, sb2 = (info[:, k].astype(np.int64) for k in range(5))
     rev = np.where(ms > 0, (se1 - sb1) * se2 + (se2 - sb2 + 1), 0)
-    return n * m, rev, info[:, 7].astype(np.int64)
+    return n * m, rev, info[:, 7].view(np.uint32).astype(np.int64)
 
 
 def host_cpu():

@@ -74,3 +74,11 @@ static inline int cw_lcap(int maxlen)
 {
     return ((maxlen + 2 * CW_SEQ_PAD + 1) + 15) & ~15;
 }
+
+// Waves per workgroup: `want`, or fewer where the waves' sequence buffers and
+// the matrix would exceed a CU's 160 KiB of LDS (0: not even one wave fits).
+static inline int cw_fit_waves(int want, int wave_lds)
+{
+    const int fit = (160 * 1024 - CW_TAB_BYTES) / wave_lds;
+    return fit < want ? fit : want;
+}

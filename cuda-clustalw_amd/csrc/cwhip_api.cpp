@@ -353,14 +353,15 @@ static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int thre
 // than pairs (a small job leaves the rest of the chip to whatever runs beside it).
 static int batched_shape(cwhip_ctx* ctx, const DevArgs& a, int64_t npairs, int* blocks, int* threads, size_t* lds)
 {
-    *threads = 64 * ctx->wpg;
-    *lds = CW_TAB_BYTES + (size_t)ctx->wpg * a.wave_lds;
-    if (*lds > 160 * 1024) return fail(ctx, CWHIP_E_TOOLONG, "per-workgroup LDS exceeds 160 KiB");
+    const int wpg = cw_fit_waves(ctx->wpg, a.wave_lds);
+    if (wpg < 1) return fail(ctx, CWHIP_E_TOOLONG, "one wave's sequence buffers exceed the LDS");
+    *threads = 64 * wpg;
+    *lds = CW_TAB_BYTES + (size_t)wpg * a.wave_lds;
     int occ_plain = 0, occ_pers = 0;
     HIPCHK(cw_occupancy(*threads, *lds, &occ_plain, &occ_pers));
     if (occ_pers < 1 || occ_plain < 1) return fail(ctx, CWHIP_E_TOOLONG, "kernel does not fit on a CU");
     const int gpc = ctx->groups_per_cu ? std::min(ctx->groups_per_cu, occ_pers) : occ_pers;
-    *blocks = (int)std::min<int64_t>((int64_t)ctx->num_cu * gpc, (npairs + ctx->wpg - 1) / ctx->wpg);
+    *blocks = (int)std::min<int64_t>((int64_t)ctx->num_cu * gpc, (npairs + wpg - 1) / wpg);
     return CWHIP_OK;
 }
 
@@ -372,8 +373,11 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     if (ctx->npairs == 0) { if (ms) *ms = 0; if (kernel_ms) *kernel_ms = 0; return CWHIP_OK; }
     DevArgs a;
     fill_args(ctx, a);
-    int threads = 64 * ctx->wpg, pers_blocks = 0;
-    size_t lds = CW_TAB_BYTES + (size_t)ctx->wpg * a.wave_lds;
+    // fewer waves per workgroup where long sequences fill the LDS
+    const int wpg = cw_fit_waves(ctx->wpg, a.wave_lds);
+    if (wpg < 1) return fail(ctx, CWHIP_E_TOOLONG, "one wave's sequence buffers exceed the LDS");
+    int threads = 64 * wpg, pers_blocks = 0;
+    size_t lds = CW_TAB_BYTES + (size_t)wpg * a.wave_lds;
     int64_t slots;
     if (kernel == CWHIP_KERNEL_BATCHED) {
         for (int c = 0; c < CW_NCLASS; ++c) {
@@ -382,14 +386,13 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
         }
         const int rc = batched_shape(ctx, a, ctx->npairs, &pers_blocks, &threads, &lds);
         if (rc) return rc;
-        slots = (int64_t)pers_blocks * ctx->wpg;
+        slots = (int64_t)pers_blocks * wpg;
     } else {
-        if (lds > 160 * 1024) return fail(ctx, CWHIP_E_TOOLONG, "per-workgroup LDS exceeds 160 KiB");
         int occ_plain = 0, occ_pers = 0;
         HIPCHK(cw_occupancy(threads, lds, &occ_plain, &occ_pers));
         if (occ_plain < 1) return fail(ctx, CWHIP_E_TOOLONG, "kernel does not fit on a CU");
         slots = std::min<int64_t>(kPlainChunk, ctx->npairs);
-        slots = (slots + ctx->wpg - 1) / ctx->wpg * ctx->wpg;
+        slots = (slots + wpg - 1) / wpg * wpg;
     }
     HIPCHK(ctx->d_scratch.ensure((size_t)slots * a.scratch_stride));
     a.scratch = ctx->d_scratch.p;
@@ -408,7 +411,7 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
             for (int64_t b0 = 0; b0 < ctx->npairs; b0 += slots) {
                 a.pair_base = b0;
                 a.pair_end = std::min<int64_t>(ctx->npairs, b0 + slots);
-                const int64_t nb = (a.pair_end - b0 + ctx->wpg - 1) / ctx->wpg;
+                const int64_t nb = (a.pair_end - b0 + wpg - 1) / wpg;
                 HIPCHK(cw_launch_plain((int)nb, threads, lds, ctx->stream, a));
             }
         }
@@ -497,7 +500,7 @@ int cwhip_autotune(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t* pa
     size_t lds = 0;
     int rc = batched_shape(ctx, a0, INT64_MAX / 2, &blocks, &threads, &lds);
     if (rc) return rc;
-    const int64_t waves = (int64_t)blocks * ctx->wpg;
+    const int64_t waves = (int64_t)blocks * (threads / 64);
     for (int c = 0; c < CW_NCLASS; ++c) {
         const auto& mem = members[c];
         if (mem.empty()) continue;

@@ -120,16 +120,17 @@ extern "C" int cwhip_pairwise_noalloc(const cwhip_pair_dev_params* p, const uint
 
     // resident workgroups: as many as fit on the chip, never more waves than
     // pairs, and no more than the DP rows the rest of the array holds
-    const int threads = 64 * kWavesPerGroup;
-    const size_t lds = CW_TAB_BYTES + (size_t)kWavesPerGroup * a.wave_lds;
-    if (lds > 160 * 1024) return CWHIP_E_TOOLONG;
+    const int wpg = cw_fit_waves(kWavesPerGroup, a.wave_lds);
+    if (wpg < 1) return CWHIP_E_TOOLONG;
+    const int threads = 64 * wpg;
+    const size_t lds = CW_TAB_BYTES + (size_t)wpg * a.wave_lds;
     int occ_plain = 0, occ_pers = 0;
     if (cw_occupancy(threads, lds, &occ_plain, &occ_pers) != hipSuccess) return CWHIP_E_HIP;
     if (occ_pers < 1) return CWHIP_E_TOOLONG;
-    const int64_t fit = (kScratchInts - rows_at) / (a.scratch_stride * kWavesPerGroup);
+    const int64_t fit = (kScratchInts - rows_at) / (a.scratch_stride * wpg);
     if (fit < 1) return CWHIP_E_NOMEM;
     int64_t blocks = (int64_t)ds.num_cu * occ_pers;
-    blocks = std::min(blocks, (npairs + kWavesPerGroup - 1) / kWavesPerGroup);
+    blocks = std::min(blocks, (npairs + wpg - 1) / wpg);
     blocks = std::min(blocks, fit);
     if (blocks < 1) blocks = 1;   // npairs == 0: the reset still clears out_status
     return cw_launch_batched((int)blocks, threads, lds, (hipStream_t)stream, a) == hipSuccess ? CWHIP_OK

@@ -824,7 +824,7 @@ int cwhip_profile_dp(cwhip_ctx* ctx, cwhip_prf_pair* pairs, int npairs, int end_
     for (int k = 0; k < npairs; ++k) {
         const cwhip_prf_pair& q = pairs[k];
         if (!q.prf1 || !q.prf2 || !q.displ || q.len1 < 0 || q.len2 < 0 ||
-            q.len1 > CWHIP_MAX_LEN * 4 || q.len2 > CWHIP_MAX_LEN * 4)
+            q.len1 > CWHIP_MAX_PROFILE_LEN || q.len2 > CWHIP_MAX_PROFILE_LEN)
             return fail(ctx, CWHIP_E_ARG, "bad profile pair");
         displ[k].assign(q.len1 + q.len2 + 4, 0);
         jobs[k].prf1 = q.prf1;

@@ -8,5 +8,5 @@ from . import seqgen  # noqa: F401
 from .lib import (KERNEL_BATCHED, KERNEL_PLAIN, Context, CwhipError,  # noqa: F401
                   FastParams, MsaParams, TuneTable, clustal_text, distance, fast_params,
                   guide_from_tree, load,
-                  JOB_ALIGNMENT, JOB_DISTANCES, Server, msa_params, nj_tree,
+                  JOB_ALIGNMENT, JOB_DISTANCES, MAX_LEN, Server, msa_params, nj_tree,
                   pair_params)

@@ -94,6 +94,7 @@ class PairDevParams(C.Structure):
 
 NOALLOC_SCRATCH_BYTES = 128 << 20
 NOALLOC_MAX_PAIRS = 1 << 22
+MAX_LEN = 32767                 # CWHIP_MAX_LEN: longest sequence the pairwise kernels take
 
 TUNE_CLASSES = 6
 TUNE_CLASS_EDGES = (128, 256, 512, 1024, 2048)   # tuning class = first edge >= the longer sequence

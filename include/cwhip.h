@@ -47,7 +47,12 @@ enum {
                                cwhip_profile_matrix_fn returned <= 0) */
 };
 
-#define CWHIP_MAX_LEN 16000  /* longest sequence the kernels accept */
+/* Longest sequence the pairwise kernels accept: a DP cell's (row, column) is
+ * packed as row << 16 | column in one signed 32-bit key.  Both sequences of a
+ * pair sit in a wave's LDS, so beyond ~16000 residues a workgroup runs fewer
+ * than four waves (two at 32767). */
+#define CWHIP_MAX_LEN 32767
+#define CWHIP_MAX_PROFILE_LEN 64000  /* longest profile (columns) cwhip_profile_dp accepts */
 
 /* kernel choice for cwhip_pairwise*; both give identical results */
 enum {
@@ -163,7 +168,8 @@ int cwhip_fetch_counts(cwhip_ctx *ctx, int32_t *out_count, int64_t npairs);
  * [maxScore, se1, se2, sb1, sb2, status, Myers-Miller internal nodes,
  *  Myers-Miller cell updates of the reference's diff()].  The reference's
  * forward pass updates n*m cells and its reverse pass (se1-sb1)*se2 +
- * (se2-sb2+1) cells (0 when maxScore is 0). */
+ * (se2-sb2+1) cells (0 when maxScore is 0).  The last field is an unsigned
+ * 32-bit count: pairs of two sequences near CWHIP_MAX_LEN reach ~3.4e9. */
 int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
 
 /* Per-wave tuning knobs (0 = keep the built-in default): rows per lane limit,

@@ -57,7 +57,7 @@ def run_and_compare(ctx, codes, offs, lens, pairs, scoring=SC, params=None,
         np.testing.assert_array_equal(info[nonempty, :5], info_o[nonempty],
                                       err_msg="maxScore/se/sb, kernel %d" % k)
         assert (info[:, 5] == 0).all()
-        np.testing.assert_array_equal(info[nonempty, 7].astype(np.int64), mm_o[nonempty])
+        np.testing.assert_array_equal(info[nonempty, 7].view(np.uint32).astype(np.int64), mm_o[nonempty])
         c2, d2 = ctx.pairwise_pairs(params, pairs, kernel=k)
         np.testing.assert_array_equal(c2, cnt_o)
         np.testing.assert_array_equal(d2.view(np.uint32), dist_o.view(np.uint32))
@@ -123,6 +123,22 @@ def test_long_sequences_multi_stripe(ctx):
     codes, offs, lens = fam.packed()
     assert lens.min() > 3500
     run_and_compare(ctx, codes, offs, lens, all_pairs(len(lens)))
+
+
+def test_longest_sequences_fewer_waves_per_workgroup(ctx):
+    # up to CWHIP_MAX_LEN residues: two sequences' LDS buffers per wave leave
+    # room for two waves per workgroup (four below ~16000); one longer is refused
+    fam = seqgen.family(31, 3, 24000)
+    rng = np.random.default_rng(31)
+    s0 = np.concatenate([fam.seqs[0], rng.choice(FULL, 40000).astype(np.uint8)])[:cwhip.MAX_LEN]
+    seqs = [s0, fam.seqs[1], fam.seqs[2][:16500]]
+    codes, offs, lens = pack(seqs)
+    assert lens.max() == cwhip.MAX_LEN and lens.min() == 16500
+    print("longest sequences", lens.tolist(), flush=True)
+    run_and_compare(ctx, codes, offs, lens, all_pairs(len(seqs)))
+    codes, offs, lens = pack([np.zeros(cwhip.MAX_LEN + 1, np.uint8), s0])
+    with pytest.raises(cwhip.CwhipError):
+        ctx.load_sequences(codes, offs, lens)
 
 
 def test_large1523_sample(ctx):

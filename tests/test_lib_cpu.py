@@ -29,6 +29,17 @@ def test_library_exports_header_symbols():
     assert set(names) == set(cl.EXPORTS)
 
 
+def test_header_limits_equal_python_constants():
+    txt = open(os.path.join(ROOT, "include", "cwhip.h")).read()
+    defs = {name: int(v) << int(sh or 0) for name, v, sh in
+            re.findall(r"#define\s+(CWHIP_[A-Z_]+)\s+\(?([0-9]+)(?:ll)?(?:\s*<<\s*([0-9]+))?", txt)}
+    assert defs["CWHIP_MAX_LEN"] == cl.MAX_LEN == 32767
+    assert defs["CWHIP_NOALLOC_SCRATCH_BYTES"] == cl.NOALLOC_SCRATCH_BYTES
+    assert defs["CWHIP_NOALLOC_MAX_PAIRS"] == cl.NOALLOC_MAX_PAIRS
+    # (row << 16 | column) keys stay non-negative up to the limit
+    assert (cl.MAX_LEN << 16 | cl.MAX_LEN) < 2 ** 31
+
+
 def test_calibration_library_loads():
     """The VALU calibration kernels bench.py --valu-probe times live in their
     own library, and without a device the call reports an error, not a

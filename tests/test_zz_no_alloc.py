@@ -179,5 +179,5 @@ def test_noalloc_argument_limits():
     dp = cl.PairDevParams(one, one, 1, 30, 31, 100)
     assert L.cwhip_pairwise_noalloc(C.byref(dp), one, one, one, 1, one, cl.NOALLOC_MAX_PAIRS + 1,
                                     one, one, None) == -6
-    dp.max_len = 16001
+    dp.max_len = cl.MAX_LEN + 1   # one past CWHIP_MAX_LEN (its value is checked against the header)
     assert L.cwhip_pairwise_noalloc(C.byref(dp), one, one, one, 1, one, 1, one, one, None) == -4
