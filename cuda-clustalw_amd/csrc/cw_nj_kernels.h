@@ -7,7 +7,7 @@
 
 #include "../../include/cwhip.h"
 
-#define CW_NJ_MAX_SEQS CWHIP_NJ_DEVICE_MAX   // per-node arrays in LDS: 44 (n+1) bytes
+#define CW_NJ_MAX_SEQS CWHIP_NJ_DEVICE_MAX   // per-node arrays in LDS: 52 (n+1) bytes
 
 struct NjArgs {
     const int32_t* pairs;     // staged pair list (si, sj), si < sj

@@ -19,6 +19,7 @@
 // scan position).  Built with FP contraction off, like the host code.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <cmath>
 
 #include "cw_nj_kernels.h"
 
@@ -27,6 +28,8 @@ namespace cwn {
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 constexpr int kPerThread = (CW_NJ_MAX_SEQS + kThreads - 1) / kThreads;   // list entries per thread
+static_assert((CW_NJ_MAX_SEQS + 1) * (6 * sizeof(double) + sizeof(int)) + 1024 <= 160 * 1024,
+              "per-node arrays exceed the LDS");
 
 // FullPairwiseAlign.cpp:172-181 (a refused pair's distance is 1.0)
 __device__ __forceinline__ float pair_distance(int count, int len1, int len2)
@@ -61,6 +64,32 @@ __device__ __forceinline__ void pair_of(long long k, int& q, int& r)
     r = (int)(k - (long long)qq * (qq - 1) / 2);
 }
 
+// s + x[a] + x[a + 1] + ... + x[b - 1], added one by one in that order (the
+// reference's rounding); the loads run eight values ahead of the adds
+__device__ double seq_sum(const double* x, int a, int b, double s)
+{
+    int q = a;
+    if (q + 8 <= b) {
+        double cur[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) cur[v] = x[q + v];
+        for (; q + 16 <= b; q += 8) {
+            double nxt[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) nxt[v] = x[q + 8 + v];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) s = s + cur[v];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) cur[v] = nxt[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s = s + cur[v];
+        q += 8;
+    }
+    for (; q < b; ++q) s = s + x[q];
+    return s;
+}
+
 // the largest i in 1 .. upto with desc(i, node) == 1 (0 if none), workgroup-wide
 __device__ int last_row_with(const int32_t* desc, int ld, int node, int upto, int* s_best)
 {
@@ -80,22 +109,25 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = a.n, ld = n + 1;
+    // by node
     double* sumCols = (double*)smem;
     double* sumRows = sumCols + (n + 1);
     double* av = sumRows + (n + 1);
-    double* join = av + (n + 1);
-    double* dq = join + (n + 1);
-    int* lv = (int*)(dq + (n + 1));                  // live nodes in list (ascending) order
+    // by position in the live list (lv[q] is the node at position q)
+    double* dq = av + (n + 1);                       // sumCols + sumRows of lv[q]
+    double* cq = dq + (n + 1);                       // sumCols of lv[q]: the terms of sumd
+    double* jq = cq + (n + 1);                       // the merged node's distance to lv[q]
+    int* lv = (int*)(jq + (n + 1));                  // live nodes in list (ascending) order
     __shared__ double red_v[kWaves];
     __shared__ long long red_k[kWaves];
-    __shared__ double s_sumd;
-    __shared__ int s_mini, s_minj, s_qpos, s_type[2], s_best;
+    __shared__ double s_sumd, s_part[3];
+    __shared__ int s_mini, s_minj, s_qpos, s_rpos, s_type[2], s_best;
     double* D = a.D;
     int32_t* desc = a.desc;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     for (int i = tid; i <= n; i += kThreads) {
-        sumCols[i] = sumRows[i] = av[i] = join[i] = 0.0;
+        sumCols[i] = sumRows[i] = av[i] = 0.0;
         if (i < n) lv[i] = i + 1;
     }
     for (int i = 1 + tid; i <= n; i += kThreads) D[(int64_t)i * ld + i] = 0.0;
@@ -109,44 +141,103 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
         tmp = 0.0;
         for (int p = j + 1; p <= n; ++p) tmp += row[p];
         sumCols[j] = tmp;
+        cq[j - 1] = sumCols[j];
+        dq[j - 1] = sumCols[j] + sumRows[j];
     }
     __syncthreads();
 
     double fnseqs = (double)n;
     int m = n;
+    // sumd: the live nodes' column sums added in list order (:282-287); for
+    // the later joins it is formed at the end of the join before
+    if (tid == 0) s_sumd = seq_sum(cq, 0, m, 0.0);
+    __syncthreads();
     for (int nc = 1; nc <= n - 3; ++nc) {
-        for (int q = tid; q < m; q += kThreads) {
-            const int j = lv[q];
-            dq[q] = sumCols[j] + sumRows[j];
-        }
-        if (tid == 0) {
-            double s = 0.0;
-            for (int q = 0; q < m; ++q) s += sumCols[lv[q]];
-            s_sumd = s;
-        }
-        __syncthreads();
-        const double sumd = s_sumd;
+        const double twosumd = 2.0 * s_sumd;          // exact: a power-of-two scale
         const double fnseqs2 = fnseqs - 2.0;
-        // the scan: total of pair (lv[r], lv[q]), r < q, at position k
-        const long long npr = (long long)m * (m - 1) / 2;
-        double tm = 99999.0 * 2.0 * fnseqs2;
+        // The scan (:289-330): the total of pair (lv[r], lv[q]), r < q, at scan
+        // position k = q (q - 1) / 2 + r.  The work is cut into runs of rows q
+        // for one block of 64 positions r (lane l takes r = 64 rb + l): block rb
+        // has rows 64 rb + 1 .. m - 1, and the blocks' rows, laid end to end,
+        // are split evenly over the waves.  A lane meets its pairs of one run
+        // in ascending k, so a strict "<" keeps the first of equal totals; runs
+        // are merged on (total, k).  Row q's node and sum are the same for
+        // every lane (LDS broadcasts), the lanes read consecutive entries of
+        // row lv[q], and 16 rows' loads are issued before their totals are
+        // compared, in q order.
+        const double bound = 99999.0 * 2.0 * fnseqs2;
+        double tm = bound;
         long long bk = LLONG_MAX;
-        long long k = tid;
-        if (k < npr) {
-            int q, r;
-            pair_of(k, q, r);
-            for (; k < npr; k += kThreads) {
-                const int jj = lv[q], ii = lv[r];
-                const double total = fnseqs2 * D[(int64_t)jj * ld + ii] + 2.0 * sumd - dq[r] - dq[q];
-                if (total < tm) {
-                    tm = total;
-                    bk = k;
+        {
+            const int nrb = (m - 1 + 63) / 64;           // blocks with a row: 64 rb + 1 < m
+            const int total_rows = nrb * (m - 1) - 32 * nrb * (nrb - 1);
+            const int s0 = (int)((long long)total_rows * wave / kWaves);
+            const int s1 = (int)((long long)total_rows * (wave + 1) / kWaves);
+            int rb = 0, base = 0;                        // rows of the blocks before rb
+            while (rb < nrb && base + (m - 1 - 64 * rb) <= s0) {
+                base += m - 1 - 64 * rb;
+                ++rb;
+            }
+            int pos = s0;
+            while (pos < s1) {
+                const int len = m - 1 - 64 * rb;
+                const int q0 = 64 * rb + 1 + (pos - base);
+                const int q1 = 64 * rb + 1 + (s1 - base < len ? s1 - base : len);
+                const int r = 64 * rb + lane;
+                const int ii = r < m ? lv[r] : 0;        // column 0 is in bounds
+                const double dr = r < m ? dq[r] : 0.0;
+                const double* col = D + ii;
+                double tu = bound;
+                int bq = -1;
+                // 16 rows: their totals, the smallest of them, and only when
+                // that beats the lane's best, the first row that has it.  Rows
+                // not above the lane's position (the first 64 rows of a block)
+                // take no part: +inf.
+                auto rows16 = [&](int q, bool diag) {
+                    int jv[16];
+                    double t[16];
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) jv[v] = lv[q + v];
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) t[v] = col[(int64_t)jv[v] * ld];
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        t[v] = fnseqs2 * t[v] + twosumd - dr - dq[q + v];
+                        if (diag && !(r < q + v)) t[v] = HUGE_VAL;
+                    }
+                    double mn = t[0];
+#pragma unroll
+                    for (int v = 1; v < 16; ++v) mn = fmin(mn, t[v]);
+                    if (mn < tu) {
+                        int f = 15;
+#pragma unroll
+                        for (int v = 14; v >= 0; --v)
+                            if (t[v] == mn) f = v;
+                        tu = mn;
+                        bq = q + f;
+                    }
+                };
+                int q = q0;
+                const int qd = 64 * rb + 64;                 // rows from here on: every lane
+                for (; q + 16 <= q1 && q < qd; q += 16) rows16(q, true);
+                for (; q + 16 <= q1; q += 16) rows16(q, false);
+                for (; q < q1; ++q) {
+                    const double total = fnseqs2 * col[(int64_t)lv[q] * ld] + twosumd - dr - dq[q];
+                    if (r < q && total < tu) {
+                        tu = total;
+                        bq = q;
+                    }
                 }
-                r += kThreads;
-                while (r >= q) {
-                    r -= q;
-                    ++q;
+                if (bq >= 0) {
+                    const long long k = (long long)bq * (bq - 1) / 2 + r;
+                    if (tu < tm || (tu == tm && k < bk)) {
+                        tm = tu;
+                        bk = k;
+                    }
                 }
+                pos = base + (s1 - base < len ? s1 - base : len);
+                base += len;
+                ++rb;
             }
         }
         for (int o = 32; o > 0; o >>= 1) {
@@ -200,9 +291,10 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
             s_mini = mini;
             s_minj = minj;
             s_qpos = qpos;
+            s_rpos = qpos < 0 ? -1 : (int)(kk - (long long)qpos * (qpos - 1) / 2);
         }
         __syncthreads();
-        const int mini = s_mini, minj = s_minj, qpos = s_qpos;
+        const int mini = s_mini, minj = s_minj, qpos = s_qpos, qm = s_rpos;
         if (qpos < 0) return;                       // status set; every thread leaves here
         // treeDesc row nc: the joined nodes' leaves (:501-535)
         for (int i = 1 + tid; i <= n; i += kThreads) desc[(int64_t)nc * ld + i] = 0;
@@ -234,7 +326,8 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
         }
         --m;
         __syncthreads();
-        // :558-605, per live node (each touches only its own entries)
+        // :558-605, per live node (each touches only its own entries); the
+        // merged node's own sums are redone below
         for (int q = tid; q < m; q += kThreads) {
             const int j = lv[q];
             double tdi = 0.0, tdj = 0.0;
@@ -254,7 +347,7 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
                 sr -= tdi;
             }
             const double jn = (tdj + tdi) * 0.5;
-            join[j] = jn;
+            jq[q] = jn;
             if (j < mini) {
                 D[(int64_t)j * ld + mini] = jn;
                 D[(int64_t)mini * ld + j] = jn;
@@ -266,24 +359,28 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
             }
             sumCols[j] = sc;
             sumRows[j] = sr;
+            cq[q] = sc;
+            dq[q] = sc + sr;
+        }
+        __syncthreads();
+        // The merged node's sums over the live nodes before and after it in
+        // the list (mini keeps its position qm: minj was after it), and the
+        // next join's sumd, each a sequential sum in list order; the three
+        // independent chains run on three waves.  minj's row and column
+        // (zeroed by the reference) are never read again.
+        if (lane == 0 && wave < 3) {
+            if (wave == 0) s_part[0] = seq_sum(jq, 0, qm, 0.0);          // sumRows[mini]
+            else if (wave == 1) s_part[1] = seq_sum(jq, qm + 1, m, 0.0); // sumCols[mini]
+            else s_part[2] = seq_sum(cq, 0, qm, 0.0);                    // sumd up to mini
         }
         __syncthreads();
         if (tid == 0) {
-            double da = 0.0;
-            int q = 0;
-            for (; q < m && lv[q] < mini; ++q) da = da + join[lv[q]];
-            sumRows[mini] = da;
-            if (q < m && lv[q] == mini) ++q;
-            da = 0.0;
-            for (; q < m; ++q) da = da + join[lv[q]];
-            sumCols[mini] = da;
+            sumRows[mini] = s_part[0];
+            sumCols[mini] = s_part[1];
             sumCols[minj] = sumRows[minj] = 0.0;
-        }
-        __syncthreads();
-        for (int j = 1 + tid; j <= n; j += kThreads) {
-            D[(int64_t)minj * ld + j] = 0.0;
-            D[(int64_t)j * ld + minj] = 0.0;
-            join[j] = 0.0;
+            cq[qm] = sumCols[mini];
+            dq[qm] = sumCols[mini] + sumRows[mini];
+            s_sumd = seq_sum(cq, qm, m, s_part[2]);
         }
         __syncthreads();
     }
@@ -322,7 +419,7 @@ extern "C" __global__ void __launch_bounds__(kThreads) cw_nj(NjArgs a)
 
 size_t cw_nj_lds_bytes(int n)
 {
-    return (size_t)(n + 1) * (5 * sizeof(double) + sizeof(int));
+    return (size_t)(n + 1) * (6 * sizeof(double) + sizeof(int));
 }
 
 hipError_t cw_launch_nj(const NjArgs& a, hipStream_t st)
