@@ -97,6 +97,7 @@ struct Sweep {
     int lh0, lhs, lfo;               // left col  : H[r][0] = lh0 + r*lhs, F = H + lfo (r>=1)
     int go, ge, goe;                 // H[0][0] = 0 in every pass
     int thr;                         // MODE_REV threshold (maxScore)
+    int kb;                          // packed tracking: column bits (ncols < 2^kb)
     lcu8* tab;                       // LDS substitution matrix, int32 [32][32], byte-addressed
 };
 
@@ -117,7 +118,15 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 // One stripe: rows row0+1 .. row0+P.  R rows per lane.  DIRECT: R==1 and P<64,
 // lanes >= P idle and lane P-1 writes its row straight to out (only ever the
 // single stripe of a sweep, so its top boundary is the formula row).
-template <int R, int MODE, bool DIRECT>
+//
+// PACK (MODE_FWD / MODE_REV, when the pair's scores fit beside S.kb column
+// bits): a row's tracked cell is one int, max over (h << kb) | (2^kb-1 - c),
+// i.e. the largest h and among equal h the smallest column -- one shift-or
+// and one max per cell instead of a compare and two selects.  MODE_FWD with
+// PACK also clamps E and F at 0: with ge >= 0 that gives max(E, 0) and
+// max(F, 0) at every cell (induction over E' = max(E - ge, H - go - ge, 0))
+// and leaves every H unchanged, so H = max3(diag + s, E, F) needs no clamp.
+template <int R, int MODE, bool DIRECT, bool PACK>
 __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int first,
                                        const int* inH, const int* inE, int* outH,
                                        int* outE, int want_out, Track& T)
@@ -144,11 +153,17 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         hL[r] = left_h(S, row);
         f[r] = hL[r] + S.lfo;
         hLg[r] = hL[r] - S.goe;
-        tv[r] = 0;
+        tv[r] = (PACK && MODE == MODE_REV) ? (int)0x80000000 : 0;
         tj[r] = BIG;
     }
     int dprev = left_h(S, firstrow - 1);
     const int ge = uni(S.ge), goe = uni(S.goe), thr = uni(S.thr);
+    const int kb = uni(S.kb), cmask = (1 << kb) - 1;
+    // 0 in a scalar register the compiler cannot see through: with a literal
+    // 0 it proves E, F >= 0, turns the H maximum into a signed and an unsigned
+    // max and loses the single v_max3
+    int zero;
+    asm("s_mov_b32 %0, 0" : "=s"(zero));
     lcu8* const tab = S.tab;
     lcu8* const cs = S.cs;
     const int cd = uni(S.cd);
@@ -189,6 +204,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         caddr += cd;
         b4 = cs[caddr];                                 // prefetch next step's column code
         const int c = t - lane;
+        const int ck = cmask - c;                       // packed column key: smaller c ranks higher
         bool act = true;
         if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= ncols;
         if (act) {
@@ -198,10 +214,17 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int s = score_at(tab, sb[r] + bcur);
-                int e = max(eup - ge, hupg);
-                f[r] = max(f[r] - ge, hLg[r]);
-                int h = max(max(dg + s, e), f[r]);
-                if (MODE == MODE_FWD) h = max(h, 0);
+                int e, h;
+                if (MODE == MODE_FWD && PACK) {
+                    e = max(max(eup - ge, hupg), zero);
+                    f[r] = max(max(f[r] - ge, hLg[r]), zero);
+                    h = max(max(dg + s, e), f[r]);
+                } else {
+                    e = max(eup - ge, hupg);
+                    f[r] = max(f[r] - ge, hLg[r]);
+                    h = max(max(dg + s, e), f[r]);
+                    if (MODE == MODE_FWD) h = max(h, 0);
+                }
                 if (r == 0 && R > 1 && !DIRECT) {
                     if (dummy) { h = hup; e = eup; }
                 }
@@ -211,7 +234,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 hup = h;
                 hupg = hLg[r];
                 eup = e;
-                if (MODE == MODE_FWD) {
+                if (PACK && MODE != MODE_GLB) {
+                    tv[r] = max(tv[r], (int)(((unsigned)h << kb) | (unsigned)ck));
+                } else if (MODE == MODE_FWD) {
                     if (h > tv[r]) { tv[r] = h; tj[r] = c; }
                 } else if (MODE == MODE_REV) {
                     tj[r] = min(tj[r], h >= thr ? c : BIG);
@@ -282,7 +307,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         for (int r = 0; r < R; ++r) {
             const int row = firstrow + r - d;
             const bool valid = lane_on && !(r == 0 && dummy);
-            if (valid && tv[r] > v) { v = tv[r]; key = (row << 16) | tj[r]; }
+            const int tvr = PACK ? (tv[r] >> kb) : tv[r];
+            const int tjr = PACK ? cmask - (tv[r] & cmask) : tj[r];
+            if (valid && tvr > v) { v = tvr; key = (row << 16) | tjr; }
         }
         const int V = wave_max(v);
         const int K = wave_min(v == V ? key : BIG);
@@ -293,7 +320,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         for (int r = 0; r < R; ++r) {
             const int row = firstrow + r - d;
             const bool valid = lane_on && !(r == 0 && dummy);
-            if (valid && tj[r] != BIG) key = min(key, (row << 16) | tj[r]);
+            const bool hit = PACK ? tv[r] >= (int)((unsigned)thr << kb) : tj[r] != BIG;
+            const int tjr = PACK ? cmask - (tv[r] & cmask) : tj[r];
+            if (valid && hit) key = min(key, (row << 16) | tjr);
         }
         const int K = wave_min(key);
         if (K != BIG) { T.best = 1; T.key = K; }
@@ -304,29 +333,29 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
 // own (no spills at <= 128 VGPRs) and the call costs nothing next to a stripe
 // of >= 128 rows.  Single-row stripes, which dominate the deep Myers-Miller
 // nodes, stay inline.
-template <int R, int MODE>
+template <int R, int MODE, bool PACK>
 __device__ __attribute__((noinline)) void stripe_call(const Sweep S, int row0, int P, int first,
                                                       const int* inH, const int* inE, int* outH,
                                                       int* outE, int want_out, Track* T)
 {
-    stripe<R, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, *T);
+    stripe<R, MODE, false, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, *T);
 }
 
-template <int MODE>
+template <int MODE, bool PACK>
 __device__ void run_stripe(int R, bool direct, const Sweep& S, int row0, int P, int first,
                            const int* inH, const int* inE, int* outH, int* outE, int want_out,
                            Track& T)
 {
-    if (direct) { stripe<1, MODE, true>(S, row0, P, first, inH, inE, outH, outE, want_out, T); return; }
+    if (direct) { stripe<1, MODE, true, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, T); return; }
     switch (R) {
-    case 1: stripe<1, MODE, false>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
-    case 2: stripe_call<2, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    case 3: stripe_call<3, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    case 4: stripe_call<4, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    case 5: stripe_call<5, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    case 6: stripe_call<6, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    case 7: stripe_call<7, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
-    default: stripe_call<8, MODE>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 1: stripe<1, MODE, false, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, T); break;
+    case 2: stripe_call<2, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 3: stripe_call<3, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 4: stripe_call<4, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 5: stripe_call<5, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 6: stripe_call<6, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    case 7: stripe_call<7, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
+    default: stripe_call<8, MODE, PACK>(S, row0, P, first, inH, inE, outH, outE, want_out, &T); break;
     }
 }
 
@@ -335,7 +364,7 @@ __device__ void run_stripe(int R, bool direct, const Sweep& S, int row0, int P, 
 // writes the final row to finH/finE (indexed by column 1..ncols).
 struct Bufs { int *h0, *e0, *h1, *e1; };   // ping-pong stripe rows in global scratch
 
-template <int MODE>
+template <int MODE, bool PACK = false>
 __device__ void sweep(const Sweep& S, int rmax, const Bufs& g, int* finH, int* finE, Track& T)
 {
     const int nrows = S.nrows;
@@ -353,7 +382,7 @@ __device__ void sweep(const Sweep& S, int rmax, const Bufs& g, int* finH, int* f
         int* oH = last ? finH : ((s & 1) ? g.h0 : g.h1);
         int* oE = last ? finE : ((s & 1) ? g.e0 : g.e1);
         const int want = (!last || MODE == MODE_GLB) ? 1 : 0;
-        run_stripe<MODE>(R, direct, S, row0, P, s == 0, inH, inE, oH, oE, want, T);
+        run_stripe<MODE, PACK>(R, direct, S, row0, P, s == 0, inH, inE, oH, oE, want, T);
         wave_fence();
         if (MODE == MODE_REV && T.best) break;
         row0 += P;
@@ -911,8 +940,12 @@ __device__ void load_seq(lu8* dst, const uint8_t* src, int n, int pad, int scale
 // persistent loop that compiled to lanes 1..63 taking work item 0 and entering
 // the next pair without lane 0 after an empty sequence (the DP then reads
 // Myers-Miller stack entries that only lane 0 would have written).
+// Bits that hold 1..v (v >= 1).
+__device__ __forceinline__ int bits_for(int v) { return 32 - __builtin_clz((unsigned)v); }
+
+// maxent: the largest entry of the substitution matrix (bounds the forward scores).
 __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab,
-                                                  lu8* wlds)
+                                                  lu8* wlds, int maxent)
 {
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
     const bool in_set = (unsigned)si < (unsigned)A.nseq && (unsigned)sj < (unsigned)A.nseq;
@@ -958,6 +991,7 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
     S.tab = tab;
     S.go = go; S.ge = ge; S.goe = go + ge;
     S.thr = 0;
+    S.kb = 0;
 
     // forward local pass (:280-341)
     Sweep F = S;
@@ -966,8 +1000,14 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
     F.cs = s2; F.cb = 0; F.cd = 1;
     F.th0 = 0; F.ths = 0; F.teo = -go;
     F.lh0 = 0; F.lhs = 0; F.lfo = -go;
+    // packed tracking when every forward score (<= min(n,m) * maxent with
+    // non-negative gap penalties) fits above the column bits
+    F.kb = bits_for(m);
+    const bool packF = go >= 0 && ge >= 0 &&
+        (int64_t)(n < m ? n : m) * (int64_t)(maxent > 0 ? maxent : 0) < ((int64_t)1 << (31 - F.kb));
     Track T{0, 0};
-    sweep<MODE_FWD>(F, rmax, g, nullptr, nullptr, T);
+    if (packF) sweep<MODE_FWD, true>(F, rmax, g, nullptr, nullptr, T);
+    else sweep<MODE_FWD, false>(F, rmax, g, nullptr, nullptr, T);
     const int maxScore = uni(T.best);
     int se1 = 0, se2 = 0, sb1 = 1, sb2 = 1, status = 0;
     if (maxScore > 0) {
@@ -981,8 +1021,15 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
         Rv.th0 = -1; Rv.ths = 0; Rv.teo = 0;
         Rv.lh0 = -1; Rv.lhs = 0; Rv.lfo = 0;
         Rv.thr = maxScore;
+        // reverse scores lie in [-1 - se2 * ge, maxScore] (paths from the
+        // corner score at most maxScore, paths from the -1 border less)
+        Rv.kb = bits_for(se2);
+        const int64_t lim = ((int64_t)1 << (31 - Rv.kb)) - 1;
+        const bool packR = go >= 0 && ge >= 0 && (int64_t)maxScore < lim &&
+                           1 + (int64_t)(se1 + se2) * ge < lim;
         Track U{0, 0};
-        sweep<MODE_REV>(Rv, rmax, g, nullptr, nullptr, U);
+        if (packR) sweep<MODE_REV, true>(Rv, rmax, g, nullptr, nullptr, U);
+        else sweep<MODE_REV, false>(Rv, rmax, g, nullptr, nullptr, U);
         if (U.best) {
             const int r = uni(U.key >> 16), c = uni(U.key & 0xffff);
             sb1 = se1 - r + 1;
@@ -1013,6 +1060,14 @@ __device__ void load_table(const DevArgs& A, CW_LDS int* tab)
     __syncthreads();
 }
 
+// Largest matrix entry, every lane of the wave active.
+__device__ int table_max(lcu8* tab)
+{
+    int v = (int)0x80000000;
+    for (int k = lane_id(); k < 32 * 32; k += WAVE) v = max(v, ((CW_LDS const int*)tab)[k]);
+    return uni(wave_max(v));
+}
+
 } // namespace cwk
 
 using namespace cwk;
@@ -1029,7 +1084,7 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_plai
     const int64_t p = A.pair_base + slot;
     if (p >= A.pair_end) return;
     const int64_t q = A.order ? A.order[p] : p;
-    do_pair(A, q, (int)slot, tab, wlds);
+    do_pair(A, q, (int)slot, tab, wlds, table_max(tab));
 }
 
 extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_persistent(DevArgs A)
@@ -1041,6 +1096,7 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
     lcu8* tab = lds;
     lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
+    const int maxent = table_max(tab);
     for (;;) {
         // one atomic per pop (lane 0's); every lane reaches the broadcast, where
         // lane 0 is the first active lane
@@ -1049,7 +1105,7 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
         const int item = uni((int)got);
         if ((int64_t)item >= A.pair_end) break;
         const int64_t q = A.order ? A.order[item] : item;
-        do_pair(A, q, slot, tab, wlds);
+        do_pair(A, q, slot, tab, wlds, maxent);
     }
 }
 
