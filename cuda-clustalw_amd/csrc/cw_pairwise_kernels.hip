@@ -589,6 +589,147 @@ __device__ __attribute__((noinline)) void halves_cols_call(lcu8* tab, lcu8* s1, 
                       fE, rH, rE);
 }
 
+// ---- small column-path nodes, four per sweep ----------------------------------
+// A node with N <= kQ4Cols and M <= kQ4Rows leaves most of a 32-lane half idle
+// in mm_halves_cols (a ramp of nl - 1 steps around a few rows, one or two
+// columns per lane).  The depth-first walk queues up to four such nodes and
+// one sweep serves them all: lanes 16k .. 16k+15 take queued node k, lanes
+// 16k .. 16k+7 its forward half and 16k+8 .. 16k+15 its reverse half, C
+// columns per lane (N <= 8C), the recurrence of mm_halves_cols.  The final rows
+// of node k go to F/R + 2 * kQ4Stride * k (H at [j], E at [kQ4Stride + j]).
+constexpr int kQ4Cols = 63;
+constexpr int kQ4Rows = 96;
+constexpr int kQ4Stride = 64;          // ints per final-row array of a queued node (j <= 63)
+
+__device__ __forceinline__ bool pw_q4(int M, int N, int colcap)
+{
+    return M >= 2 && M <= kQ4Rows && N >= 1 && N <= kQ4Cols && N <= colcap;
+}
+
+template <int C>
+__device__ __attribute__((noinline)) void cols4_sweep(lcu8* tab, lcu8* s1, lcu8* s2, const int* q, int cnt,
+                                                      int go, int ge, int* F, int* R)
+{
+    const int lane = lane_id();
+    const int k = lane >> 4, grp = (lane >> 3) & 1, L = lane & 7;
+    const bool on = k < cnt;
+    const int* e = q + 6 * (on ? k : 0);
+    const int A = e[0], B = e[1], M = on ? e[2] : 0, N = on ? e[3] : 0;
+    const int midi = M / 2;
+    const int rows = grp ? (M - midi) : midi;
+    const int t0 = grp ? e[5] : e[4];
+    const int goe = go + ge;
+    const int rb = grp ? (A + M + 1) : A;      // code of half-row i: s1[rb + rd*i]
+    const int rd = grp ? -1 : 1;
+    const int nl = (N + C - 1) / C;            // lanes in use in this half
+    int b4[C], hup[C], hupg[C], ev[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;          // half-local column
+        const int jc = jj <= N ? jj : (N > 0 ? N : 1);
+        b4[c] = s2[grp ? (B + N + 1 - jc) : (B + jc)];
+        hup[c] = -t0 - jj * ge;                // top row H[0][jj]
+        ev[c] = hup[c] - go;
+        hupg[c] = hup[c] - goe;
+    }
+    int dleft = (L == 0) ? 0 : (-t0 - (L * C) * ge);   // H[0][L*C]; corner is 0
+    int hl_out = 0, fl_out = 0;
+    const int T = wave_max(rows > 0 && L < nl ? rows + nl - 1 : 0);
+    for (int t = 1; t <= T; ++t) {
+        int hl = shr1(0, hl_out);
+        int fl = shr1(0, fl_out);
+        const int i = t - L;
+        if (L == 0) {                          // left column H[i][0], F[i][0]
+            hl = -t0 - i * ge;
+            fl = hl - go;
+        }
+        if (i >= 1 && i <= rows && L < nl) {
+            const int a128 = (int)s1[rb + rd * i] * 128;
+            int d = dleft;
+            int hc = hl, fc = fl;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int sc = score_at(tab, a128 + b4[c]);
+                const int en = max(ev[c] - ge, hupg[c]);
+                const int fn = max(fc - ge, hc - goe);
+                const int h = max(max(d + sc, en), fn);
+                d = hup[c];
+                hup[c] = h;
+                hupg[c] = h - goe;
+                ev[c] = en;
+                hc = h;
+                fc = fn;
+            }
+            dleft = hl;
+            hl_out = hc;
+            fl_out = fc;
+        }
+    }
+    int* oH = (grp ? R : F) + 2 * kQ4Stride * k;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;
+        if (on && jj <= N) { oH[jj] = hup[c]; oH[kQ4Stride + jj] = ev[c]; }
+    }
+}
+
+// Reductions within each row of 16 lanes; every lane of a row holds the result.
+template <typename Op>
+__device__ __forceinline__ int row16_reduce(int x, Op op)
+{
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false));  // row_half_mirror
+    x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false));  // row_mirror
+    return x;
+}
+
+// The midpoints of the queued nodes (:570-593), node k in lanes 16k .. 16k+15:
+// returns type | midj << 2 in every lane of the node's row.
+__device__ __attribute__((noinline)) int cols4_midpoints(const int* q, int cnt, int go, int ge,
+                                                         const int* F, const int* R)
+{
+    const int lane = lane_id();
+    const int k = lane >> 4, l16 = lane & 15;
+    const bool on = k < cnt;
+    const int* e = q + 6 * (on ? k : 0);
+    const int M = on ? e[2] : 2, N = on ? e[3] : 1, tb = e[4], te = e[5];
+    const int midi = M / 2;
+    const int* fH = F + 2 * kQ4Stride * k;
+    const int* fE = fH + kQ4Stride;
+    const int* rH = R + 2 * kQ4Stride * k;
+    const int* rE = rH + kQ4Stride;
+    const int HH0 = -tb - midi * ge;
+    const int RRN = -te - (M - midi) * ge;
+    int vmax = -BIG, jfirst = BIG, jc = -1, wmax = -BIG, jw = -1;
+#pragma unroll
+    for (int it = 0; it < (kQ4Cols + 16) / 16; ++it) {
+        const int j = l16 + 16 * it;
+        if (on && j <= N) {
+            const int HHj = (j == 0) ? HH0 : fH[j];
+            const int DDj = (j == 0) ? HH0 : fE[j];
+            const int RRj = (j == N) ? RRN : rH[N - j];
+            const int SSj = (j == N) ? RRN : rE[N - j];
+            const int v = HHj + RRj;
+            const int w = DDj + SSj + go;
+            if (v > vmax) { vmax = v; jfirst = j; jc = -1; }
+            if (v == vmax && HHj != DDj && RRj == SSj) jc = j;
+            if (w >= wmax) { wmax = w; jw = j; }
+        }
+    }
+    auto mx = [](int a, int b) { return a > b ? a : b; };
+    auto mn = [](int a, int b) { return a < b ? a : b; };
+    const int Mv = row16_reduce(vmax, mx);
+    const int J0 = row16_reduce(vmax == Mv ? jfirst : BIG, mn);
+    const int JC = row16_reduce(vmax == Mv ? jc : -1, mx);
+    const int Wv = row16_reduce(wmax, mx);
+    const int JW = row16_reduce(wmax == Wv ? jw : -1, mx);
+    int type, midj;
+    if (Wv > Mv) { type = 2; midj = JW; }
+    else { type = 1; midj = J0 > JC ? J0 : JC; }
+    return type | (midj << 2);
+}
+
 // ---- tiny subtrees, one lane each ---------------------------------------------
 // A node with 2 <= M <= kTinyRows and N <= kTinyCols, and every M == 1 leaf, is
 // not worth a wave: the depth-first walk queues such nodes (two ints each) and
@@ -777,13 +918,61 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         nq = 0;
         wave_fence();
     };
+    // the top four stack entries hold the queue of small column-path nodes
+    int* const q4 = stack + 6 * (stack_cap - 4);
+    const int cap = stack_cap - 4;
+    int n4 = 0;
+    bool bad = false;
+    auto flush4 = [&]() {
+        const int cnt = n4;
+        int mN = 1;
+        for (int kk = 0; kk < cnt; ++kk) mN = max(mN, uni(q4[6 * kk + 3]));
+        switch ((mN + 7) >> 3) {                         // columns per lane: N <= 8C
+        case 1: cols4_sweep<1>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 2: cols4_sweep<2>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 3: cols4_sweep<3>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 4: cols4_sweep<4>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 5: cols4_sweep<5>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 6: cols4_sweep<6>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        case 7: cols4_sweep<7>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        default: cols4_sweep<8>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        }
+        wave_fence();
+        const int tm = cols4_midpoints(q4, cnt, go, ge, smallF, smallR);
+        for (int kk = 0; kk < cnt; ++kk) {
+            const int v = __builtin_amdgcn_readlane(tm, 16 * kk);
+            const int type = v & 3, midj = v >> 2;
+            const int* e = q4 + 6 * kk;
+            const int A = uni(e[0]), B = uni(e[1]), M = uni(e[2]), N = uni(e[3]);
+            const int tb = uni(e[4]), te = uni(e[5]);
+            const int midi = M / 2;
+            if (sp + 2 > cap) { bad = true; break; }
+            int* c0 = stack + 6 * sp;                    // every lane stores the same values
+            int* c1 = stack + 6 * (sp + 1);
+            if (type == 1) {
+                c0[0] = A; c0[1] = B; c0[2] = midi; c0[3] = midj; c0[4] = tb; c0[5] = go;
+                c1[0] = A + midi; c1[1] = B + midj; c1[2] = M - midi; c1[3] = N - midj; c1[4] = go; c1[5] = te;
+            } else {
+                c0[0] = A; c0[1] = B; c0[2] = midi - 1; c0[3] = midj; c0[4] = tb; c0[5] = 0;
+                c1[0] = A + midi + 1; c1[1] = B + midj; c1[2] = M - midi - 1; c1[3] = N - midj; c1[4] = 0; c1[5] = te;
+            }
+            sp += 2;
+        }
+        n4 = 0;
+        wave_fence();
+    };
     {
         int* e = stack;                                  // every lane stores the same values
         e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
     }
     sp = 1;
     wave_fence();
-    while (sp > 0) {
+    while (sp > 0 || n4 > 0) {
+        if (sp == 0) {                                   // only queued nodes left
+            flush4();
+            if (bad) { atomicOr(status, 2); break; }
+            continue;
+        }
         --sp;
         const int* e = stack + 6 * sp;
         const int A = uni(e[0]), B = uni(e[1]), M = uni(e[2]), N = uni(e[3]);
@@ -823,6 +1012,17 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         }
         ++nodes;
         cells += M * N;
+        if (pw_q4(M, N, colcap)) {
+            int* qe = q4 + 6 * n4;                       // same value from every lane
+            qe[0] = A; qe[1] = B; qe[2] = M; qe[3] = N; qe[4] = tb; qe[5] = te;
+            n4 += 1;
+            wave_fence();
+            if (n4 == 4) {
+                flush4();
+                if (bad) { atomicOr(status, 2); break; }
+            }
+            continue;
+        }
         const int midi = M / 2;
         int *fH, *fE, *rH, *rE;          // HH/DD[j] at [j]; RR/SS[j] at [N-j]
         if (N <= colcap) {
@@ -898,7 +1098,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         midj = uni(midj);
         type = uni(type);
         wave_fence();
-        if (sp + 2 > stack_cap) { atomicOr(status, 2); break; }
+        if (sp + 2 > cap) { atomicOr(status, 2); break; }
         {
             int* c0 = stack + 6 * sp;
             int* c1 = stack + 6 * (sp + 1);
