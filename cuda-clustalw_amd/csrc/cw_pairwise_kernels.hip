@@ -24,6 +24,7 @@
 //     is order-free, so the recursion runs as a depth-first stack in LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "cw_pairwise_kernels.h"
 
@@ -589,29 +590,35 @@ __device__ __attribute__((noinline)) void halves_cols_call(lcu8* tab, lcu8* s1, 
                       fE, rH, rE);
 }
 
-// ---- small column-path nodes, four per sweep ----------------------------------
-// A node with N <= kQ4Cols and M <= kQ4Rows leaves most of a 32-lane half idle
-// in mm_halves_cols (a ramp of nl - 1 steps around a few rows, one or two
-// columns per lane).  The depth-first walk queues up to four such nodes and
-// one sweep serves them all: lanes 16k .. 16k+15 take queued node k, lanes
-// 16k .. 16k+7 its forward half and 16k+8 .. 16k+15 its reverse half, C
-// columns per lane (N <= 8C), the recurrence of mm_halves_cols.  The final rows
-// of node k go to F/R + 2 * kQ4Stride * k (H at [j], E at [kQ4Stride + j]).
-constexpr int kQ4Cols = 63;
-constexpr int kQ4Rows = 96;
-constexpr int kQ4Stride = 64;          // ints per final-row array of a queued node (j <= 63)
+// ---- small column-path nodes, two or four per sweep ---------------------------
+// A node with few columns leaves most of a 32-lane half idle in
+// mm_halves_cols (a ramp of nl - 1 steps around a few rows, one or two columns
+// per lane).  The depth-first walk queues such nodes, G at a time (G = 4 for
+// N <= 63, G = 2 for 64 <= N <= 127), and one sweep serves all G: lanes
+// (64/G)k .. (64/G)(k+1)-1 take queued node k, the first half of them its
+// forward half and the second half its reverse half, C columns per lane
+// (N <= 32C/G), the recurrence of mm_halves_cols.  The final rows of node k go
+// to F/R + 2 * (256/G) * k (H at [j], E at [256/G + j]).
+constexpr int kQ4Cols = 63, kQ4Rows = 96;
+constexpr int kQ2Cols = 127, kQ2Rows = 192;
 
 __device__ __forceinline__ bool pw_q4(int M, int N, int colcap)
 {
     return M >= 2 && M <= kQ4Rows && N >= 1 && N <= kQ4Cols && N <= colcap;
 }
+__device__ __forceinline__ bool pw_q2(int M, int N, int colcap)
+{
+    return M >= 2 && M <= kQ2Rows && N > kQ4Cols && N <= kQ2Cols && N <= colcap;
+}
 
-template <int C>
-__device__ __attribute__((noinline)) void cols4_sweep(lcu8* tab, lcu8* s1, lcu8* s2, const int* q, int cnt,
+template <int G, int C>
+__device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8* s2, const int* q, int cnt,
                                                       int go, int ge, int* F, int* R)
 {
+    constexpr int LPH = 32 / G;                // lanes per half
+    constexpr int STRIDE = 256 / G;
     const int lane = lane_id();
-    const int k = lane >> 4, grp = (lane >> 3) & 1, L = lane & 7;
+    const int k = lane / (2 * LPH), grp = (lane / LPH) & 1, L = lane % LPH;
     const bool on = k < cnt;
     const int* e = q + 6 * (on ? k : 0);
     const int A = e[0], B = e[1], M = on ? e[2] : 0, N = on ? e[3] : 0;
@@ -665,46 +672,51 @@ __device__ __attribute__((noinline)) void cols4_sweep(lcu8* tab, lcu8* s1, lcu8*
             fl_out = fc;
         }
     }
-    int* oH = (grp ? R : F) + 2 * kQ4Stride * k;
+    int* oH = (grp ? R : F) + 2 * STRIDE * k;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int jj = L * C + c + 1;
-        if (on && jj <= N) { oH[jj] = hup[c]; oH[kQ4Stride + jj] = ev[c]; }
+        if (on && jj <= N) { oH[jj] = hup[c]; oH[STRIDE + jj] = ev[c]; }
     }
 }
 
-// Reductions within each row of 16 lanes; every lane of a row holds the result.
-template <typename Op>
-__device__ __forceinline__ int row16_reduce(int x, Op op)
+// Reductions within each group of 64/G lanes; every lane of a group holds the
+// result (rows of 16 by DPP, then the two rows of a 32-lane group by a swizzle).
+template <int G, typename Op>
+__device__ __forceinline__ int group_reduce(int x, Op op)
 {
     x = op(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xf, 0xf, false));   // quad_perm [1,0,3,2]
     x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xf, 0xf, false));   // quad_perm [2,3,0,1]
     x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false));  // row_half_mirror
     x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false));  // row_mirror
+    if (G == 2) x = op(x, __shfl_xor(x, 16));
     return x;
 }
 
-// The midpoints of the queued nodes (:570-593), node k in lanes 16k .. 16k+15:
-// returns type | midj << 2 in every lane of the node's row.
-__device__ __attribute__((noinline)) int cols4_midpoints(const int* q, int cnt, int go, int ge,
+// The midpoints of the queued nodes (:570-593), node k in lanes
+// (64/G)k .. (64/G)(k+1)-1: returns type | midj << 2 in every lane of the group.
+template <int G>
+__device__ __attribute__((noinline)) int colsG_midpoints(const int* q, int cnt, int go, int ge,
                                                          const int* F, const int* R)
 {
+    constexpr int W = 64 / G;                  // lanes per node
+    constexpr int STRIDE = 256 / G;
     const int lane = lane_id();
-    const int k = lane >> 4, l16 = lane & 15;
+    const int k = lane / W, lw = lane % W;
     const bool on = k < cnt;
     const int* e = q + 6 * (on ? k : 0);
     const int M = on ? e[2] : 2, N = on ? e[3] : 1, tb = e[4], te = e[5];
     const int midi = M / 2;
-    const int* fH = F + 2 * kQ4Stride * k;
-    const int* fE = fH + kQ4Stride;
-    const int* rH = R + 2 * kQ4Stride * k;
-    const int* rE = rH + kQ4Stride;
+    const int* fH = F + 2 * STRIDE * k;
+    const int* fE = fH + STRIDE;
+    const int* rH = R + 2 * STRIDE * k;
+    const int* rE = rH + STRIDE;
     const int HH0 = -tb - midi * ge;
     const int RRN = -te - (M - midi) * ge;
     int vmax = -BIG, jfirst = BIG, jc = -1, wmax = -BIG, jw = -1;
 #pragma unroll
-    for (int it = 0; it < (kQ4Cols + 16) / 16; ++it) {
-        const int j = l16 + 16 * it;
+    for (int it = 0; it < STRIDE / W; ++it) {
+        const int j = lw + W * it;
         if (on && j <= N) {
             const int HHj = (j == 0) ? HH0 : fH[j];
             const int DDj = (j == 0) ? HH0 : fE[j];
@@ -719,11 +731,11 @@ __device__ __attribute__((noinline)) int cols4_midpoints(const int* q, int cnt, 
     }
     auto mx = [](int a, int b) { return a > b ? a : b; };
     auto mn = [](int a, int b) { return a < b ? a : b; };
-    const int Mv = row16_reduce(vmax, mx);
-    const int J0 = row16_reduce(vmax == Mv ? jfirst : BIG, mn);
-    const int JC = row16_reduce(vmax == Mv ? jc : -1, mx);
-    const int Wv = row16_reduce(wmax, mx);
-    const int JW = row16_reduce(wmax == Wv ? jw : -1, mx);
+    const int Mv = group_reduce<G>(vmax, mx);
+    const int J0 = group_reduce<G>(vmax == Mv ? jfirst : BIG, mn);
+    const int JC = group_reduce<G>(vmax == Mv ? jc : -1, mx);
+    const int Wv = group_reduce<G>(wmax, mx);
+    const int JW = group_reduce<G>(wmax == Wv ? jw : -1, mx);
     int type, midj;
     if (Wv > Mv) { type = 2; midj = JW; }
     else { type = 1; midj = J0 > JC ? J0 : JC; }
@@ -918,31 +930,36 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         nq = 0;
         wave_fence();
     };
-    // the top four stack entries hold the queue of small column-path nodes
-    int* const q4 = stack + 6 * (stack_cap - 4);
-    const int cap = stack_cap - 4;
-    int n4 = 0;
+    // the top six stack entries hold the queues of small column-path nodes
+    // (four entries for N <= 63, two for 64 <= N <= 127)
+    int* const q4 = stack + 6 * (stack_cap - 6);
+    int* const q2 = stack + 6 * (stack_cap - 2);
+    const int cap = stack_cap - 6;
+    int n4 = 0, n2 = 0;
     bool bad = false;
-    auto flush4 = [&]() {
-        const int cnt = n4;
+    auto flushG = [&](auto gtag) {
+        constexpr int G = decltype(gtag)::value;
+        int* const q = G == 4 ? q4 : q2;
+        const int cnt = G == 4 ? n4 : n2;
         int mN = 1;
-        for (int kk = 0; kk < cnt; ++kk) mN = max(mN, uni(q4[6 * kk + 3]));
-        switch ((mN + 7) >> 3) {                         // columns per lane: N <= 8C
-        case 1: cols4_sweep<1>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 2: cols4_sweep<2>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 3: cols4_sweep<3>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 4: cols4_sweep<4>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 5: cols4_sweep<5>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 6: cols4_sweep<6>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        case 7: cols4_sweep<7>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
-        default: cols4_sweep<8>(base.tab, s1, s2, q4, cnt, go, ge, smallF, smallR); break;
+        for (int kk = 0; kk < cnt; ++kk) mN = max(mN, uni(q[6 * kk + 3]));
+        constexpr int LPH = 32 / G;
+        switch ((mN + LPH - 1) / LPH) {                  // columns per lane: N <= C * lanes per half
+        case 1: colsG_sweep<G, 1>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 2: colsG_sweep<G, 2>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 3: colsG_sweep<G, 3>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 4: colsG_sweep<G, 4>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 5: colsG_sweep<G, 5>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 6: colsG_sweep<G, 6>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 7: colsG_sweep<G, 7>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        default: colsG_sweep<G, 8>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
         }
         wave_fence();
-        const int tm = cols4_midpoints(q4, cnt, go, ge, smallF, smallR);
+        const int tm = colsG_midpoints<G>(q, cnt, go, ge, smallF, smallR);
         for (int kk = 0; kk < cnt; ++kk) {
-            const int v = __builtin_amdgcn_readlane(tm, 16 * kk);
+            const int v = __builtin_amdgcn_readlane(tm, (64 / G) * kk);
             const int type = v & 3, midj = v >> 2;
-            const int* e = q4 + 6 * kk;
+            const int* e = q + 6 * kk;
             const int A = uni(e[0]), B = uni(e[1]), M = uni(e[2]), N = uni(e[3]);
             const int tb = uni(e[4]), te = uni(e[5]);
             const int midi = M / 2;
@@ -958,18 +975,21 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             }
             sp += 2;
         }
-        n4 = 0;
+        if (G == 4) n4 = 0; else n2 = 0;
         wave_fence();
     };
+    using G4 = std::integral_constant<int, 4>;
+    using G2 = std::integral_constant<int, 2>;
     {
         int* e = stack;                                  // every lane stores the same values
         e[0] = sb1 - 1; e[1] = sb2 - 1; e[2] = se1 - sb1 + 1; e[3] = se2 - sb2 + 1; e[4] = 0; e[5] = 0;
     }
     sp = 1;
     wave_fence();
-    while (sp > 0 || n4 > 0) {
+    while (sp > 0 || n4 > 0 || n2 > 0) {
         if (sp == 0) {                                   // only queued nodes left
-            flush4();
+            if (n4 > 0) flushG(G4{});
+            else flushG(G2{});
             if (bad) { atomicOr(status, 2); break; }
             continue;
         }
@@ -1018,7 +1038,18 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             n4 += 1;
             wave_fence();
             if (n4 == 4) {
-                flush4();
+                flushG(G4{});
+                if (bad) { atomicOr(status, 2); break; }
+            }
+            continue;
+        }
+        if (pw_q2(M, N, colcap)) {
+            int* qe = q2 + 6 * n2;                       // same value from every lane
+            qe[0] = A; qe[1] = B; qe[2] = M; qe[3] = N; qe[4] = tb; qe[5] = te;
+            n2 += 1;
+            wave_fence();
+            if (n2 == 2) {
+                flushG(G2{});
                 if (bad) { atomicOr(status, 2); break; }
             }
             continue;
