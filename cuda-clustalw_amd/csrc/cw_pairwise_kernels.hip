@@ -1174,9 +1174,34 @@ __device__ void load_seq(lu8* dst, const uint8_t* src, int n, int pad, int scale
 // Bits that hold 1..v (v >= 1).
 __device__ __forceinline__ int bits_for(int v) { return 32 - __builtin_clz((unsigned)v); }
 
-// maxent: the largest entry of the substitution matrix (bounds the forward scores).
+// Upper bound of every forward score of the pair: an alignment pairs each
+// residue of a sequence at most once and (with non-negative gap penalties)
+// gaps only subtract, so the score is at most the sum over seq1 of the
+// largest entry of each residue's matrix row, and likewise over seq2's
+// columns.  rcmax: lane c < 32 holds max(row c), lane 32 + c max(column c),
+// clamped to [0, 32767] (the caller refuses packing when a clamp bit), so the
+// per-lane sums (<= 512 residues each) and their total stay below 2^31.
+__device__ int score_bound(lcu8* s1, lcu8* s2, int n, int m, int rcmax)
+{
+    const int lane = lane_id();
+    int a1 = 0, a2 = 0;
+    for (int b = 0; b < n || b < m; b += WAVE) {        // uniform trip count
+        const int k = b + 1 + lane;
+        const int c1 = k <= n ? (int)s1[k] : 0;
+        const int c2 = k <= m ? ((int)s2[k] >> 2) : 0;
+        const int r1 = __shfl(rcmax, c1);
+        const int r2 = __shfl(rcmax, 32 + c2);
+        a1 += k <= n ? r1 : 0;
+        a2 += k <= m ? r2 : 0;
+    }
+    const int S1 = wave_sum(a1), S2 = wave_sum(a2);
+    return S1 < S2 ? S1 : S2;
+}
+
+// rcmax: row and column maxima of the matrix (see score_bound), rcfit: none
+// of them was clamped.
 __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, int slot, lcu8* tab,
-                                                  lu8* wlds, int maxent)
+                                                  lu8* wlds, int rcmax, int rcfit)
 {
     const int si = uni(A.pairs[2 * p]), sj = uni(A.pairs[2 * p + 1]);
     const bool in_set = (unsigned)si < (unsigned)A.nseq && (unsigned)sj < (unsigned)A.nseq;
@@ -1231,11 +1256,11 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
     F.cs = s2; F.cb = 0; F.cd = 1;
     F.th0 = 0; F.ths = 0; F.teo = -go;
     F.lh0 = 0; F.lhs = 0; F.lfo = -go;
-    // packed tracking when every forward score (<= min(n,m) * maxent with
+    // packed tracking when every forward score (at most score_bound with
     // non-negative gap penalties) fits above the column bits
     F.kb = bits_for(m);
-    const bool packF = go >= 0 && ge >= 0 &&
-        (int64_t)(n < m ? n : m) * (int64_t)(maxent > 0 ? maxent : 0) < ((int64_t)1 << (31 - F.kb));
+    const bool packF = go >= 0 && ge >= 0 && rcfit &&
+        uni(score_bound(s1, s2, n, m, rcmax)) < (1 << (31 - F.kb));
     Track T{0, 0};
     if (packF) sweep<MODE_FWD, true>(F, rmax, g, nullptr, nullptr, T);
     else sweep<MODE_FWD, false>(F, rmax, g, nullptr, nullptr, T);
@@ -1291,12 +1316,17 @@ __device__ void load_table(const DevArgs& A, CW_LDS int* tab)
     __syncthreads();
 }
 
-// Largest matrix entry, every lane of the wave active.
-__device__ int table_max(lcu8* tab)
+// Row maxima (lanes 0..31) and column maxima (lanes 32..63) of the matrix,
+// clamped to [0, 32767]; every lane of the wave active.  *fit: no clamp from above.
+__device__ int table_rcmax(lcu8* tab, int* fit)
 {
+    const int lane = lane_id();
+    const int c = lane & 31;
+    const CW_LDS int* t = (const CW_LDS int*)tab;
     int v = (int)0x80000000;
-    for (int k = lane_id(); k < 32 * 32; k += WAVE) v = max(v, ((CW_LDS const int*)tab)[k]);
-    return uni(wave_max(v));
+    for (int k = 0; k < 32; ++k) v = max(v, lane < 32 ? t[c * 32 + k] : t[k * 32 + c]);
+    *fit = uni(wave_max(v)) <= 32767;
+    return v < 0 ? 0 : (v > 32767 ? 32767 : v);
 }
 
 } // namespace cwk
@@ -1315,7 +1345,9 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_plai
     const int64_t p = A.pair_base + slot;
     if (p >= A.pair_end) return;
     const int64_t q = A.order ? A.order[p] : p;
-    do_pair(A, q, (int)slot, tab, wlds, table_max(tab));
+    int rcfit;
+    const int rcmax = table_rcmax(tab, &rcfit);
+    do_pair(A, q, (int)slot, tab, wlds, rcmax, rcfit);
 }
 
 extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_persistent(DevArgs A)
@@ -1327,7 +1359,8 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
     lcu8* tab = lds;
     lu8* wlds = lds + CW_TAB_BYTES + wave * A.wave_lds;
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
-    const int maxent = table_max(tab);
+    int rcfit;
+    const int rcmax = table_rcmax(tab, &rcfit);
     for (;;) {
         // one atomic per pop (lane 0's); every lane reaches the broadcast, where
         // lane 0 is the first active lane
@@ -1336,7 +1369,7 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
         const int item = uni((int)got);
         if ((int64_t)item >= A.pair_end) break;
         const int64_t q = A.order ? A.order[item] : item;
-        do_pair(A, q, slot, tab, wlds, maxent);
+        do_pair(A, q, slot, tab, wlds, rcmax, rcfit);
     }
 }
 
