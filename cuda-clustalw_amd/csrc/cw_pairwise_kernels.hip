@@ -120,13 +120,13 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 // lanes >= P idle and lane P-1 writes its row straight to out (only ever the
 // single stripe of a sweep, so its top boundary is the formula row).
 //
-// PACK (MODE_FWD / MODE_REV, when the pair's scores fit beside S.kb column
-// bits): a row's tracked cell is one int, max over (h << kb) | (2^kb-1 - c),
-// i.e. the largest h and among equal h the smallest column -- one shift-or
-// and one max per cell instead of a compare and two selects.  MODE_FWD with
-// PACK also clamps E and F at 0: with ge >= 0 that gives max(E, 0) and
-// max(F, 0) at every cell (induction over E' = max(E - ge, H - go - ge, 0))
-// and leaves every H unchanged, so H = max3(diag + s, E, F) needs no clamp.
+// PACK (MODE_FWD, when the pair's scores fit beside S.kb column bits): a
+// row's tracked cell is one int, max over (h << kb) | (2^kb-1 - c), i.e. the
+// largest h and among equal h the smallest column -- one shift-or and one max
+// per cell instead of a compare and two selects.  PACK also clamps E and F at
+// 0: with ge >= 0 that gives max(E, 0) and max(F, 0) at every cell (induction
+// over E' = max(E - ge, H - go - ge, 0)) and leaves every H unchanged, so
+// H = max3(diag + s, E, F) needs no clamp.
 template <int R, int MODE, bool DIRECT, bool PACK>
 __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int first,
                                        const int* inH, const int* inE, int* outH,
@@ -154,7 +154,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         hL[r] = left_h(S, row);
         f[r] = hL[r] + S.lfo;
         hLg[r] = hL[r] - S.goe;
-        tv[r] = (PACK && MODE == MODE_REV) ? (int)0x80000000 : 0;
+        tv[r] = 0;
         tj[r] = BIG;
     }
     int dprev = left_h(S, firstrow - 1);
@@ -235,12 +235,22 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 hup = h;
                 hupg = hLg[r];
                 eup = e;
-                if (PACK && MODE != MODE_GLB) {
+                if (PACK && MODE == MODE_FWD) {
                     tv[r] = max(tv[r], (int)(((unsigned)h << kb) | (unsigned)ck));
                 } else if (MODE == MODE_FWD) {
                     if (h > tv[r]) { tv[r] = h; tj[r] = c; }
-                } else if (MODE == MODE_REV) {
-                    tj[r] = min(tj[r], h >= thr ? c : BIG);
+                }
+            }
+            if (MODE == MODE_REV) {
+                // no reverse score exceeds maxScore, so a hit is h == thr, which
+                // few cells reach: one max over the lane's rows per step, and
+                // the columns are recorded only in a step where some row hits
+                int mrow = hL[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) mrow = max(mrow, hL[r]);
+                if (mrow >= thr) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) tj[r] = min(tj[r], hL[r] >= thr ? c : BIG);
                 }
             }
             outHv = hup;
@@ -321,9 +331,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         for (int r = 0; r < R; ++r) {
             const int row = firstrow + r - d;
             const bool valid = lane_on && !(r == 0 && dummy);
-            const bool hit = PACK ? tv[r] >= (int)((unsigned)thr << kb) : tj[r] != BIG;
-            const int tjr = PACK ? cmask - (tv[r] & cmask) : tj[r];
-            if (valid && hit) key = min(key, (row << 16) | tjr);
+            if (valid && tj[r] != BIG) key = min(key, (row << 16) | tj[r]);
         }
         const int K = wave_min(key);
         if (K != BIG) { T.best = 1; T.key = K; }
@@ -1277,15 +1285,8 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
         Rv.th0 = -1; Rv.ths = 0; Rv.teo = 0;
         Rv.lh0 = -1; Rv.lhs = 0; Rv.lfo = 0;
         Rv.thr = maxScore;
-        // reverse scores lie in [-1 - se2 * ge, maxScore] (paths from the
-        // corner score at most maxScore, paths from the -1 border less)
-        Rv.kb = bits_for(se2);
-        const int64_t lim = ((int64_t)1 << (31 - Rv.kb)) - 1;
-        const bool packR = go >= 0 && ge >= 0 && (int64_t)maxScore < lim &&
-                           1 + (int64_t)(se1 + se2) * ge < lim;
         Track U{0, 0};
-        if (packR) sweep<MODE_REV, true>(Rv, rmax, g, nullptr, nullptr, U);
-        else sweep<MODE_REV, false>(Rv, rmax, g, nullptr, nullptr, U);
+        sweep<MODE_REV, false>(Rv, rmax, g, nullptr, nullptr, U);
         if (U.best) {
             const int r = uni(U.key >> 16), c = uni(U.key & 0xffff);
             sb1 = se1 - r + 1;
