@@ -127,6 +127,13 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 // 0: with ge >= 0 that gives max(E, 0) and max(F, 0) at every cell (induction
 // over E' = max(E - ge, H - go - ge, 0)) and leaves every H unchanged, so
 // H = max3(diag + s, E, F) needs no clamp.
+//
+// PACK with MODE_GLB: the stripe carries every H, E and F of cell (i, j) as
+// X + (i + j) * ge (i, j the sweep's own row and column).  In that frame both
+// gap recurrences lose their extension term, E' = max(E~, H~ - go) and
+// F' = max(F~, H~ - go) share one H~ - go, and the diagonal carries + 2 ge,
+// kept in the stored H; the top input row and the bottom output row convert
+// once per 64-column chunk.  One add per cell fewer, values identical.
 template <int R, int MODE, bool DIRECT, bool PACK>
 __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int first,
                                        const int* inH, const int* inE, int* outH,
@@ -145,6 +152,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     const int d = dummy ? 1 : 0;
     const int firstrow = row0 + 1 + (lane < k1 ? lane * R : k1 * R + (lane - k1) * (R - 1));
 
+    constexpr bool SH = PACK && MODE == MODE_GLB;        // shifted frame
+    const int ge = uni(S.ge), goe = uni(S.goe), thr = uni(S.thr);
+    const int go = uni(S.go), ge2 = 2 * ge;
     int sb[R], hL[R], hLg[R], f[R], tj[R], tv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -154,11 +164,17 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         hL[r] = left_h(S, row);
         f[r] = hL[r] + S.lfo;
         hLg[r] = hL[r] - S.goe;
+        if (SH) {
+            const int sh = row * ge;                      // column 0
+            f[r] += sh;
+            hLg[r] = hL[r] + sh - go;
+            hL[r] += sh + ge2;
+        }
         tv[r] = 0;
         tj[r] = BIG;
     }
     int dprev = left_h(S, firstrow - 1);
-    const int ge = uni(S.ge), goe = uni(S.goe), thr = uni(S.thr);
+    if (SH) dprev += (firstrow - 1) * ge + ge2;
     const int kb = uni(S.kb), cmask = (1 << kb) - 1;
     // 0 in a scalar register the compiler cannot see through: with a literal
     // 0 it proves E, F >= 0, turns the H maximum into a signed and an unsigned
@@ -181,6 +197,11 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         } else {
             h = 0;
             e = 0;
+        }
+        if (SH) {
+            const int sh = (row0 + col) * ge;
+            h += sh;
+            e += sh;
         }
     };
     int cinH, cinE, nxtH = 0, nxtE = 0;
@@ -210,8 +231,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
         if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= ncols;
         if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
-            int hupg = inHv - goe;
-            dprev = inHv;
+            int hupg = SH ? inHv - go : inHv - goe;
+            dprev = SH ? inHv + ge2 : inHv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int s = score_at(tab, sb[r] + bcur);
@@ -219,6 +240,10 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 if (MODE == MODE_FWD && PACK) {
                     e = max(max(eup - ge, hupg), zero);
                     f[r] = max(max(f[r] - ge, hLg[r]), zero);
+                    h = max(max(dg + s, e), f[r]);
+                } else if (SH) {
+                    e = max(eup, hupg);
+                    f[r] = max(f[r], hLg[r]);
                     h = max(max(dg + s, e), f[r]);
                 } else {
                     e = max(eup - ge, hupg);
@@ -230,8 +255,8 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                     if (dummy) { h = hup; e = eup; }
                 }
                 dg = hL[r];
-                hL[r] = h;
-                hLg[r] = h - goe;
+                hL[r] = SH ? h + ge2 : h;
+                hLg[r] = SH ? h - go : h - goe;
                 hup = h;
                 hupg = hLg[r];
                 eup = e;
@@ -256,8 +281,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
             outHv = hup;
             outEv = eup;
             if (DIRECT && want_out && lane == P - 1) {
-                ((gint*)outH)[c] = hup;
-                ((gint*)outE)[c] = eup;
+                const int sh = SH ? (row0 + P + c) * ge : 0;
+                ((gint*)outH)[c] = hup - sh;
+                ((gint*)outE)[c] = eup - sh;
             }
         }
         if (!DIRECT && want_out) {
@@ -268,8 +294,9 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
-                    ((gint*)outH)[col] = coutH;
-                    ((gint*)outE)[col] = coutE;
+                    const int sh = SH ? (row0 + P + col) * ge : 0;
+                    ((gint*)outH)[col] = coutH - sh;
+                    ((gint*)outE)[col] = coutE - sh;
                 }
             }
         }
@@ -1100,7 +1127,10 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             F.th0 = -tb; F.ths = -ge; F.teo = -go;
             F.lh0 = -tb; F.lhs = -ge; F.lfo = -go;
             fH = g2H; fE = g2E;
-            sweep<MODE_GLB>(F, rmax, g, fH, fE, T);
+            // shifted frame while (i + j) * ge stays small beside the scores
+            const bool shf = (int64_t)(M + N + 2) * (ge < 0 ? -(int64_t)ge : (int64_t)ge) < (1 << 26);
+            if (shf) sweep<MODE_GLB, true>(F, rmax, g, fH, fE, T);
+            else sweep<MODE_GLB, false>(F, rmax, g, fH, fE, T);
             Sweep Rv = base;
             Rv.nrows = M - midi; Rv.ncols = N;
             Rv.rs = s1; Rv.rb = A + M + 1; Rv.rd = -1;
@@ -1108,7 +1138,8 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             Rv.th0 = -te; Rv.ths = -ge; Rv.teo = -go;
             Rv.lh0 = -te; Rv.lhs = -ge; Rv.lfo = -go;
             rH = g3H; rE = g3E;
-            sweep<MODE_GLB>(Rv, rmax, g, rH, rE, T);
+            if (shf) sweep<MODE_GLB, true>(Rv, rmax, g, rH, rE, T);
+            else sweep<MODE_GLB, false>(Rv, rmax, g, rH, rE, T);
         }
         wave_fence();
         // midpoint (:570-593)
