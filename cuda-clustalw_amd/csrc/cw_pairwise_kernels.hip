@@ -434,7 +434,7 @@ struct Node { int A, B, M, N, tb, te; };
 // Lane l of a half works on its row t-l at step t; the (H,F) of its last
 // column moves to the next lane by DPP.  Writes HH/DD[j] to fH/fE[j] and
 // RR/SS[j] to rH/rE[N-j] (j = 1..N and 0..N-1), the layout of the row path.
-template <int C>
+template <int C, bool SH>
 __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B, int M,
                                                int N, int tb, int te, int go, int ge, int* fH,
                                                int* fE, int* rH, int* rE)
@@ -458,8 +458,14 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
         hup[c] = -t0 - jj * ge;                // top row H[0][jj]
         e[c] = hup[c] - go;
         hupg[c] = hup[c] - goe;
+        if (SH) {                              // cell (i, jj) carried as X + (i + jj) ge
+            hupg[c] = -t0 - go;
+            e[c] = -t0 - go;
+            hup[c] = -t0 + 2 * ge;             // stored with the diagonal's + 2 ge
+        }
     }
     int dleft = (L == 0) ? 0 : (-t0 - (L * C) * ge);   // H[0][L*C]; corner is 0
+    if (SH) dleft = (L == 0 ? 0 : -t0) + 2 * ge;
     int hl_out = 0, fl_out = 0;
     int raddr = rb + rd * (1 - L);
     const int T = (midi > M - midi ? midi : M - midi) + nl - 1;
@@ -468,7 +474,7 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
         int fl = shr1(0, fl_out);
         const int i = t - L;
         if (L == 0) {                          // left column H[i][0], F[i][0]
-            hl = -t0 - i * ge;
+            hl = SH ? -t0 : -t0 - i * ge;
             fl = hl - go;
         }
         const int a128 = (int)s1[raddr] * 128;
@@ -476,20 +482,34 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
         if (i >= 1 && i <= rows && L < nl) {
             int d = dleft;
             int hc = hl, fc = fl;
+            int hcg = SH ? hl - go : 0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int sc = score_at(tab, a128 + b4[c]);
-                const int en = max(e[c] - ge, hupg[c]);
-                const int fn = max(fc - ge, hc - goe);
-                const int h = max(max(d + sc, en), fn);
-                d = hup[c];
-                hup[c] = h;
-                hupg[c] = h - goe;
-                e[c] = en;
-                hc = h;
-                fc = fn;
+                if (SH) {
+                    const int en = max(e[c], hupg[c]);
+                    const int fn = max(fc, hcg);
+                    const int h = max(max(d + sc, en), fn);
+                    d = hup[c];
+                    hup[c] = h + 2 * ge;
+                    hupg[c] = h - go;
+                    hcg = hupg[c];
+                    e[c] = en;
+                    hc = h;
+                    fc = fn;
+                } else {
+                    const int en = max(e[c] - ge, hupg[c]);
+                    const int fn = max(fc - ge, hc - goe);
+                    const int h = max(max(d + sc, en), fn);
+                    d = hup[c];
+                    hup[c] = h;
+                    hupg[c] = h - goe;
+                    e[c] = en;
+                    hc = h;
+                    fc = fn;
+                }
             }
-            dleft = hl;
+            dleft = SH ? hl + 2 * ge : hl;
             hl_out = hc;
             fl_out = fc;
         }
@@ -498,8 +518,10 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
     for (int c = 0; c < C; ++c) {
         const int jj = L * C + c + 1;
         if (jj <= N) {
-            if (grp == 0) { fH[jj] = hup[c]; fE[jj] = e[c]; }
-            else { rH[jj] = hup[c]; rE[jj] = e[c]; }
+            const int sh = SH ? (rows + jj) * ge : 0;
+            const int hv = SH ? hup[c] - 2 * ge - sh : hup[c];
+            if (grp == 0) { fH[jj] = hv; fE[jj] = e[c] - sh; }
+            else { rH[jj] = hv; rE[jj] = e[c] - sh; }
         }
     }
 }
@@ -515,7 +537,7 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
 // top row from its formula; the last lane of each half writes the half's final
 // row (H and E at each column, indexed by the half's own column) to fH/fE and
 // rH/rE, the layout the two row-path sweeps leave.
-template <int R>
+template <int R, bool SH>
 __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B, int M,
                                                       int N, int tb, int te, int go, int ge, int* fH,
                                                       int* fE, int* rH, int* rE)
@@ -538,6 +560,9 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
     const int lastlane = (R > 1) ? 31 : P - 1;
     const int d = dummy ? 1 : 0;
     const int firstrow = 1 + (L < k1 ? L * R : k1 * R + (L - k1) * (R - 1));
+    // SH: H, E and F of cell (i, j) carried as X + (i + j) * ge, as the
+    // row-path stripes do (the gap recurrences lose their extension term)
+    const int ge2 = 2 * ge;
     int sb[R], hL[R], hLg[R], f[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -547,8 +572,15 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
         hL[r] = row == 0 ? 0 : -t0v - row * ge;                // left column H[row][0]
         f[r] = hL[r] - go;
         hLg[r] = hL[r] - goe;
+        if (SH) {
+            const int sh = row * ge;
+            f[r] += sh;
+            hLg[r] = hL[r] + sh - go;
+            hL[r] += sh + ge2;
+        }
     }
     int dprev = (firstrow == 1) ? 0 : -t0v - (firstrow - 1) * ge;
+    if (SH) dprev += (firstrow - 1) * ge + ge2;
     int outHv = 0, outEv = 0;
     int caddr = cb + cd * (1 - L);
     int b4 = s2[caddr];
@@ -558,7 +590,7 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
 
     auto step = [&](int t, auto masked) {
         // lanes 0 and 32: their half's top row H[0][t] = -t0 - t*ge, E = H - go
-        const int topH = -t0v - t * ge;
+        const int topH = SH ? -t0v : -t0v - t * ge;
         int inHv = shr1(topH, outHv);
         int inEv = shr1(topH - go, outEv);
         if (top) {
@@ -573,20 +605,20 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
         if constexpr (decltype(masked)::value) act = lane_on && c >= 1 && c <= N;
         if (act) {
             int hup = inHv, eup = inEv, dg = dprev;
-            int hupg = inHv - goe;
-            dprev = inHv;
+            int hupg = SH ? inHv - go : inHv - goe;
+            dprev = SH ? inHv + ge2 : inHv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int sc = score_at(tab, sb[r] + bcur);
-                int e = max(eup - ge, hupg);
-                f[r] = max(f[r] - ge, hLg[r]);
+                int e = SH ? max(eup, hupg) : max(eup - ge, hupg);
+                f[r] = SH ? max(f[r], hLg[r]) : max(f[r] - ge, hLg[r]);
                 int h = max(max(dg + sc, e), f[r]);
                 if (r == 0 && R > 1) {
                     if (dummy) { h = hup; e = eup; }
                 }
                 dg = hL[r];
-                hL[r] = h;
-                hLg[r] = h - goe;
+                hL[r] = SH ? h + ge2 : h;
+                hLg[r] = SH ? h - go : h - goe;
                 hup = h;
                 hupg = hLg[r];
                 eup = e;
@@ -594,8 +626,9 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
             outHv = hup;
             outEv = eup;
             if (bottom) {
-                oH[c] = hup;
-                oE[c] = eup;
+                const int sh = SH ? (P + c) * ge : 0;             // the bottom lane's row is P
+                oH[c] = hup - sh;
+                oE[c] = eup - sh;
             }
         }
     };
@@ -616,12 +649,12 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
     for (; t <= tend; ++t) step(t, Tag<true>{});
 }
 
-template <int C>
+template <int C, bool SH>
 __device__ __attribute__((noinline)) void halves_cols_call(lcu8* tab, lcu8* s1, lcu8* s2, int A, int B,
                                                            int M, int N, int tb, int te, int go, int ge,
                                                            int* fH, int* fE, int* rH, int* rE)
 {
-    mm_halves_cols<C>(tab, s1, s2, uni(A), uni(B), uni(M), uni(N), uni(tb), uni(te), uni(go), uni(ge), fH,
+    mm_halves_cols<C, SH>(tab, s1, s2, uni(A), uni(B), uni(M), uni(N), uni(tb), uni(te), uni(go), uni(ge), fH,
                       fE, rH, rE);
 }
 
@@ -646,7 +679,7 @@ __device__ __forceinline__ bool pw_q2(int M, int N, int colcap)
     return M >= 2 && M <= kQ2Rows && N > kQ4Cols && N <= kQ2Cols && N <= colcap;
 }
 
-template <int G, int C>
+template <int G, int C, bool SH>
 __device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8* s2, const int* q, int cnt,
                                                       int go, int ge, int* F, int* R)
 {
@@ -673,8 +706,14 @@ __device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8*
         hup[c] = -t0 - jj * ge;                // top row H[0][jj]
         ev[c] = hup[c] - go;
         hupg[c] = hup[c] - goe;
+        if (SH) {                              // cell (i, jj) carried as X + (i + jj) ge
+            hupg[c] = -t0 - go;
+            ev[c] = -t0 - go;
+            hup[c] = -t0 + 2 * ge;             // stored with the diagonal's + 2 ge
+        }
     }
     int dleft = (L == 0) ? 0 : (-t0 - (L * C) * ge);   // H[0][L*C]; corner is 0
+    if (SH) dleft = (L == 0 ? 0 : -t0) + 2 * ge;
     int hl_out = 0, fl_out = 0;
     const int T = wave_max(rows > 0 && L < nl ? rows + nl - 1 : 0);
     for (int t = 1; t <= T; ++t) {
@@ -682,27 +721,41 @@ __device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8*
         int fl = shr1(0, fl_out);
         const int i = t - L;
         if (L == 0) {                          // left column H[i][0], F[i][0]
-            hl = -t0 - i * ge;
+            hl = SH ? -t0 : -t0 - i * ge;
             fl = hl - go;
         }
         if (i >= 1 && i <= rows && L < nl) {
             const int a128 = (int)s1[rb + rd * i] * 128;
             int d = dleft;
             int hc = hl, fc = fl;
+            int hcg = SH ? hl - go : 0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const int sc = score_at(tab, a128 + b4[c]);
-                const int en = max(ev[c] - ge, hupg[c]);
-                const int fn = max(fc - ge, hc - goe);
-                const int h = max(max(d + sc, en), fn);
-                d = hup[c];
-                hup[c] = h;
-                hupg[c] = h - goe;
-                ev[c] = en;
-                hc = h;
-                fc = fn;
+                if (SH) {
+                    const int en = max(ev[c], hupg[c]);
+                    const int fn = max(fc, hcg);
+                    const int h = max(max(d + sc, en), fn);
+                    d = hup[c];
+                    hup[c] = h + 2 * ge;
+                    hupg[c] = h - go;
+                    hcg = hupg[c];
+                    ev[c] = en;
+                    hc = h;
+                    fc = fn;
+                } else {
+                    const int en = max(ev[c] - ge, hupg[c]);
+                    const int fn = max(fc - ge, hc - goe);
+                    const int h = max(max(d + sc, en), fn);
+                    d = hup[c];
+                    hup[c] = h;
+                    hupg[c] = h - goe;
+                    ev[c] = en;
+                    hc = h;
+                    fc = fn;
+                }
             }
-            dleft = hl;
+            dleft = SH ? hl + 2 * ge : hl;
             hl_out = hc;
             fl_out = fc;
         }
@@ -711,7 +764,11 @@ __device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8*
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int jj = L * C + c + 1;
-        if (on && jj <= N) { oH[jj] = hup[c]; oH[STRIDE + jj] = ev[c]; }
+        if (on && jj <= N) {
+            const int sh = SH ? (rows + jj) * ge : 0;
+            oH[jj] = SH ? hup[c] - 2 * ge - sh : hup[c];
+            oH[STRIDE + jj] = ev[c] - sh;
+        }
     }
 }
 
@@ -967,6 +1024,8 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
     };
     // the top six stack entries hold the queues of small column-path nodes
     // (four entries for N <= 63, two for 64 <= N <= 127)
+    // queued nodes always run in the shifted frame: (i + j) * ge < 320 * 2^16
+    const bool small_ge = ge > -65536 && ge < 65536;
     int* const q4 = stack + 6 * (stack_cap - 6);
     int* const q2 = stack + 6 * (stack_cap - 2);
     const int cap = stack_cap - 6;
@@ -980,14 +1039,14 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         for (int kk = 0; kk < cnt; ++kk) mN = max(mN, uni(q[6 * kk + 3]));
         constexpr int LPH = 32 / G;
         switch ((mN + LPH - 1) / LPH) {                  // columns per lane: N <= C * lanes per half
-        case 1: colsG_sweep<G, 1>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 2: colsG_sweep<G, 2>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 3: colsG_sweep<G, 3>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 4: colsG_sweep<G, 4>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 5: colsG_sweep<G, 5>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 6: colsG_sweep<G, 6>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        case 7: colsG_sweep<G, 7>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
-        default: colsG_sweep<G, 8>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 1: colsG_sweep<G, 1, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 2: colsG_sweep<G, 2, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 3: colsG_sweep<G, 3, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 4: colsG_sweep<G, 4, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 5: colsG_sweep<G, 5, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 6: colsG_sweep<G, 6, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        case 7: colsG_sweep<G, 7, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
+        default: colsG_sweep<G, 8, true>(base.tab, s1, s2, q, cnt, go, ge, smallF, smallR); break;
         }
         wave_fence();
         const int tm = colsG_midpoints<G>(q, cnt, go, ge, smallF, smallR);
@@ -1067,7 +1126,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         }
         ++nodes;
         cells += M * N;
-        if (pw_q4(M, N, colcap)) {
+        if (pw_q4(M, N, colcap) && small_ge) {
             int* qe = q4 + 6 * n4;                       // same value from every lane
             qe[0] = A; qe[1] = B; qe[2] = M; qe[3] = N; qe[4] = tb; qe[5] = te;
             n4 += 1;
@@ -1078,7 +1137,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             }
             continue;
         }
-        if (pw_q2(M, N, colcap)) {
+        if (pw_q2(M, N, colcap) && small_ge) {
             int* qe = q2 + 6 * n2;                       // same value from every lane
             qe[0] = A; qe[1] = B; qe[2] = M; qe[3] = N; qe[4] = tb; qe[5] = te;
             n2 += 1;
@@ -1091,32 +1150,50 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
         }
         const int midi = M / 2;
         int *fH, *fE, *rH, *rE;          // HH/DD[j] at [j]; RR/SS[j] at [N-j]
+        // shifted frame while (i + j) * ge stays small beside the scores
+        const bool shf = (int64_t)(M + N + 2) * (ge < 0 ? -(int64_t)ge : (int64_t)ge) < (1 << 26);
         if (N <= colcap) {
             fH = smallF; fE = smallF + (small_cap + 1);
             rH = smallR; rE = smallR + (small_cap + 1);
             // C = ceil(N / 32) columns per lane: every lane of a half in use
             switch ((N + 31) >> 5) {
-            case 1: halves_cols_call<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 2: halves_cols_call<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 3: halves_cols_call<3>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 4: halves_cols_call<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 5: halves_cols_call<5>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 6: halves_cols_call<6>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 7: halves_cols_call<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            default: halves_cols_call<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 1: if (shf) halves_cols_call<1, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<1, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 2: if (shf) halves_cols_call<2, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<2, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 3: if (shf) halves_cols_call<3, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<3, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 4: if (shf) halves_cols_call<4, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<4, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 5: if (shf) halves_cols_call<5, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<5, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 6: if (shf) halves_cols_call<6, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<6, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 7: if (shf) halves_cols_call<7, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else halves_cols_call<7, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            default: if (shf) halves_cols_call<8, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                     else halves_cols_call<8, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             }
         } else if (M - midi <= 32 * CW_RMAX) {
             fH = g2H; fE = g2E;
             rH = g3H; rE = g3E;
             switch ((M - midi + 31) >> 5) {               // rows per lane of the taller half
-            case 1: dual_halves<1>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 2: dual_halves<2>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 3: dual_halves<3>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 4: dual_halves<4>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 5: dual_halves<5>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 6: dual_halves<6>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            case 7: dual_halves<7>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
-            default: dual_halves<8>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 1: if (shf) dual_halves<1, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<1, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 2: if (shf) dual_halves<2, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<2, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 3: if (shf) dual_halves<3, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<3, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 4: if (shf) dual_halves<4, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<4, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 5: if (shf) dual_halves<5, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<5, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 6: if (shf) dual_halves<6, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<6, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            case 7: if (shf) dual_halves<7, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                    else dual_halves<7, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
+            default: if (shf) dual_halves<8, true>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE);
+                     else dual_halves<8, false>(base.tab, s1, s2, A, B, M, N, tb, te, go, ge, fH, fE, rH, rE); break;
             }
         } else {
             Track T{0, 0};
@@ -1127,8 +1204,6 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             F.th0 = -tb; F.ths = -ge; F.teo = -go;
             F.lh0 = -tb; F.lhs = -ge; F.lfo = -go;
             fH = g2H; fE = g2E;
-            // shifted frame while (i + j) * ge stays small beside the scores
-            const bool shf = (int64_t)(M + N + 2) * (ge < 0 ? -(int64_t)ge : (int64_t)ge) < (1 << 26);
             if (shf) sweep<MODE_GLB, true>(F, rmax, g, fH, fE, T);
             else sweep<MODE_GLB, false>(F, rmax, g, fH, fE, T);
             Sweep Rv = base;
