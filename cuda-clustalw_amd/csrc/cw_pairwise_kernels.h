@@ -4,9 +4,12 @@
 
 #define CW_SEQ_PAD 64          // code-0 padding on each side of a per-wave LDS sequence
 #define CW_SMALL_COLS 256      // Myers-Miller nodes with N <= this run lanes-own-columns, rows in LDS
-#define CW_STACK_CAP 48        // Myers-Miller stack entries per wave (depth <= log2(L)+2)
+#define CW_STACK_CAP 40        // Myers-Miller stack entries per wave (depth <= log2(L)+2, 6 of them queues)
 #define CW_TINYQ 64            // queued tiny Myers-Miller subtrees per wave (one per lane)
-#define CW_TAB_BYTES 4096      // int32 [32][32] substitution matrix in LDS
+#define CW_TAB_STRIDE 33       // ints per matrix row in LDS: an odd stride spreads a wave's
+                               // (row, column) lookups over more banks than 32 would
+#define CW_TAB_ROW (4 * CW_TAB_STRIDE)        // bytes per matrix row in LDS
+#define CW_TAB_BYTES (32 * CW_TAB_ROW)        // int32 [32][CW_TAB_STRIDE] substitution matrix in LDS
 #define CW_MAX_THREADS 256     // threads per workgroup (4 waves)
 #define CW_RMAX 8              // rows per lane the kernels are instantiated for
 #define CW_NCLASS 6            // tuning classes by the longer sequence: <=128 <=256 <=512 <=1024 <=2048 longer

@@ -99,7 +99,7 @@ struct Sweep {
     int go, ge, goe;                 // H[0][0] = 0 in every pass
     int thr;                         // MODE_REV threshold (maxScore)
     int kb;                          // packed tracking: column bits (ncols < 2^kb)
-    lcu8* tab;                       // LDS substitution matrix, int32 [32][32], byte-addressed
+    lcu8* tab;                       // LDS substitution matrix, int32 [32][CW_TAB_STRIDE], byte-addressed
 };
 
 struct Track {
@@ -160,7 +160,7 @@ __device__ __forceinline__ void stripe(const Sweep& S, int row0, int P, int firs
     for (int r = 0; r < R; ++r) {
         int row = firstrow + r - d;
         int rr = row < 1 ? 1 : (row > S.nrows ? S.nrows : row);
-        sb[r] = (int)S.rs[S.rb + S.rd * rr] * 128;
+        sb[r] = (int)S.rs[S.rb + S.rd * rr] * CW_TAB_ROW;
         hL[r] = left_h(S, row);
         f[r] = hL[r] + S.lfo;
         hLg[r] = hL[r] - S.goe;
@@ -477,7 +477,7 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
             hl = SH ? -t0 : -t0 - i * ge;
             fl = hl - go;
         }
-        const int a128 = (int)s1[raddr] * 128;
+        const int arow = (int)s1[raddr] * CW_TAB_ROW;
         raddr += rd;
         if (i >= 1 && i <= rows && L < nl) {
             int d = dleft;
@@ -485,7 +485,7 @@ __device__ __forceinline__ void mm_halves_cols(lcu8* tab, lcu8* s1, lcu8* s2, in
             int hcg = SH ? hl - go : 0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const int sc = score_at(tab, a128 + b4[c]);
+                const int sc = score_at(tab, arow + b4[c]);
                 if (SH) {
                     const int en = max(e[c], hupg[c]);
                     const int fn = max(fc, hcg);
@@ -568,7 +568,7 @@ __device__ __attribute__((noinline)) void dual_halves(lcu8* tab, lcu8* s1, lcu8*
     for (int r = 0; r < R; ++r) {
         const int row = firstrow + r - d;
         const int rr = row < 1 ? 1 : (row > P ? P : row);
-        sb[r] = (int)s1[rb + rd * rr] * 128;
+        sb[r] = (int)s1[rb + rd * rr] * CW_TAB_ROW;
         hL[r] = row == 0 ? 0 : -t0v - row * ge;                // left column H[row][0]
         f[r] = hL[r] - go;
         hLg[r] = hL[r] - goe;
@@ -725,13 +725,13 @@ __device__ __attribute__((noinline)) void colsG_sweep(lcu8* tab, lcu8* s1, lcu8*
             fl = hl - go;
         }
         if (i >= 1 && i <= rows && L < nl) {
-            const int a128 = (int)s1[rb + rd * i] * 128;
+            const int arow = (int)s1[rb + rd * i] * CW_TAB_ROW;
             int d = dleft;
             int hc = hl, fc = fl;
             int hcg = SH ? hl - go : 0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                const int sc = score_at(tab, a128 + b4[c]);
+                const int sc = score_at(tab, arow + b4[c]);
                 if (SH) {
                     const int en = max(ev[c], hupg[c]);
                     const int fn = max(fc, hcg);
@@ -885,7 +885,7 @@ __device__ __attribute__((noinline)) void pw_tiny_subtree(lcu8* tab, lcu8* s1, l
             int bv = -BIG, bj = 0;
             for (int j = 1; j <= N; ++j) {
                 const int k1 = N - j, k2 = j - 1;
-                const int v = score_at(tab, a * 128 + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
+                const int v = score_at(tab, a * CW_TAB_ROW + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
                               (k2 <= 0 ? 0 : tb + ge * k2);
                 if (v > bv) { bv = v; bj = j; }
             }
@@ -907,14 +907,14 @@ __device__ __attribute__((noinline)) void pw_tiny_subtree(lcu8* tab, lcu8* s1, l
         }
         H[0] = 0;
         for (int i = 1; i <= midi; ++i) {
-            const int a128 = (int)s1[A + i] * 128;
+            const int arow = (int)s1[A + i] * CW_TAB_ROW;
             int d = H[0];
             int hc = -tb - i * ge, fc = hc - go;
             H[0] = hc;
 #pragma unroll
             for (int k = 1; k <= NT; ++k) {
                 if (k <= N) {
-                    const int sc = score_at(tab, a128 + s2[B + k]);
+                    const int sc = score_at(tab, arow + s2[B + k]);
                     const int en = max(E[k] - ge, H[k] - goe);
                     const int fn = max(fc - ge, hc - goe);
                     const int h = max(max(d + sc, en), fn);
@@ -936,14 +936,14 @@ __device__ __attribute__((noinline)) void pw_tiny_subtree(lcu8* tab, lcu8* s1, l
         }
         int rprev = 0;                                   // H of the reverse left column, row r-1
         for (int r = 1; r <= M - midi; ++r) {
-            const int a128 = (int)s1[A + M + 1 - r] * 128;
+            const int arow = (int)s1[A + M + 1 - r] * CW_TAB_ROW;
             int d = rprev;
             int hc = -te - r * ge, fc = hc - go;
             rprev = hc;
 #pragma unroll
             for (int k = NT; k >= 1; --k) {
                 if (k <= N) {
-                    const int sc = score_at(tab, a128 + s2[B + k]);
+                    const int sc = score_at(tab, arow + s2[B + k]);
                     const int en = max(RE[k] - ge, RH[k] - goe);
                     const int fn = max(fc - ge, hc - goe);
                     const int h = max(max(d + sc, en), fn);
@@ -1112,7 +1112,7 @@ __device__ int mm_count(const Sweep& base, lcu8* s1, lcu8* s2, int go, int ge,
             int bv = -BIG, bj = BIG;
             for (int j = 1 + lane; j <= N; j += WAVE) {
                 const int k1 = N - j, k2 = j - 1;
-                const int v = score_at(base.tab, a * 128 + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
+                const int v = score_at(base.tab, a * CW_TAB_ROW + s2[B + j]) - (k1 <= 0 ? 0 : te + ge * k1) -
                               (k2 <= 0 ? 0 : tb + ge * k2);
                 if (v > bv) { bv = v; bj = j; }
             }
@@ -1419,7 +1419,7 @@ __device__ __attribute__((noinline)) void do_pair(const DevArgs& A, int64_t p, i
 
 __device__ void load_table(const DevArgs& A, CW_LDS int* tab)
 {
-    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) tab[k] = A.matrix[k];
+    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) tab[(k >> 5) * CW_TAB_STRIDE + (k & 31)] = A.matrix[k];
     __syncthreads();
 }
 
@@ -1431,7 +1431,7 @@ __device__ int table_rcmax(lcu8* tab, int* fit)
     const int c = lane & 31;
     const CW_LDS int* t = (const CW_LDS int*)tab;
     int v = (int)0x80000000;
-    for (int k = 0; k < 32; ++k) v = max(v, lane < 32 ? t[c * 32 + k] : t[k * 32 + c]);
+    for (int k = 0; k < 32; ++k) v = max(v, lane < 32 ? t[c * CW_TAB_STRIDE + k] : t[k * CW_TAB_STRIDE + c]);
     *fit = uni(wave_max(v)) <= 32767;
     return v < 0 ? 0 : (v > 32767 ? 32767 : v);
 }
