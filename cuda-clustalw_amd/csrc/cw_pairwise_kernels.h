@@ -6,7 +6,7 @@
 #define CW_SMALL_COLS 256      // Myers-Miller nodes with N <= this run lanes-own-columns, rows in LDS
 #define CW_STACK_CAP 40        // Myers-Miller stack entries per wave (depth <= log2(L)+2, 6 of them queues)
 #define CW_TINYQ 64            // queued tiny Myers-Miller subtrees per wave (one per lane)
-#define CW_TAB_STRIDE 33       // ints per matrix row in LDS: an odd stride spreads a wave's
+#define CW_TAB_STRIDE 35       // ints per matrix row in LDS: an odd stride spreads a wave's
                                // (row, column) lookups over more banks than 32 would
 #define CW_TAB_ROW (4 * CW_TAB_STRIDE)        // bytes per matrix row in LDS
 #define CW_TAB_BYTES (32 * CW_TAB_ROW)        // int32 [32][CW_TAB_STRIDE] substitution matrix in LDS
