@@ -14,6 +14,8 @@
 #define CW_RMAX 8              // rows per lane the kernels are instantiated for
 #define CW_NCLASS 6            // tuning classes by the longer sequence: <=128 <=256 <=512 <=1024 <=2048 longer
 #define CW_NBINS 136           // work-queue bins: 4 per octave of the DP area n*m (n, m < 2^16)
+#define CW_QBINS (2 * CW_NBINS)            // queue bins: the short pairs' area bins, then the long pairs'
+#define CW_BINS_INTS (2 * CW_QBINS + 2)    // counts, cursors, then the long pairs' count and the total
 
 struct DevArgs {
     const uint8_t* codes;      // packed residue codes of all sequences
@@ -36,13 +38,23 @@ struct DevArgs {
     int32_t* out_count;        // per pair identity count (-1: empty sequence)
     int32_t* out_info;         // optional per pair [maxScore,se1,se2,sb1,sb2,status,nodes,mm cells]
     int32_t* status;           // OR of per-pair status bits (0 = consistent)
-    unsigned int* queue;       // persistent kernel work counter
-    int32_t* bins;             // batched queue: CW_NBINS counts, then CW_NBINS cursors
+    unsigned int* queue;       // persistent kernel work counters (two: the long part, the short part)
+    int32_t* bins;             // batched queue: CW_BINS_INTS ints (see CW_BINS_INTS)
     int32_t* order_out;        // batched queue: the binned work order the persistent kernel pops
     int32_t rmax_cls[CW_NCLASS];   // stripe rows per lane by tuning class
     int32_t colcap_cls[CW_NCLASS]; // Myers-Miller nodes with N <= this use the column path (<= CW_SMALL_COLS)
     int32_t nseq;              // sequences behind lens/offs: a pair naming another is refused
+    int32_t lsplit;            // pairs whose longer sequence exceeds this queue first (the long part)
+    int32_t qpart;             // persistent kernel: 0 the whole queue, 1 its long part, 2 its short part
     int32_t pad_;
+};
+
+// The short part's launch of a split batched run: its arguments (smaller
+// sequence buffers) and shape.
+struct CwSplitShape {
+    DevArgs args;
+    int blocks, threads;
+    size_t lds;
 };
 
 // Status bits of DevArgs::status: 1 reverse pass found no start cell, 2

@@ -1468,13 +1468,23 @@ extern "C" __global__ void __launch_bounds__(CW_MAX_THREADS, 4) cw_pairwise_pers
     const int slot = blockIdx.x * (blockDim.x / WAVE) + wave;
     int rcfit;
     const int rcmax = table_rcmax(tab, &rcfit);
+    // this launch's slice of the binned order (the binning kernels left the
+    // long part's size and the total after the cursors) and its own counter
+    int qb = 0, qe = (int)A.pair_end;
+    unsigned int* qc = A.queue;
+    if (A.qpart) {
+        const int nl = uni(A.bins[2 * CW_QBINS]), tot = uni(A.bins[2 * CW_QBINS + 1]);
+        qb = A.qpart == 1 ? 0 : nl;
+        qe = A.qpart == 1 ? nl : tot;
+        qc = A.queue + (A.qpart - 1);
+    }
     for (;;) {
         // one atomic per pop (lane 0's); every lane reaches the broadcast, where
         // lane 0 is the first active lane
         unsigned got = 0;
-        if (lane_id() == 0) got = atomicAdd(A.queue, 1u);
-        const int item = uni((int)got);
-        if ((int64_t)item >= A.pair_end) break;
+        if (lane_id() == 0) got = atomicAdd(qc, 1u);
+        const int item = qb + uni((int)got);
+        if (item >= qe) break;
         const int64_t q = A.order ? A.order[item] : item;
         do_pair(A, q, slot, tab, wlds, rcmax, rcfit);
     }
@@ -1507,6 +1517,10 @@ hipError_t cw_launch_reset(hipStream_t st, unsigned int* q, int32_t* status)
 // ---- length-binned work queue of the batched kernel --------------------------
 // A counting sort of the pairs by cw_bin_of (4 bins per octave of n*m), largest
 // bins first, built on the device in three launches: count, scan, scatter.
+// Pairs whose longer sequence exceeds DevArgs::lsplit take bins of their own
+// above all others, so they form the front of the order (the long part) and
+// can run in a launch of their own with per-wave sequence buffers that hold
+// them, the rest in a launch whose smaller buffers leave room for more waves.
 // Within a bin the order is whatever the atomics give; every pair's result is
 // independent of the order, so the output does not depend on it.
 
@@ -1514,8 +1528,8 @@ constexpr int kBinItems = 1024;       // pairs per workgroup in count/scatter
 
 extern "C" __global__ void __launch_bounds__(256) cw_bin_reset(DevArgs A)
 {
-    for (int k = threadIdx.x; k < 2 * CW_NBINS; k += blockDim.x) A.bins[k] = 0;
-    if (threadIdx.x == 0) { *A.queue = 0u; *A.status = 0; }
+    for (int k = threadIdx.x; k < CW_BINS_INTS; k += blockDim.x) A.bins[k] = 0;
+    if (threadIdx.x == 0) { A.queue[0] = 0u; A.queue[1] = 0u; *A.status = 0; }
 }
 
 __device__ __forceinline__ int pair_bin(const DevArgs& A, int64_t p)
@@ -1523,13 +1537,14 @@ __device__ __forceinline__ int pair_bin(const DevArgs& A, int64_t p)
     const int si = A.pairs[2 * p], sj = A.pairs[2 * p + 1];
     if ((unsigned)si >= (unsigned)A.nseq || (unsigned)sj >= (unsigned)A.nseq) return 0;  // refused by do_pair
     const int n = A.lens[si], m = A.lens[sj];
-    return cw_bin_of(n < 0 ? 0 : n > 0xffff ? 0xffff : n, m < 0 ? 0 : m > 0xffff ? 0xffff : m);
+    const int b = cw_bin_of(n < 0 ? 0 : n > 0xffff ? 0xffff : n, m < 0 ? 0 : m > 0xffff ? 0xffff : m);
+    return (n > A.lsplit || m > A.lsplit) ? b + CW_NBINS : b;
 }
 
 extern "C" __global__ void __launch_bounds__(256) cw_bin_count(DevArgs A)
 {
-    __shared__ int cnt[CW_NBINS];
-    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
+    __shared__ int cnt[CW_QBINS];
+    for (int k = threadIdx.x; k < CW_QBINS; k += blockDim.x) cnt[k] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kBinItems;
     for (int k = threadIdx.x; k < kBinItems; k += blockDim.x) {
@@ -1537,26 +1552,29 @@ extern "C" __global__ void __launch_bounds__(256) cw_bin_count(DevArgs A)
         if (p < A.pair_end) atomicAdd(&cnt[pair_bin(A, p)], 1);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x)
+    for (int k = threadIdx.x; k < CW_QBINS; k += blockDim.x)
         if (cnt[k]) atomicAdd(&A.bins[k], cnt[k]);
 }
 
-// cursors: bin CW_NBINS-1 starts at 0, each smaller bin after the larger ones
+// cursors: bin CW_QBINS-1 starts at 0, each smaller bin after the larger ones;
+// then the long part's size and the total
 extern "C" __global__ void __launch_bounds__(64) cw_bin_scan(DevArgs A)
 {
     if (threadIdx.x == 0) {
         int run = 0;
-        for (int b = CW_NBINS - 1; b >= 0; --b) {
-            A.bins[CW_NBINS + b] = run;
+        for (int b = CW_QBINS - 1; b >= 0; --b) {
+            if (b == CW_NBINS - 1) A.bins[2 * CW_QBINS] = run;
+            A.bins[CW_QBINS + b] = run;
             run += A.bins[b];
         }
+        A.bins[2 * CW_QBINS + 1] = run;
     }
 }
 
 extern "C" __global__ void __launch_bounds__(256) cw_bin_scatter(DevArgs A)
 {
-    __shared__ int cnt[CW_NBINS], at[CW_NBINS];
-    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
+    __shared__ int cnt[CW_QBINS], at[CW_QBINS];
+    for (int k = threadIdx.x; k < CW_QBINS; k += blockDim.x) cnt[k] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kBinItems;
     constexpr int per = kBinItems / 256;
@@ -1571,8 +1589,8 @@ extern "C" __global__ void __launch_bounds__(256) cw_bin_scatter(DevArgs A)
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x)
-        at[k] = cnt[k] ? atomicAdd(&A.bins[CW_NBINS + k], cnt[k]) : 0;
+    for (int k = threadIdx.x; k < CW_QBINS; k += blockDim.x)
+        at[k] = cnt[k] ? atomicAdd(&A.bins[CW_QBINS + k], cnt[k]) : 0;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < per; ++u)
@@ -1585,18 +1603,20 @@ constexpr int kBinSmallPairs = 65536;
 
 extern "C" __global__ void __launch_bounds__(1024) cw_bin_small(DevArgs A)
 {
-    __shared__ int cnt[CW_NBINS], at[CW_NBINS];
-    for (int k = threadIdx.x; k < CW_NBINS; k += blockDim.x) cnt[k] = 0;
-    if (threadIdx.x == 0) { *A.queue = 0u; *A.status = 0; }
+    __shared__ int cnt[CW_QBINS], at[CW_QBINS];
+    for (int k = threadIdx.x; k < CW_QBINS; k += blockDim.x) cnt[k] = 0;
+    if (threadIdx.x == 0) { A.queue[0] = 0u; A.queue[1] = 0u; *A.status = 0; }
     __syncthreads();
     for (int64_t p = threadIdx.x; p < A.pair_end; p += blockDim.x) atomicAdd(&cnt[pair_bin(A, p)], 1);
     __syncthreads();
-    if (threadIdx.x == 0) {                 // bin CW_NBINS-1 first
+    if (threadIdx.x == 0) {                 // bin CW_QBINS-1 first
         int run = 0;
-        for (int b = CW_NBINS - 1; b >= 0; --b) {
+        for (int b = CW_QBINS - 1; b >= 0; --b) {
+            if (b == CW_NBINS - 1) A.bins[2 * CW_QBINS] = run;
             at[b] = run;
             run += cnt[b];
         }
+        A.bins[2 * CW_QBINS + 1] = run;
     }
     __syncthreads();
     for (int64_t p = threadIdx.x; p < A.pair_end; p += blockDim.x)
@@ -1605,8 +1625,11 @@ extern "C" __global__ void __launch_bounds__(1024) cw_bin_small(DevArgs A)
 
 // The batched run's stages on one stream: reset, count, scan, scatter (or
 // cw_bin_small for a small batch), then the persistent kernel over the binned
-// order (captured into one hipGraph by the host).
-hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a)
+// order (captured into one hipGraph by the host).  With `s` (a.lsplit below the
+// longest sequence), the persistent kernel runs twice: over the long part
+// with a's shape, then over the short part with s's (smaller sequence buffers,
+// more waves per CU).
+static hipError_t launch_bins(hipStream_t st, const DevArgs& a)
 {
     const int nb = (int)((a.pair_end + kBinItems - 1) / kBinItems);
     if (a.pair_end <= kBinSmallPairs) {
@@ -1617,11 +1640,24 @@ hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st
         hipLaunchKernelGGL(cw_bin_scan, dim3(1), dim3(64), 0, st, a);
         hipLaunchKernelGGL(cw_bin_scatter, dim3(nb), dim3(256), 0, st, a);
     }
-    hipError_t e = hipGetLastError();
+    return hipGetLastError();
+}
+
+hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a,
+                             const CwSplitShape* s)
+{
+    hipError_t e = launch_bins(st, a);
     if (e != hipSuccess) return e;
     DevArgs b = a;
     b.order = a.order_out;
+    b.qpart = s ? 1 : 0;
     hipLaunchKernelGGL(cw_pairwise_persistent, dim3(blocks), dim3(threads), lds, st, b);
+    e = hipGetLastError();
+    if (e != hipSuccess || !s) return e;
+    DevArgs c = s->args;
+    c.order = a.order_out;
+    c.qpart = 2;
+    hipLaunchKernelGGL(cw_pairwise_persistent, dim3(s->blocks), dim3(s->threads), s->lds, st, c);
     return hipGetLastError();
 }
 

@@ -19,7 +19,8 @@
 hipError_t cw_launch_plain(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
 hipError_t cw_launch_persistent(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
 hipError_t cw_launch_reset(hipStream_t st, unsigned int* q, int32_t* status);
-hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
+hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a,
+                             const CwSplitShape* s);
 hipError_t cw_occupancy(int threads, size_t lds, int* plain_blocks, int* pers_blocks);
 
 #include <map>
@@ -267,7 +268,7 @@ int cwhip_stage_pairs(cwhip_ctx* ctx, const cwhip_pair_params* p, const int32_t*
     HIPCHK(ctx->d_go.ensure(p->max_len + 1));
     HIPCHK(ctx->d_status.ensure(4));
     HIPCHK(ctx->d_queue.ensure(4));
-    HIPCHK(ctx->d_bins.ensure(2 * CW_NBINS));
+    HIPCHK(ctx->d_bins.ensure(CW_BINS_INTS));
     if (npairs)
         HIPCHK(hipMemcpyAsync(ctx->d_pairs.p, pairs, 2 * npairs * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_matrix.p, p->matrix, 32 * 32 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
@@ -295,6 +296,8 @@ static void fill_args(cwhip_ctx* ctx, DevArgs& a)
     a.rowcap = ((ctx->maxlen + 2) + 63) & ~63;
     a.maxlen = ctx->maxlen;
     a.nseq = ctx->nseq;
+    a.lsplit = 0x7fffffff;
+    a.qpart = 0;
     a.scratch_stride = 8 * (int64_t)a.rowcap;
     a.out_count = ctx->d_count.p;
     a.out_info = ctx->d_info.p;
@@ -312,16 +315,18 @@ static void fill_args(cwhip_ctx* ctx, DevArgs& a)
 // time it is seen and is captured when the same launch (buffers, sizes, shapes)
 // comes again, so one-off jobs (a server's stream of different jobs) pay no
 // capture and repeated runs of one staged job replay the graph.
-static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int threads, size_t lds)
+static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int threads, size_t lds,
+                          const CwSplitShape* sp)
 {
-    std::vector<uint8_t> key(sizeof(DevArgs) + 3 * sizeof(int64_t));
+    std::vector<uint8_t> key(sizeof(DevArgs) + 3 * sizeof(int64_t) + (sp ? sizeof(CwSplitShape) : 0));
     std::memcpy(key.data(), &a, sizeof(DevArgs));
     const int64_t dims[3] = {blocks, threads, (int64_t)lds};
     std::memcpy(key.data() + sizeof(DevArgs), dims, sizeof dims);
+    if (sp) std::memcpy(key.data() + sizeof(DevArgs) + sizeof dims, sp, sizeof(CwSplitShape));
     const bool replay = ctx->gexec && key == ctx->gkey;
     if (!ctx->use_graph || (!replay && key != ctx->glast)) {
         ctx->glast = key;
-        HIPCHK(cw_launch_batched(blocks, threads, lds, ctx->stream, a));
+        HIPCHK(cw_launch_batched(blocks, threads, lds, ctx->stream, a, sp));
         return CWHIP_OK;
     }
     if (!replay) {
@@ -331,7 +336,7 @@ static int launch_batched(cwhip_ctx* ctx, const DevArgs& a, int blocks, int thre
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t e1 = cw_launch_batched(blocks, threads, lds, ctx->stream, a);
+        const hipError_t e1 = cw_launch_batched(blocks, threads, lds, ctx->stream, a, sp);
         const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
@@ -365,6 +370,37 @@ static int batched_shape(cwhip_ctx* ctx, const DevArgs& a, int64_t npairs, int* 
     return CWHIP_OK;
 }
 
+// Waves per CU of the batched kernel when every sequence is at most L long.
+static int batched_waves_per_cu(cwhip_ctx* ctx, int L, int* wpg_out)
+{
+    const int wave_lds = cw_wave_lds_bytes(cw_lcap(L));
+    const int wpg = cw_fit_waves(ctx->wpg, wave_lds);
+    if (wpg < 1) return 0;
+    int occ_plain = 0, occ_pers = 0;
+    if (cw_occupancy(64 * wpg, CW_TAB_BYTES + (size_t)wpg * wave_lds, &occ_plain, &occ_pers) != hipSuccess)
+        return 0;
+    if (ctx->groups_per_cu) occ_pers = std::min(occ_pers, ctx->groups_per_cu);
+    if (wpg_out) *wpg_out = wpg;
+    return occ_pers * wpg;
+}
+
+// The longest sequence length at which the batched kernel still runs its most
+// waves per CU, when the job's longest sequence would run fewer (0: no split).
+// Pairs with a longer sequence then run first, in a launch of their own.
+static int batched_split_len(cwhip_ctx* ctx)
+{
+    const int full = batched_waves_per_cu(ctx, ctx->maxlen, nullptr);
+    const int most = batched_waves_per_cu(ctx, 64, nullptr);
+    if (full >= most || ctx->maxlen <= 64) return 0;
+    int lo = 64, hi = ctx->maxlen - 1;                 // lo keeps `most`, hi + 1 does not
+    while (lo < hi) {
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (batched_waves_per_cu(ctx, mid, nullptr) >= most) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* kernel_ms)
 {
     if (!ctx || reps < 1) return fail(ctx, CWHIP_E_ARG, "bad run arguments");
@@ -379,6 +415,9 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     int threads = 64 * wpg, pers_blocks = 0;
     size_t lds = CW_TAB_BYTES + (size_t)wpg * a.wave_lds;
     int64_t slots;
+    CwSplitShape split;
+    std::memset(&split, 0, sizeof split);                // compared byte for byte as a graph key
+    bool use_split = false;
     if (kernel == CWHIP_KERNEL_BATCHED) {
         for (int c = 0; c < CW_NCLASS; ++c) {
             a.rmax_cls[c] = ctx->tune.rmax[c];
@@ -386,7 +425,22 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
         }
         const int rc = batched_shape(ctx, a, ctx->npairs, &pers_blocks, &threads, &lds);
         if (rc) return rc;
-        slots = (int64_t)pers_blocks * wpg;
+        slots = (int64_t)pers_blocks * (threads / 64);
+        // long sequences shrink the waves per CU: pairs with a sequence longer
+        // than the split length run first with a's buffers, the rest after
+        // them with buffers for the split length (more waves per CU)
+        const int T = batched_split_len(ctx);
+        if (T > 0) {
+            a.lsplit = T;
+            split.args = a;
+            split.args.maxlen = T;
+            split.args.lcap = cw_lcap(T);
+            split.args.wave_lds = cw_wave_lds_bytes(split.args.lcap);
+            const int rc2 = batched_shape(ctx, split.args, ctx->npairs, &split.blocks, &split.threads, &split.lds);
+            if (rc2) return rc2;
+            slots = std::max<int64_t>(slots, (int64_t)split.blocks * (split.threads / 64));
+            use_split = true;
+        }
     } else {
         int occ_plain = 0, occ_pers = 0;
         HIPCHK(cw_occupancy(threads, lds, &occ_plain, &occ_pers));
@@ -396,13 +450,14 @@ int cwhip_run_staged(cwhip_ctx* ctx, int kernel, int reps, float* ms, float* ker
     }
     HIPCHK(ctx->d_scratch.ensure((size_t)slots * a.scratch_stride));
     a.scratch = ctx->d_scratch.p;
+    split.args.scratch = ctx->d_scratch.p;
     for (int rep = 0; rep < reps; ++rep) {
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
         if (kernel == CWHIP_KERNEL_BATCHED) {
             // queue reset, binning and the persistent kernel form one graph
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
             a.pair_base = 0;
-            const int rc = launch_batched(ctx, a, pers_blocks, threads, lds);
+            const int rc = launch_batched(ctx, a, pers_blocks, threads, lds, use_split ? &split : nullptr);
             if (rc) return rc;
         } else {
             HIPCHK(cw_launch_reset(ctx->stream, ctx->d_queue.p, ctx->d_status.p));

@@ -9,8 +9,8 @@
 // with that device current, and is kept per device.
 //
 // Layout of the array (int32 units):
-//   [0]                    queue counter of the persistent kernel
-//   [64, 64 + 2*NBINS)     bin counts and cursors
+//   [0]                    queue counters of the persistent kernel
+//   [64, 64 + BINS_INTS)   bin counts and cursors, the long part's size and the total
 //   [kOrderAt, + npairs)   the binned work order
 //   [rows_at, ...)         per wave 8 DP boundary rows of rowcap ints
 // The queue and the bins are cleared by the first kernel of every call; the
@@ -21,14 +21,15 @@
 
 #include "cwhip_internal.h"
 
-hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a);
+hipError_t cw_launch_batched(int blocks, int threads, size_t lds, hipStream_t st, const DevArgs& a,
+                             const CwSplitShape* s);
 hipError_t cw_occupancy(int threads, size_t lds, int* plain_blocks, int* pers_blocks);
 
 namespace {
 
 constexpr int64_t kScratchInts = CWHIP_NOALLOC_SCRATCH_BYTES / 4;
 constexpr int64_t kBinsAt = 64;
-constexpr int64_t kOrderAt = (kBinsAt + 2 * CW_NBINS + 63) & ~int64_t(63);
+constexpr int64_t kOrderAt = (kBinsAt + CW_BINS_INTS + 63) & ~int64_t(63);
 constexpr int kWavesPerGroup = CW_MAX_THREADS / 64;
 constexpr int kMaxDevices = 64;
 
@@ -101,6 +102,8 @@ extern "C" int cwhip_pairwise_noalloc(const cwhip_pair_dev_params* p, const uint
     a.rmax = CW_RMAX;
     a.maxlen = p->max_len;
     a.nseq = nseq;
+    a.lsplit = 0x7fffffff;                  // one launch over the whole queue
+    a.qpart = 0;
     a.lcap = cw_lcap(p->max_len);
     a.wave_lds = cw_wave_lds_bytes(a.lcap);
     a.rowcap = ((p->max_len + 2) + 63) & ~63;
@@ -133,6 +136,7 @@ extern "C" int cwhip_pairwise_noalloc(const cwhip_pair_dev_params* p, const uint
     blocks = std::min(blocks, (npairs + wpg - 1) / wpg);
     blocks = std::min(blocks, fit);
     if (blocks < 1) blocks = 1;   // npairs == 0: the reset still clears out_status
-    return cw_launch_batched((int)blocks, threads, lds, (hipStream_t)stream, a) == hipSuccess ? CWHIP_OK
-                                                                                           : CWHIP_E_HIP;
+    return cw_launch_batched((int)blocks, threads, lds, (hipStream_t)stream, a, nullptr) == hipSuccess
+               ? CWHIP_OK
+               : CWHIP_E_HIP;
 }

@@ -74,6 +74,40 @@ def test_binned_queue_matches_plain_and_oracle(job, plain):
         ctx.close()
 
 
+@pytest.mark.parametrize("nlong", [0, 1, 3])
+def test_long_and_short_parts(nlong):
+    """Pairs with a sequence longer than the length at which the batched kernel
+    still runs its most waves per CU are queued first and run in a launch of
+    their own; the rest run after them with smaller sequence buffers.  Sets with
+    no, one and three 2,400-2,900-residue sequences among short ones: every
+    count equals the plain kernel's and a sample the oracle's."""
+    rng = np.random.default_rng(40 + nlong)
+    fam = seqgen.workload("uneven")
+    lens_all = np.array([len(s) for s in fam.seqs])
+    long_ix = np.flatnonzero((lens_all > 2400) & (lens_all < 2900))[:nlong]
+    short_ix = rng.choice(np.flatnonzero(lens_all < 700), 60, replace=False)
+    seqs = [fam.seqs[i] for i in np.concatenate([long_ix, short_ix])]
+    assert len(seqs) == 60 + nlong
+    lens = np.array([len(x) for x in seqs], np.int32)
+    offs = np.zeros(len(seqs), np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate(seqs).astype(np.uint8)
+    pairs = O.range_pairs(len(lens), 0, len(lens), 0, len(lens))
+    params = cwhip.pair_params(max_len=int(lens.max()))
+    ctx = fresh_ctx(codes, offs, lens, params)
+    try:
+        cnt_p, dist_p = ctx.pairwise_pairs(params, pairs, kernel=cwhip.KERNEL_PLAIN)
+        for _ in range(2):                       # eager, then captured
+            cnt, dist = ctx.pairwise_pairs(params, pairs, kernel=cwhip.KERNEL_BATCHED)
+            np.testing.assert_array_equal(cnt, cnt_p)
+            np.testing.assert_array_equal(dist.view(np.uint32), dist_p.view(np.uint32))
+    finally:
+        ctx.close()
+    sample = np.arange(0, len(pairs), 7)
+    cnt_o, _ = O.pairwise_pairs(codes, offs, lens, pairs[sample], SC, nthreads=16)
+    np.testing.assert_array_equal(cnt_p[sample], cnt_o)
+
+
 def test_graph_replays_match_eager(job, plain):
     # the captured stages are replayed several times with other device work and
     # host synchronisation between the replays; every replay must give the
