@@ -12,12 +12,15 @@
 #define CWP_NODE_INTS 8       // node record: A B M N flags type midj child
 #define CWP_SCR_ROWS 8        // per-wave scratch rows: 4 ping-pong + 4 final
 #define CWP_LOCAL_INTS 8192   // per-wave LDS for a subtree solved locally (records, profile rows, scores)
+#define CWP_ROOT_WIDE 8       // PrfProb::flags: cw_prf_wide has swept the root's halves
+#define CWP_WIDE_MAXG 8       // cw_prf_wide workgroups (one stripe each) per half
+#define CWP_WIDE_MAXP 64      // profile pairs a batch may have for cw_prf_wide to run
 
 // Per profile pair; offsets are in int32 units into PrfArgs::pool.
 struct PrfProb {
     int32_t L1, L2;           // prfLength1, prfLength2
     int32_t ldS;              // S row stride (L1 + 2)
-    int32_t flags;            // bit0: go1, bit1: go2 of the top call
+    int32_t flags;            // bit0: go1, bit1: go2 of the top call; CWP_ROOT_WIDE
     int32_t node_cap;         // node records available
     int32_t rowcap;           // ints per scratch row (>= L2 + 2)
     int64_t prf1, prf2;       // raw profiles, (L + 2) x CWP_PRF_W
@@ -25,7 +28,7 @@ struct PrfProb {
     int64_t pl1, pl2;         // prefix sums of LENCOL, (L + 2) ints
     int64_t S;                // score matrix S[J * ldS + I] = prfScore(I, J)
     int64_t nodes;            // node records
-    int64_t scratch;          // CWP_WAVES x CWP_SCR_ROWS x rowcap
+    int64_t scratch;          // CWP_WAVES (or 2 CWP_WIDE_MAXG) x CWP_SCR_ROWS x rowcap
     int64_t lvl;              // final rows of a level's half sweeps: 4 x (L2 + 2 + node_cap)
 };
 
@@ -38,6 +41,13 @@ struct PrfArgs {
     int32_t rmax;
     int32_t* local_list;      // {problem, node} of every subtree cw_prf_mm leaves to cw_prf_local
     int32_t* local_ctr;       // [0] entries in local_list, [1] next entry to take (both 0 at launch)
+};
+
+// cw_prf_wide: one task (a one-wave workgroup) per stripe of a root's half.
+struct WideArgs {
+    const int32_t* tasks;     // 4 per task: problem, half, member m, members G of the half
+    int32_t* prog;            // per task: chunks its stripe has flushed (0 at launch)
+    int32_t* err;             // set to 1 when a stripe's wait timed out (0 at launch)
 };
 
 // ---- profile construction on the device ------------------------------------

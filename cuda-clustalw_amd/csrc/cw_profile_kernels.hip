@@ -12,6 +12,10 @@
 //                 to a node table.  The host turns the table into the edit
 //                 script in the reference's depth-first order (progAdd/progDel
 //                 merge runs, so the order of emission matters).
+//   cw_prf_wide   (batches of few pairs, before cw_prf_mm) the two half sweeps
+//                 of each large root as pipelined stripes of one-wave
+//                 workgroups on separate CUs; cw_prf_mm then starts at the
+//                 root's midpoint.
 //
 // Gap model.  With g/l the GAPCOL/LENCOL of a profile position and m = 0 at
 // the profile ends (endGapPenalties off) else all ones:
@@ -160,15 +164,56 @@ template <bool V> struct Tag { static constexpr bool value = V; };
 template <int V> struct Par { static constexpr int value = V; };
 
 // Two waves sharing one half sweep: stripe j reads the bottom row stripe j-1
-// writes, 64 columns at a time.  `pub` (LDS) counts the chunks a stripe has
-// flushed to memory; the next stripe waits on it (`wait`) before it loads a
-// chunk.  A wait that outlasts ~2^24 sleeps sets *err and goes on, so a
-// mistake shows up as a reported error, not a hang.
+// writes, 64 columns at a time.  `pub` counts the chunks a stripe has flushed
+// to memory; the next stripe waits on it (`wait`) before it loads a chunk.  A
+// wait that outlasts ~2^24 sleeps sets *err and goes on, so a mistake shows up
+// as a reported error, not a hang.
 struct StripeSync {
-    const int* wait;   // generic pointers, tested against null; the counters
-    int* pub;          // and the error flag are in LDS (offset 0 is a valid
-    int* err;          // LDS address) and are accessed as such
-};
+    const int* wait;   // generic pointers, tested against null.  Stripes of one
+    int* pub;          // workgroup: the counters and the error flag are in LDS
+    int* err;          // (offset 0 is a valid LDS address) and are accessed as
+};                     // such; stripes of different workgroups: global memory
+
+// Stripes of different workgroups (cw_prf_wide) may run on different XCDs,
+// whose L2 caches are not coherent with each other.  Everything they hand over
+// (chunk rows and counters) is written and read with relaxed agent-scope
+// atomics, which go past the non-coherent caches for that one access (global
+// loads and stores with sc1), so no fence has to write back or invalidate a
+// whole L2.  Order comes from the producer waiting for its chunk stores to
+// complete (vmcnt(0)) before it stores the counter, and from the consumer
+// issuing its chunk loads only after the counter load has returned.
+__device__ __forceinline__ int ld_agent(const int* q)
+{
+    return __hip_atomic_load((CWP_GLB int*)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int* q, int v)
+{
+    __hip_atomic_store((CWP_GLB int*)q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void wait_vm_stores()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// wait_chunks for a producer in another workgroup.  A producer that never
+// publishes (it failed its own wait) ends the wait through the shared error
+// flag, so one failure stops every stripe of the launch within one timeout.
+__device__ __forceinline__ void wait_chunks_glb(const StripeSync& sy, int need)
+{
+    int spins = 0;
+    while (uni(ld_agent(sy.wait)) < need) {
+        if (uni(ld_agent(sy.err)) != 0) break;
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1 << 20)) {
+            st_agent(sy.err, 1);
+            break;
+        }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 {
@@ -200,7 +245,9 @@ __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 // scores S(I_r, J) are loaded four steps ahead into buffer t & 3, which step
 // t-4 refilled after its own use.  MODE_REV also needs col[J-1], the record
 // of column c+1: the one the lane loaded for step t+1.
-template <int R, int MODE, bool DIRECT>
+// X: the stripe above and below run in other workgroups (chunk rows and
+// counters through agent-scope accesses, counters in global memory).
+template <int R, int MODE, bool DIRECT, bool X = false>
 __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int first,
                                         const int* inH, const int* inE, int* outH, int* outE,
                                         StripeSync sy)
@@ -249,11 +296,19 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 
     // lane-0 input chunks: lane l of chunk ci holds column 64*ci + 1 + l
     auto chunk = [&](int ci, int& h, int& e) {
-        if (!DIRECT && sy.wait) wait_chunks(sy, ci + 1);
+        if (!DIRECT && sy.wait) {
+            if constexpr (X) wait_chunks_glb(sy, ci + 1);
+            else wait_chunks(sy, ci + 1);
+        }
         const int col = 64 * ci + 1 + lane;
         if (col <= ncols) {
-            h = ((gcint*)inH)[col];
-            e = first ? NEG : ((gcint*)inE)[col];
+            if (X && sy.wait) {                         // written by the workgroup above
+                h = ld_agent(inH + col);
+                e = ld_agent(inE + col);
+            } else {
+                h = ((gcint*)inH)[col];
+                e = first ? NEG : ((gcint*)inE)[col];
+            }
         } else {
             h = 0;
             e = NEG;
@@ -362,12 +417,22 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                 const int col = oc - 63 + lane;
                 const int cstart = ((oc - 1) & ~63) + 1;
                 if (col >= cstart) {
-                    ((gint*)outH)[col] = coutH;
-                    ((gint*)outE)[col] = coutE;
+                    if constexpr (X) {
+                        st_agent(outH + col, coutH);
+                        st_agent(outE + col, coutE);
+                    } else {
+                        ((gint*)outH)[col] = coutH;
+                        ((gint*)outE)[col] = coutE;
+                    }
                 }
                 if (sy.pub) {
-                    wave_fence();
-                    *(volatile lint*)sy.pub = (oc + 63) >> 6;   // same value from every lane
+                    if constexpr (X) {
+                        wait_vm_stores();
+                        st_agent(sy.pub, (oc + 63) >> 6);
+                    } else {
+                        wave_fence();
+                        *(volatile lint*)sy.pub = (oc + 63) >> 6;   // same value from every lane
+                    }
                 }
             }
         }
@@ -419,12 +484,29 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 // Every stripe runs out of line: its loads' waits are then placed from the
 // stripe's own accesses alone (all global or LDS; a flat access pending in
 // the caller would make every wait for a load a wait for all of them).
-template <int R, int MODE, bool DIRECT = false>
+template <int R, int MODE, bool DIRECT = false, bool X = false>
 __device__ __attribute__((noinline)) void pstripe_call(const PSweep W, int row0, int P, int first,
                                                        const int* inH, const int* inE, int* outH,
                                                        int* outE, StripeSync sy)
 {
-    pstripe<R, MODE, DIRECT>(W, row0, P, first, inH, inE, outH, outE, sy);
+    pstripe<R, MODE, DIRECT, X>(W, row0, P, first, inH, inE, outH, outE, sy);
+}
+
+// A stripe of 64..512 rows whose neighbours run in other workgroups.
+template <int MODE>
+__device__ void run_pstripe_x(int R, const PSweep& W, int row0, int P, int first, const int* inH,
+                              const int* inE, int* outH, int* outE, StripeSync sy)
+{
+    switch (R) {
+    case 1: pstripe_call<1, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 2: pstripe_call<2, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 3: pstripe_call<3, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 4: pstripe_call<4, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 5: pstripe_call<5, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 6: pstripe_call<6, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    case 7: pstripe_call<7, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    default: pstripe_call<8, MODE, false, true>(W, row0, P, first, inH, inE, outH, outE, sy); break;
+    }
 }
 
 template <int MODE>
@@ -524,6 +606,29 @@ __device__ void psweep_part(const PSweep& W, int rmax, int m, int S, int* const*
     }
 }
 
+// Member m of the G one-wave workgroups of a half sweep in cw_prf_wide runs
+// stripe m of G (64..512 rows each).  Member regions as in psweep_part, one
+// output slot each (member r's region at reg0 + r * CWP_SCR_ROWS * rowcap);
+// prog[j] counts stripe j's flushed chunks, in global memory.  Stripe m waits
+// only on stripe m-1, i.e. on a workgroup of lower index.
+template <int MODE>
+__device__ void psweep_wide(const PSweep& W, int m, int G, int* reg0, int rowcap, int* finH, int* finE,
+                            int* prog, int* err)
+{
+    const int nrows = W.nrows;
+    const int base = nrows / G, rem = nrows % G;
+    const int row0 = uni(m * base + (m < rem ? m : rem));
+    const int P = uni(base + (m < rem ? 1 : 0));
+    const int R = uni((P + WAVE - 1) / WAVE);
+    const int64_t stride = (int64_t)CWP_SCR_ROWS * rowcap;
+    const int* inH = m == 0 ? reg0 : reg0 + (m - 1) * stride + 2 * rowcap;
+    const int* inE = inH + rowcap;
+    int* oH = m == G - 1 ? finH : reg0 + m * stride + 2 * rowcap;
+    int* oE = m == G - 1 ? finE : oH + rowcap;
+    const StripeSync sy{m > 0 ? prog + (m - 1) : nullptr, m < G - 1 ? prog + m : nullptr, err};
+    run_pstripe_x<MODE>(R, W, row0, P, m == 0, inH, inE, oH, oE, sy);
+}
+
 // The top boundary row H(0,c), c = 1..N, of a half sweep (:636-641, :684-693).
 template <int MODE>
 __device__ void fill_top(const Prob& p, int A, int B, int M, int N, int* h0)
@@ -597,26 +702,42 @@ __device__ __attribute__((noinline)) void do_leaf(const Prob& p, int* nd, int is
     if (is_root) *out_score = midh;
 }
 
-// Forward (half 0, rows 1..midi) or reverse (half 1, rows M..midi+1) sweep of
-// node nd (M >= 2, N >= 1) into the level buffers.
+// The forward (half 0, rows 1..midi) or reverse (half 1, rows M..midi+1)
+// half sweep of node (A, B, M, N) with gap-open flags `flags`.
+__device__ __forceinline__ PSweep half_def(const Prob& p, int A, int B, int M, int N, int flags, int half,
+                                          int* err)
+{
+    const int go1 = flags & 1, go2 = (flags >> 1) & 1;
+    const int midi = M / 2;
+    PSweep W;
+    W.p = p;
+    W.ncols = N;
+    W.err = (lint*)err;
+    if (half == 0) {
+        W.nrows = midi;
+        W.ib = A; W.id = 1; W.jb = B; W.jd = 1;
+        W.lt0 = go1 ? -open2(p, A + 1, B) : 0;
+        W.ltl = -ext2(p, A + 1, B);
+        W.go2 = 0; W.jedge = 0;
+    } else {
+        W.nrows = M - midi;
+        W.ib = A + M + 1; W.id = -1; W.jb = B + N + 1; W.jd = -1;
+        W.lt0 = 0; W.ltl = 0;
+        W.go2 = go2; W.jedge = B + N;
+    }
+    return W;
+}
+
+// Forward or reverse sweep of node nd (M >= 2, N >= 1) into the level buffers.
 __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, int half, int rmax,
                                                   int* scr, int rowcap, LevelBufs lv, int m, int S,
                                                   int* const* regions, int* prog, int* err)
 {
     const int A = uni(nd[0]), B = uni(nd[1]), M = uni(nd[2]), N = uni(nd[3]);
     const int flags = uni(nd[4]);
-    const int go1 = flags & 1, go2 = (flags >> 1) & 1;
-    const int midi = M / 2;
     const Bufs g{scr, scr + rowcap, scr + 2 * rowcap, scr + 3 * rowcap};
     if (half == 0) {
-        PSweep F;
-        F.p = p;
-        F.nrows = midi; F.ncols = N;
-        F.ib = A; F.id = 1; F.jb = B; F.jd = 1;
-        F.lt0 = go1 ? -open2(p, A + 1, B) : 0;
-        F.ltl = -ext2(p, A + 1, B);
-        F.go2 = 0; F.jedge = 0;
-        F.err = (lint*)err;
+        const PSweep F = half_def(p, A, B, M, N, flags, 0, err);
         if (S > 1) {
             if (m == 0) fill_top<MODE_FWD>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_FWD>(F, rmax, m, S, regions, rowcap, lv.fH, lv.fE, prog, err);
@@ -625,13 +746,7 @@ __device__ __attribute__((noinline)) void do_half(const Prob& p, const int* nd, 
             psweep<MODE_FWD>(F, rmax, g, lv.fH, lv.fE);
         }
     } else {
-        PSweep Rv;
-        Rv.p = p;
-        Rv.nrows = M - midi; Rv.ncols = N;
-        Rv.ib = A + M + 1; Rv.id = -1; Rv.jb = B + N + 1; Rv.jd = -1;
-        Rv.lt0 = 0; Rv.ltl = 0;
-        Rv.go2 = go2; Rv.jedge = B + N;
-        Rv.err = (lint*)err;
+        const PSweep Rv = half_def(p, A, B, M, N, flags, 1, err);
         if (S > 1) {
             if (m == 0) fill_top<MODE_REV>(p, A, B, M, N, regions[0]);
             psweep_part<MODE_REV>(Rv, rmax, m, S, regions, rowcap, lv.rH, lv.rK, prog, err);
@@ -1401,8 +1516,9 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             if (sk >= 2) S = sk;
         }
         // tasks 2S k .. 2S k + 2S - 1: the halves of node lb+k, S members each
-        // (a local subtree or a leaf is one task)
-        const int ntask = 2 * S * (le - lb);
+        // (a local subtree or a leaf is one task); none for a root whose
+        // halves cw_prf_wide has swept
+        const int ntask = (lb == 0 && (pp.flags & CWP_ROOT_WIDE)) ? 0 : 2 * S * (le - lb);
         for (;;) {
             const int got = atomicAdd(&s_ctr, lane == 0 ? 1 : 0);
             const int t = uni(__shfl(got, 0, WAVE));
@@ -1474,6 +1590,38 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         a.out[4 * prob + 2] = s_err;
         a.out[4 * prob + 3] = levels;
     }
+}
+
+// The half sweeps of the roots of large profile pairs (PrfProb::flags has
+// CWP_ROOT_WIDE), before cw_prf_mm: one wave per workgroup, G workgroups per
+// half, each running one stripe, so a pair's level 0 is spread over 2G CUs
+// instead of the four waves of its cw_prf_mm workgroup.  Task b = {problem,
+// half, member m, G}; the members of a half are consecutive workgroups.  The
+// final rows go where cw_prf_mm's level 0 would leave them.
+extern "C" __global__ void __launch_bounds__(WAVE) cw_prf_wide(PrfArgs a, WideArgs w)
+{
+    __shared__ int s_err;
+    const int* tk = w.tasks + 4 * blockIdx.x;
+    const int prob = uni(tk[0]), half = uni(tk[1]), m = uni(tk[2]), G = uni(tk[3]);
+    s_err = 0;                                   // one wave: every lane writes the same value
+    __syncthreads();
+    const PrfProb pp = a.probs[prob];
+    const Prob p = make_prob(a, pp);
+    const int M = pp.L1, N = pp.L2;
+    const int rowcap = pp.rowcap;
+    int* reg0 = a.pool + pp.scratch + (int64_t)half * G * CWP_SCR_ROWS * rowcap;
+    const LevelBufs lv = level_bufs(a.pool + pp.lvl, pp.L2 + 2 + pp.node_cap, 0, 0);
+    int* prog = w.prog + (blockIdx.x - m);
+    const PSweep W = half_def(p, 0, 0, M, N, pp.flags & 3, half, &s_err);
+    if (half == 0) {
+        if (m == 0) fill_top<MODE_FWD>(p, 0, 0, M, N, reg0);
+        psweep_wide<MODE_FWD>(W, m, G, reg0, rowcap, lv.fH, lv.fE, prog, w.err);
+    } else {
+        if (m == 0) fill_top<MODE_REV>(p, 0, 0, M, N, reg0);
+        psweep_wide<MODE_REV>(W, m, G, reg0, rowcap, lv.rH, lv.rK, prog, w.err);
+    }
+    wave_fence();
+    if (uni(s_err) != 0) st_agent(w.err, 1);
 }
 
 // The subtrees cw_prf_mm listed, each solved by one wave in its LDS
@@ -1791,6 +1939,12 @@ hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a)
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 {
     hipLaunchKernelGGL(cw_prf_mm, dim3(nprob), dim3(CWP_THREADS), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t cwp_launch_wide(int ntask, hipStream_t st, const PrfArgs& a, const WideArgs& w)
+{
+    hipLaunchKernelGGL(cw_prf_wide, dim3(ntask), dim3(WAVE), 0, st, a, w);
     return hipGetLastError();
 }
 

@@ -123,6 +123,7 @@ struct cwhip_ctx {
     DevBuf<int32_t> d_bins;
     // profile alignment
     DevBuf<int32_t> p_pool, p_out, p_tiles;
+    DevBuf<int32_t> p_wide;                     // cw_prf_wide: tasks, chunk counters, error flag
     DevBuf<PrfProb> p_probs;
     PinBuf<int32_t> h_prf_in, h_prf_nodes;      // staging of a profile batch's inputs and node tables
     DevBuf<int32_t> f_scratch, f_pairs;         // k-tuple distances: per-lane arrays, pair list
@@ -137,6 +138,7 @@ struct cwhip_ctx {
     PinBuf<uint8_t> h_rows;                     // staging of the rows
     std::vector<PrfSide> h_sides;               // host image of p_sides
     int prf_rmax = CWP_RMAX;
+    int prf_wide_rows = 256;                    // rows per cw_prf_wide stripe (0: no cw_prf_wide)
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;
     std::vector<std::vector<uint8_t>> msa_rows;   // rows of the last cwhip_msa
@@ -149,7 +151,7 @@ struct cwhip_ctx {
     {
         f(d_codes); f(d_offs); f(d_lens); f(d_gapfree); f(d_pairs); f(d_order); f(d_count); f(d_info);
         f(d_matrix); f(d_go); f(d_status); f(d_scratch); f(d_queue); f(d_bins);
-        f(p_pool); f(p_out); f(p_tiles); f(p_probs); f(f_scratch); f(f_pairs); f(f_dist);
+        f(p_pool); f(p_out); f(p_tiles); f(p_wide); f(p_probs); f(f_scratch); f(f_pairs); f(f_dist);
         f(f_ctr); f(t_dist); f(t_desc); f(t_status); f(t_branch); f(p_rows); f(p_sides);
     }
     void bind_account() { for_each_buf([this](auto& b) { b.acct = &mem; }); }

@@ -30,6 +30,7 @@ hipError_t cwp_launch_score(int tiles, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a);
 hipError_t cwp_launch_local(int groups, hipStream_t st, const PrfArgs& a);
+hipError_t cwp_launch_wide(int ntask, hipStream_t st, const PrfArgs& a, const WideArgs& w);
 
 namespace {
 
@@ -214,15 +215,35 @@ void parallel_for(size_t n, F fn)
 struct BatchLayout {
     std::vector<PrfProb> probs;
     std::vector<int32_t> tile_base;
+    std::vector<char> wide_ok;       // scratch for cw_prf_wide's 2 x CWP_WIDE_MAXG stripes
     int64_t in_sz = 0, nodes_sz = 0, total = 0;
     int64_t list = 0, ctr = 0;       // the subtrees left to cw_prf_local, and its two counters
 };
+
+// Rows of the smaller half of a root that cw_prf_wide may sweep: each of its
+// G >= 2 stripes keeps at least one row per lane.
+constexpr int kWideMinHalf = 128;
+constexpr int kWideMinCols = 64;
+
+// cw_prf_wide's members per half for a root of L1 x L2 (0: leave it to
+// cw_prf_mm): about `rows` rows per stripe, 2..CWP_WIDE_MAXG stripes of
+// 64..512 rows in both halves.
+int wide_members(int L1, int L2, int rows)
+{
+    const int h = L1 / 2;                            // forward half; the reverse has h or h + 1
+    if (rows <= 0 || h < kWideMinHalf || L2 < kWideMinCols) return 0;
+    int G = std::min(CWP_WIDE_MAXG, std::max(2, h / rows));
+    G = std::min(G, h / 64);
+    if (G < 2 || (L1 - h + G - 1) / G > 512) return 0;
+    return G;
+}
 
 void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
 {
     const int n = (int)jobs.size();
     lo.probs.assign(n, PrfProb{});
     lo.tile_base.assign(n + 1, 0);
+    lo.wide_ok.assign(n, 0);
     int64_t in_sz = 0, work_sz = 0;
     for (int k = 0; k < n; ++k) {
         const int L1 = jobs[k].L1, L2 = jobs[k].L2;
@@ -248,7 +269,9 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
     for (int k = 0; k < n; ++k) {
         PrfProb& pp = lo.probs[k];
         pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(pp.L2 + 2) * pp.ldS);
-        pp.scratch = work_sz; work_sz += (int64_t)CWP_WAVES * CWP_SCR_ROWS * pp.rowcap;
+        lo.wide_ok[k] = n <= CWP_WIDE_MAXP && wide_members(pp.L1, pp.L2, 64) > 0;
+        const int sets = lo.wide_ok[k] ? std::max(CWP_WAVES, 2 * CWP_WIDE_MAXG) : CWP_WAVES;
+        pp.scratch = work_sz; work_sz += (int64_t)sets * CWP_SCR_ROWS * pp.rowcap;
         pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)pp.L2 + 2 + pp.node_cap));
     }
     for (auto& pp : lo.probs) { pp.S += in_sz; pp.nodes += in_sz; pp.scratch += in_sz; pp.lvl += in_sz; }
@@ -264,9 +287,20 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
 // (queued on the context's stream behind whatever wrote them; ev0 is recorded
 // by the caller before its first kernel), then the node tables back in one
 // copy and the edit scripts on host threads.
-int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>& jobs)
+int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
 {
     const int n = (int)jobs.size();
+    // the roots whose half sweeps cw_prf_wide spreads over CUs
+    std::vector<int32_t> wide;
+    for (int k = 0; k < n; ++k) {
+        PrfProb& pp = lo.probs[k];
+        const int G = lo.wide_ok[k] ? wide_members(pp.L1, pp.L2, ctx->prf_wide_rows) : 0;
+        if (G == 0) continue;
+        pp.flags |= CWP_ROOT_WIDE;
+        for (int half = 0; half < 2; ++half)
+            for (int m = 0; m < G; ++m) wide.insert(wide.end(), {k, half, m, G});
+    }
+    const int nwide = (int)wide.size() / 4;
     HIPCHK(ctx->p_probs.ensure(n));
     HIPCHK(ctx->p_out.ensure(4 * (size_t)n));
     HIPCHK(ctx->p_tiles.ensure(n + 1));
@@ -283,8 +317,19 @@ int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>&
     a.local_list = ctx->p_pool.p + lo.list;
     a.local_ctr = ctx->p_pool.p + lo.ctr;
     HIPCHK(hipMemsetAsync(a.local_ctr, 0, 2 * sizeof(int32_t), ctx->stream));
+    WideArgs wa{};
+    if (nwide > 0) {
+        HIPCHK(ctx->p_wide.ensure(5 * (size_t)nwide + 1));
+        HIPCHK(hipMemcpyAsync(ctx->p_wide.p, wide.data(), wide.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->p_wide.p + 4 * nwide, 0, ((size_t)nwide + 1) * sizeof(int32_t), ctx->stream));
+        wa.tasks = ctx->p_wide.p;
+        wa.prog = ctx->p_wide.p + 4 * nwide;
+        wa.err = wa.prog + nwide;
+    }
     if (lo.tile_base[n] > 0) HIPCHK(cwp_launch_score(lo.tile_base[n], ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    if (nwide > 0) HIPCHK(cwp_launch_wide(nwide, ctx->stream, a, wa));
     HIPCHK(cwp_launch_mm(n, ctx->stream, a));
     HIPCHK(cwp_launch_local(std::max(1, ctx->num_cu), ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
@@ -292,6 +337,9 @@ int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>&
     int32_t* h_nodes = ctx->h_prf_nodes.p;
     int32_t* out = h_nodes + lo.nodes_sz;
     HIPCHK(hipMemcpyAsync(out, ctx->p_out.p, 4 * (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    out[4 * n] = 0;
+    if (nwide > 0)
+        HIPCHK(hipMemcpyAsync(out + 4 * n, wa.err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(h_nodes, ctx->p_pool.p + lo.in_sz, lo.nodes_sz * sizeof(int32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -300,6 +348,7 @@ int dp_launch_collect(cwhip_ctx* ctx, const BatchLayout& lo, std::vector<DpJob>&
     HIPCHK(hipEventElapsedTime(&ms2, ctx->ev1, ctx->ev2));
     ctx->prf_score_ms += ms1;
     ctx->prf_mm_ms += ms2;
+    if (out[4 * n] != 0) return fail(ctx, CWHIP_E_KERNEL, "profile DP stripe hand-off timed out");
     for (int k = 0; k < n; ++k) {
         const int nn = out[4 * k + 1];
         if (out[4 * k + 2] != 0 || nn < 1 || nn > lo.probs[k].node_cap)
