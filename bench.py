@@ -228,7 +228,8 @@ def progressive_stage(cpu: bool):
                        "traffic": traffic, "traffic_source": tsrc,
                        "hbm_gbs": round(traffic / dev_s / 1e9, 1) if traffic else None,
                        "note": "device time of the profile kernels (cw_prf_cols, cw_prf_gapcols, "
-                               "cw_prf_score, cw_prf_mm, cw_prf_local) summed over the "
+                               "cw_prf_score, cw_prf_wide, cw_prf_level, cw_prf_mm, "
+                               "cw_prf_local) summed over the "
                                "dependency-level batches; achieved = 10 ops per Myers-Miller "
                                "cell update + 54 per prfScore entry (each computed once); "
                                "achieved_64_ops_per_cell = 64 ops per cell update; traffic = "

@@ -14,7 +14,8 @@
 #define CWP_LOCAL_INTS 8192   // per-wave LDS for a subtree solved locally (records, profile rows, scores)
 #define CWP_ROOT_WIDE 8       // PrfProb::flags: cw_prf_wide has swept the root's halves
 #define CWP_WIDE_MAXG 8       // cw_prf_wide workgroups (one stripe each) per half
-#define CWP_WIDE_MAXP 64      // profile pairs a batch may have for cw_prf_wide to run
+#define CWP_WIDE_MAXP 512     // profile pairs a batch may have for cw_prf_wide / cw_prf_level to run
+#define CWP_WIDE_SETS 64      // most cw_prf_level tasks (scratch sets of CWP_SCR_ROWS rows) per pair
 
 // Per profile pair; offsets are in int32 units into PrfArgs::pool.
 struct PrfProb {
@@ -41,6 +42,10 @@ struct PrfArgs {
     int32_t rmax;
     int32_t* local_list;      // {problem, node} of every subtree cw_prf_mm leaves to cw_prf_local
     int32_t* local_ctr;       // [0] entries in local_list, [1] next entry to take (both 0 at launch)
+    int32_t* state;           // per problem: lev_b, lev_e, nodes allocated, levels done
+    int32_t resume;           // cw_prf_mm: continue at the level in state (else start at the root)
+    int32_t ext;              // cw_prf_mm: this level's half sweeps are done (cw_prf_level); finish
+                              // it and stop (else sweep and finish every remaining level)
 };
 
 // cw_prf_wide: one task (a one-wave workgroup) per stripe of a root's half.

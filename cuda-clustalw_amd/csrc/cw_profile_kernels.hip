@@ -12,10 +12,16 @@
 //                 to a node table.  The host turns the table into the edit
 //                 script in the reference's depth-first order (progAdd/progDel
 //                 merge runs, so the order of emission matters).
-//   cw_prf_wide   (batches of few pairs, before cw_prf_mm) the two half sweeps
-//                 of each large root as pipelined stripes of one-wave
-//                 workgroups on separate CUs; cw_prf_mm then starts at the
-//                 root's midpoint.
+//   cw_prf_wide   (batches of up to CWP_WIDE_MAXP pairs) the two half sweeps of
+//                 each large root as pipelined stripes of one-wave workgroups
+//                 on separate CUs.
+//   cw_prf_level  (same batches) the half sweeps of the next levels' nodes, one
+//                 wave per half over the chip.  cw_prf_mm then finishes one
+//                 level per launch (leaves, local subtrees, midpoints,
+//                 children) and, in its last launch, sweeps the remaining
+//                 levels itself.  A lone wave on a CU sweeps about twice as many
+//                 cells per cycle as one of the four that share a CU's texture
+//                 path in cw_prf_mm.
 //
 // Gap model.  With g/l the GAPCOL/LENCOL of a profile position and m = 0 at
 // the profile ends (endGapPenalties off) else all ones:
@@ -1474,22 +1480,33 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
     const int lane = lane_id();
     int* scr = a.pool + pp.scratch + (int64_t)wave * CWP_SCR_ROWS * pp.rowcap;
     const int cap = pp.node_cap;
+    int* state = a.state + 4 * prob;
     if (threadIdx.x == 0) {
-        int* nd = nodes;
-        nd[0] = 0; nd[1] = 0; nd[2] = pp.L1; nd[3] = pp.L2;
-        nd[4] = pp.flags & 3; nd[5] = 0; nd[6] = 0; nd[7] = -1;
-        s_lev_b = 0;
-        s_lev_e = 1;
-        s_alloc = 1;
-        s_err = 0;
+        if (a.resume) {                          // the level a previous launch stopped at
+            s_lev_b = state[0];
+            s_lev_e = state[1];
+            s_alloc = state[2];
+            s_err = a.out[4 * prob + 2];
+        } else {
+            int* nd = nodes;
+            nd[0] = 0; nd[1] = 0; nd[2] = pp.L1; nd[3] = pp.L2;
+            nd[4] = pp.flags & 3; nd[5] = 0; nd[6] = 0; nd[7] = -1;
+            s_lev_b = 0;
+            s_lev_e = 1;
+            s_alloc = 1;
+            s_err = 0;
+        }
     }
     __syncthreads();
     int* lvl = a.pool + pp.lvl;
     const int lcap = pp.L2 + 2 + cap;
-    int levels = 0;
+    int levels = a.resume ? uni(state[3]) : 0;
     for (;;) {
         const int lb = s_lev_b, le = s_lev_e;
         if (lb >= le) break;
+        // the half sweeps of this level are already in the level buffers:
+        // the root's from cw_prf_wide, a later level's from cw_prf_level
+        const bool swept = lb == 0 ? (pp.flags & CWP_ROOT_WIDE) != 0 : a.ext != 0;
         __syncthreads();
         for (int k = threadIdx.x; k < kMaxSplitNodes * 2 * kMaxSplitStripes; k += blockDim.x) s_prog[k] = 0;
         if (threadIdx.x == 0) s_ctr = 0;
@@ -1501,7 +1518,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         // the smaller half has >= 64 S rows, and a stripe holds up to 128 or
         // more.
         int S = 1;
-        if (le - lb <= kMaxSplitNodes) {
+        if (!swept && le - lb <= kMaxSplitNodes) {
             int sk = CWP_WAVES / (2 * (le - lb));
             for (int k = lb; k < le; ++k) {
                 const int* ndk = nodes + (int64_t)k * CWP_NODE_INTS;
@@ -1516,9 +1533,8 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             if (sk >= 2) S = sk;
         }
         // tasks 2S k .. 2S k + 2S - 1: the halves of node lb+k, S members each
-        // (a local subtree or a leaf is one task); none for a root whose
-        // halves cw_prf_wide has swept
-        const int ntask = (lb == 0 && (pp.flags & CWP_ROOT_WIDE)) ? 0 : 2 * S * (le - lb);
+        // (a local subtree or a leaf is one task)
+        const int ntask = 2 * S * (le - lb);
         for (;;) {
             const int got = atomicAdd(&s_ctr, lane == 0 ? 1 : 0);
             const int t = uni(__shfl(got, 0, WAVE));
@@ -1531,7 +1547,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
                 if (within == 0) defer_local(a, prob, idx);
             } else if (N <= 0 || M <= 1) {
                 if (within == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
-            } else {
+            } else if (!swept) {
                 // a split half's members use the scratch of waves
                 // (2 (idx - lb) + half) S .. + S - 1 (no wave sweeps alone then)
                 const int grp = 2 * (idx - lb) + half;
@@ -1584,12 +1600,44 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
         }
         __syncthreads();
         ++levels;
+        if (a.ext) break;                        // one level per launch: the next one's sweeps come first
     }
     if (threadIdx.x == 0) {
+        state[0] = s_lev_b;
+        state[1] = s_lev_e;
+        state[2] = s_alloc;
+        state[3] = levels;
         a.out[4 * prob + 1] = s_alloc;
         a.out[4 * prob + 2] = s_err;
         a.out[4 * prob + 3] = levels;
     }
+}
+
+// The half sweeps of one level of every pair of the batch, one wave per half
+// (cw_prf_mm with PrfArgs::ext then finishes the level): workgroup b works on
+// pair b / T, half (b % T) & 1 of node lev_b + (b % T) / 2 of the level the
+// pair's state names, if that node is neither a leaf nor a local subtree.  T
+// covers the level's nodes (at most 2^k at level k) and is at most the
+// scratch sets a pair has.
+extern "C" __global__ void __launch_bounds__(WAVE) cw_prf_level(PrfArgs a, int T)
+{
+    __shared__ int s_err;
+    const int prob = blockIdx.x / T, t = blockIdx.x % T;
+    const int* state = a.state + 4 * prob;
+    const int lb = uni(state[0]), le = uni(state[1]);
+    const int idx = lb + t / 2, half = t & 1;
+    if (idx >= le) return;                       // the whole (one-wave) workgroup
+    const PrfProb pp = a.probs[prob];
+    int* nd = a.pool + pp.nodes + (int64_t)idx * CWP_NODE_INTS;
+    const int M = uni(nd[2]), N = uni(nd[3]);
+    if (local_fits(M, N) || N <= 0 || M <= 1) return;
+    s_err = 0;                                   // every lane writes the same value
+    __syncthreads();
+    const Prob p = make_prob(a, pp);
+    int* scr = a.pool + pp.scratch + (int64_t)t * CWP_SCR_ROWS * pp.rowcap;
+    const LevelBufs lv = level_bufs(a.pool + pp.lvl, pp.L2 + 2 + pp.node_cap, uni(nd[1]), idx - lb);
+    do_half(p, nd, half, a.rmax, scr, pp.rowcap, lv, 0, 1, nullptr, nullptr, &s_err);
+    if (uni(s_err) != 0) a.out[4 * prob + 2] = 1;
 }
 
 // The half sweeps of the roots of large profile pairs (PrfProb::flags has
@@ -1945,6 +1993,12 @@ hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a)
 hipError_t cwp_launch_wide(int ntask, hipStream_t st, const PrfArgs& a, const WideArgs& w)
 {
     hipLaunchKernelGGL(cw_prf_wide, dim3(ntask), dim3(WAVE), 0, st, a, w);
+    return hipGetLastError();
+}
+
+hipError_t cwp_launch_level(int nprob, int T, hipStream_t st, const PrfArgs& a)
+{
+    hipLaunchKernelGGL(cw_prf_level, dim3(nprob * T), dim3(WAVE), 0, st, a, T);
     return hipGetLastError();
 }
 

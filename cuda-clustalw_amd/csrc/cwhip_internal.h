@@ -139,6 +139,8 @@ struct cwhip_ctx {
     std::vector<PrfSide> h_sides;               // host image of p_sides
     int prf_rmax = CWP_RMAX;
     int prf_wide_rows = 256;                    // rows per cw_prf_wide stripe (0: no cw_prf_wide)
+    int prf_wide_levels = 5;                    // levels before cw_prf_mm's last launch (<= 6; 1: none)
+    int prf_wide_pairs = CWP_WIDE_MAXP;         // batches of at most this many pairs use them
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;
     std::vector<std::vector<uint8_t>> msa_rows;   // rows of the last cwhip_msa

@@ -31,6 +31,7 @@ hipError_t cwp_launch_mm(int nprob, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_build(int ncolblk, int nside, hipStream_t st, const BuildArgs& a);
 hipError_t cwp_launch_local(int groups, hipStream_t st, const PrfArgs& a);
 hipError_t cwp_launch_wide(int ntask, hipStream_t st, const PrfArgs& a, const WideArgs& w);
+hipError_t cwp_launch_level(int nprob, int T, hipStream_t st, const PrfArgs& a);
 
 namespace {
 
@@ -215,15 +216,25 @@ void parallel_for(size_t n, F fn)
 struct BatchLayout {
     std::vector<PrfProb> probs;
     std::vector<int32_t> tile_base;
-    std::vector<char> wide_ok;       // scratch for cw_prf_wide's 2 x CWP_WIDE_MAXG stripes
+    bool wide = false;               // few pairs: CWP_WIDE_SETS scratch sets each, for
+                                     // cw_prf_wide and cw_prf_level
     int64_t in_sz = 0, nodes_sz = 0, total = 0;
     int64_t list = 0, ctr = 0;       // the subtrees left to cw_prf_local, and its two counters
+    int64_t state = 0;               // cw_prf_mm's level state, 4 per pair
 };
 
 // Rows of the smaller half of a root that cw_prf_wide may sweep: each of its
 // G >= 2 stripes keeps at least one row per lane.
 constexpr int kWideMinHalf = 128;
 constexpr int kWideMinCols = 64;
+
+// cw_prf_level's tasks per pair at level k >= 1: both halves of up to 2^k nodes
+int wide_tasks(int k) { return k < 1 ? 0 : 2 << k; }
+// levels whose half sweeps run before cw_prf_mm's last launch (the root's
+// through cw_prf_wide): every task of the deepest one has a scratch set
+constexpr int kWideMaxLevels = 6;
+static_assert(2 << (kWideMaxLevels - 1) <= CWP_WIDE_SETS, "scratch sets for cw_prf_level");
+int wide_levels(const cwhip_ctx* ctx) { return std::max(1, std::min(ctx->prf_wide_levels, kWideMaxLevels)); }
 
 // cw_prf_wide's members per half for a root of L1 x L2 (0: leave it to
 // cw_prf_mm): about `rows` rows per stripe, 2..CWP_WIDE_MAXG stripes of
@@ -238,12 +249,14 @@ int wide_members(int L1, int L2, int rows)
     return G;
 }
 
-void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
+void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo, const cwhip_ctx* ctx)
 {
     const int n = (int)jobs.size();
     lo.probs.assign(n, PrfProb{});
     lo.tile_base.assign(n + 1, 0);
-    lo.wide_ok.assign(n, 0);
+    lo.wide = n <= ctx->prf_wide_pairs;
+    // cw_prf_wide's 2 CWP_WIDE_MAXG stripes, cw_prf_level's 2^levels tasks per pair
+    const int wide_sets = std::max({CWP_WAVES, 2 * CWP_WIDE_MAXG, wide_tasks(wide_levels(ctx) - 1)});
     int64_t in_sz = 0, work_sz = 0;
     for (int k = 0; k < n; ++k) {
         const int L1 = jobs[k].L1, L2 = jobs[k].L2;
@@ -269,8 +282,7 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
     for (int k = 0; k < n; ++k) {
         PrfProb& pp = lo.probs[k];
         pp.S = work_sz; work_sz = up4(work_sz + (int64_t)(pp.L2 + 2) * pp.ldS);
-        lo.wide_ok[k] = n <= CWP_WIDE_MAXP && wide_members(pp.L1, pp.L2, 64) > 0;
-        const int sets = lo.wide_ok[k] ? std::max(CWP_WAVES, 2 * CWP_WIDE_MAXG) : CWP_WAVES;
+        const int sets = lo.wide ? wide_sets : CWP_WAVES;
         pp.scratch = work_sz; work_sz += (int64_t)sets * CWP_SCR_ROWS * pp.rowcap;
         pp.lvl = work_sz; work_sz = up4(work_sz + 4 * ((int64_t)pp.L2 + 2 + pp.node_cap));
     }
@@ -280,7 +292,8 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo)
     lo.in_sz = in_sz;
     lo.list = in_sz + work_sz;
     lo.ctr = up4(lo.list + 2 * roots);
-    lo.total = lo.ctr + 4;
+    lo.state = lo.ctr + 4;
+    lo.total = lo.state + up4(4 * (int64_t)n);
 }
 
 // The score and Myers-Miller kernels over a batch whose inputs are in the pool
@@ -294,7 +307,7 @@ int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
     std::vector<int32_t> wide;
     for (int k = 0; k < n; ++k) {
         PrfProb& pp = lo.probs[k];
-        const int G = lo.wide_ok[k] ? wide_members(pp.L1, pp.L2, ctx->prf_wide_rows) : 0;
+        const int G = lo.wide ? wide_members(pp.L1, pp.L2, ctx->prf_wide_rows) : 0;
         if (G == 0) continue;
         pp.flags |= CWP_ROOT_WIDE;
         for (int half = 0; half < 2; ++half)
@@ -316,6 +329,9 @@ int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
     a.rmax = ctx->prf_rmax;
     a.local_list = ctx->p_pool.p + lo.list;
     a.local_ctr = ctx->p_pool.p + lo.ctr;
+    a.state = ctx->p_pool.p + lo.state;
+    a.resume = 0;
+    a.ext = 0;
     HIPCHK(hipMemsetAsync(a.local_ctr, 0, 2 * sizeof(int32_t), ctx->stream));
     WideArgs wa{};
     if (nwide > 0) {
@@ -330,6 +346,18 @@ int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
     if (lo.tile_base[n] > 0) HIPCHK(cwp_launch_score(lo.tile_base[n], ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     if (nwide > 0) HIPCHK(cwp_launch_wide(nwide, ctx->stream, a, wa));
+    if (lo.wide && wide_levels(ctx) > 1) {
+        // levels 1 .. prf_wide_levels-1: their half sweeps one wave each over
+        // the chip, then cw_prf_mm finishes the level; the rest in one launch
+        a.ext = 1;
+        HIPCHK(cwp_launch_mm(n, ctx->stream, a));
+        a.resume = 1;
+        for (int k = 1; k < wide_levels(ctx); ++k) {
+            HIPCHK(cwp_launch_level(n, wide_tasks(k), ctx->stream, a));
+            HIPCHK(cwp_launch_mm(n, ctx->stream, a));
+        }
+        a.ext = 0;
+    }
     HIPCHK(cwp_launch_mm(n, ctx->stream, a));
     HIPCHK(cwp_launch_local(std::max(1, ctx->num_cu), ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev2, ctx->stream));
@@ -383,7 +411,7 @@ int run_dp_batch(cwhip_ctx* ctx, std::vector<DpJob>& jobs, int egp)
     if (n == 0) return CWHIP_OK;
     HIPCHK(hipSetDevice(ctx->device));
     BatchLayout lo;
-    layout_batch(jobs, lo);
+    layout_batch(jobs, lo, ctx);
     for (int k = 0; k < n; ++k)
         lo.probs[k].flags = (jobs[k].prf1[GAPCOL] != 0 ? 1 : 0) |
                             (jobs[k].prf1[(int64_t)jobs[k].L1 * W + GAPCOL] != 0 ? 2 : 0);
@@ -698,7 +726,7 @@ int run_built_batch(cwhip_ctx* ctx, const cwhip_msa_params& P, const Seqs& seqs,
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<DpJob> jobs(n);
     for (int k = 0; k < n; ++k) jobs[k] = st[k]->job;
-    layout_batch(jobs, lo);
+    layout_batch(jobs, lo, ctx);
     const bool egp = P.end_gap_penalties != 0;
     for (int k = 0; k < n; ++k)        // nonzero-ness of prf1 GAPCOL at rows 0 and L1 (the top call's go1/go2)
         lo.probs[k].flags = (egp && st[k]->gapcoef != 0 ? 1 : 0) |
