@@ -940,14 +940,6 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
         hup[c] = grp ? (-(rm.z & (rm.x + cF[c].x)) - (rm.z & (p2e - q.pl2(J)))) : (t0 + jj * tl);
         vst[c] = NEG;
     }
-    // left boundary H of half-row i (lane L == 0 only)
-    auto left = [&](int i) -> int {
-        const int ic = i < 1 ? 1 : (i > M ? M : i);
-        const int I = A + M + 1 - ic;                // a valid row for either half
-        const int g = go2 ? -(ce.z & (q.row(I).x + ce.x)) : 0;
-        const int rv = g - (ce.z & (p1e - q.pl1(I)));
-        return i <= 0 ? 0 : (grp ? rv : lt0 + i * ltl);
-    };
     int dleft;
     if (L == 0) {
         dleft = 0;
@@ -956,26 +948,37 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
         const int Jl = grp ? (B + N + 1 - (jl <= N ? jl : N)) : 0;
         dleft = grp ? (-(rm.z & (rm.x + q.col(Jl).x)) - (rm.z & (p2e - q.pl2(Jl)))) : (t0 + jl * tl);
     }
-    // row data of the lane's row, one step ahead
+    // S offsets of the lane's columns: S(I, J_c) = sc[so[c] + I]
+    int so[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int jj = L * C + c + 1;
+        const int jc = jj <= N ? jj : N;
+        const int J = grp ? (B + N + 1 - jc) : (B + jc);
+        so[c] = (J - q.B0 - 1) * q.M0 - q.A0 - 1;
+    }
+    // row data of the lane's row (records, pl1, scores), one step ahead
     auto rowI = [&](int i) {
         const int ic = i < 1 ? 1 : (i > rows ? rows : i);
         return grp ? (A + M + 1 - ic) : (A + ic);
     };
     int4 nrf, nre;
+    int npl;
     int ns[C];
     auto fetch = [&](int i) {
         const int I = rowI(i);
         nre = q.row(I);
         nrf = q.row(grp ? I - 1 : I);
+        npl = q.pl1(I);
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const int jj = L * C + c + 1;
-            const int jc = jj <= N ? jj : N;
-            const int J = grp ? (B + N + 1 - jc) : (B + jc);
-            ns[c] = q.S(I, J);
-        }
+        for (int c = 0; c < C; ++c) ns[c] = q.sc[so[c] + I];
     };
     fetch(1 - L);
+    // H of the left column at the lane-0 rows: forward lt0 + i*ltl, reverse
+    // (go2 ? -open2(I, B+N) : 0) - (m2[B+N] & (pl1[A+M+1] - pl1[I])) with I the
+    // lane's own row, whose record and pl1 the fetch brought; lprev: row i-1
+    const int go2m = go2 ? -1 : 0;
+    int lprev = 0;
     int hl_out = 0, fl_out = NEG;
     const int T = (midi > M - midi ? midi : M - midi) + nl - 1;
     for (int t = 1; t <= T; ++t) {
@@ -983,14 +986,19 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
         int fl = shr1(NEG, fl_out);
         const int i = t - L;
         const int4 rf = nrf, re = nre;
+        const int plc = npl;
         int s[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) s[c] = ns[c];
         fetch(i + 1);
         if (L == 0) {
-            hl = left(i);
+            const int lv = i <= 0 ? 0
+                           : grp ? (-(go2m & ce.z & (re.x + ce.x)) - (ce.z & (p1e - plc)))
+                                 : lt0 + i * ltl;
+            hl = lv;
             fl = NEG;
-            dleft = left(i - 1);
+            dleft = lprev;
+            lprev = lv;
         }
         if (i >= 1 && i <= rows && L < nl) {
             int d = dleft, hc = hl, fc = fl;
