@@ -169,6 +169,19 @@ __device__ __forceinline__ int left_h(const PSweep& W, int i) {
 template <bool V> struct Tag { static constexpr bool value = V; };
 template <int V> struct Par { static constexpr int value = V; };
 
+// f(Par<0>{}), f(Par<1>{}), ..., f(Par<N-1>{}) (compile-time unrolled)
+template <int N, int I = 0>
+struct Unroll {
+    template <typename F>
+    __device__ __forceinline__ static void run(F&& f)
+    {
+        if constexpr (I < N) {
+            f(Par<I>{});
+            Unroll<N, I + 1>::run(f);
+        }
+    }
+};
+
 // Two waves sharing one half sweep: stripe j reads the bottom row stripe j-1
 // writes, 64 columns at a time.  `pub` counts the chunks a stripe has flushed
 // to memory; the next stripe waits on it (`wait`) before it loads a chunk.  A
@@ -248,9 +261,11 @@ __device__ __forceinline__ void wait_chunks(const StripeSync& sy, int need)
 // blocks in which every lane stays left of that column leave it out too.
 //
 // At step t lane L works on column c = t - L.  Its column record and its R
-// scores S(I_r, J) are loaded four steps ahead into buffer t & 3, which step
-// t-4 refilled after its own use.  MODE_REV also needs col[J-1], the record
-// of column c+1: the one the lane loaded for step t+1.
+// scores S(I_r, J) are loaded D steps ahead into buffer t % D, which step t-D
+// refilled after its own use (D = 8 up to four rows per lane, else 4: a lone
+// wave on a CU waits out most of a score load's latency otherwise).  MODE_REV
+// also needs col[J-1], the record of column c+1: the one the lane loaded for
+// step t+1.
 // X: the stripe above and below run in other workgroups (chunk rows and
 // counters through agent-scope accesses, counters in global memory).
 template <int R, int MODE, bool DIRECT, bool X = false>
@@ -259,6 +274,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                                         StripeSync sy)
 {
     constexpr int DIR = (MODE == MODE_FWD) ? 1 : -1;
+    constexpr int D = R <= 4 ? 8 : 4;
     const int lane = lane_id();
     // wave-uniform values in scalar registers, so the control of the step
     // loops (chunk turns, flushes, bounds) runs on the scalar unit
@@ -327,17 +343,17 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
 
     int outHv = 0, outEv = NEG, coutH = 0, coutE = 0;
 
-    // The column record and scores of step t go into rec/sv[t & 3], read
-    // through pointers cp/sp[t & 3] to the record and to S(I0, J).  Columns
+    // The column record and scores of step t go into rec/sv[t % D], read
+    // through pointers cp/sp[t % D] to the record and to S(I0, J).  Columns
     // are clamped to 0 .. ncols+1, which keeps J and every S index inside the
     // pair's arrays.  Within a block whose lanes all stay on columns
-    // 1 .. ncols-1 and that starts at t0 with t0 + 66 <= ncols, no load is
-    // clamped (t+4 - lane <= ncols+1), so a pointer moves on by four columns
-    // per use.
-    int4 rec[4];
-    int sv[4][R];
-    gcint4* cp[4];
-    gcint* sp[4];
+    // 1 .. ncols-1 and that starts at t0 with t0 + 62 + D <= ncols, no load is
+    // clamped (t+D - lane <= ncols+1), so a pointer moves on by D columns per
+    // use.
+    int4 rec[D];
+    int sv[D][R];
+    gcint4* cp[D];
+    gcint* sp[D];
     auto load = [&](int t, auto par, auto from_scratch) {
         constexpr int Q = decltype(par)::value;
         if constexpr (decltype(from_scratch)::value) {
@@ -346,19 +362,19 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
             cp[Q] = p.col + J;
             sp[Q] = p.S + (__umul24((unsigned)J, ldS) + sI0);
         } else {
-            cp[Q] += 4 * DIR;
-            sp[Q] += 4 * DIR * (int)ldS;
+            cp[Q] += D * DIR;
+            sp[Q] += D * DIR * (int)ldS;
         }
         rec[Q] = ld4(cp[Q]);
 #pragma unroll
         for (int r = 0; r < R; ++r) sv[Q][r] = sp[Q][DIR * r];
     };
-    load(1, Par<1>{}, Tag<true>{});
-    load(2, Par<2>{}, Tag<true>{});
-    load(3, Par<3>{}, Tag<true>{});
-    load(4, Par<0>{}, Tag<true>{});
+    Unroll<D>::run([&](auto i) {
+        constexpr int I = decltype(i)::value;
+        load(I + 1, Par<(I + 1) % D>{}, Tag<true>{});
+    });
 
-    // par: t & 3.  masked: some lane may be off its columns, or on the last
+    // par: t % D.  masked: some lane may be off its columns, or on the last
     // one (exec masking and the column mask).  flush: this step may end an
     // output chunk (every 64th output column, and the last one).
     auto step = [&](int t, auto par, auto masked, auto flush) {
@@ -371,7 +387,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
         cinH = nH;
         cinE = nE;
         const int4 cF = rec[Q];
-        const int4 cE = (MODE == MODE_FWD) ? rec[Q] : rec[(Q + 1) & 3];
+        const int4 cE = (MODE == MODE_FWD) ? rec[Q] : rec[(Q + 1) % D];
         const int* const sc = sv[Q];
         const int c = t - lane;
         bool act = true;
@@ -414,7 +430,7 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
                 ((gint*)outE)[c] = eup;
             }
         }
-        load(t + 4, par, Tag<MK>{});                   // buffer Q is free again
+        load(t + D, par, Tag<MK>{});                   // buffer Q is free again
         if (!DIRECT) {
             coutH = shl1(outHv, coutH);
             coutE = shl1(outEv, coutE);
@@ -445,11 +461,11 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
     };
 
     // Steps run in blocks of 64: block b covers t = 64b+1 .. 64b+64 (t = 1
-    // mod 4 at its start, so steps take buffers 1, 2, 3, 0 in turn), the
+    // mod D at its start, so steps take buffers 1, 2, .., D-1, 0 in turn), the
     // lane-0 input chunk turns at its start.  A block in which every lane
     // stays on columns 1 .. ncols-1 and no load is clamped (t0 >= 64,
-    // t0 + 66 <= ncols) runs without exec masking or column mask and flushes
-    // its output chunk only after its step 64b+63 (output column 64b).
+    // t0 + 62 + D <= ncols) runs without exec masking or column mask and
+    // flushes its output chunk only after its step 64b+63 (output column 64b).
     const int tend = DIRECT ? ncols + P - 1 : ncols + 63;
     for (int b = 0; 64 * b + 1 <= tend; ++b) {
         const int t0 = 64 * b + 1;
@@ -459,30 +475,29 @@ __device__ __forceinline__ void pstripe(const PSweep& W, int row0, int P, int fi
             if (b + 1 < nchunks) chunk(b + 1, nxtH, nxtE);
         }
         const int t1 = t0 + 63 < tend ? t0 + 63 : tend;
-        if (!DIRECT && t0 >= 64 && t0 + 66 <= ncols) {
+        if (!DIRECT && t0 >= 64 && t0 + 62 + D <= ncols) {
 #pragma unroll 1
-            for (int k = 0; k < 60; k += 4) {
-                step(t0 + k, Par<1>{}, Tag<false>{}, Tag<false>{});
-                step(t0 + k + 1, Par<2>{}, Tag<false>{}, Tag<false>{});
-                step(t0 + k + 2, Par<3>{}, Tag<false>{}, Tag<false>{});
-                step(t0 + k + 3, Par<0>{}, Tag<false>{}, Tag<false>{});
-            }
-            step(t0 + 60, Par<1>{}, Tag<false>{}, Tag<false>{});
-            step(t0 + 61, Par<2>{}, Tag<false>{}, Tag<false>{});
-            step(t0 + 62, Par<3>{}, Tag<false>{}, Tag<true>{});
-            step(t0 + 63, Par<0>{}, Tag<false>{}, Tag<false>{});
+            for (int k = 0; k < 64 - D; k += D)
+                Unroll<D>::run([&](auto i) {
+                    constexpr int I = decltype(i)::value;
+                    step(t0 + k + I, Par<(I + 1) % D>{}, Tag<false>{}, Tag<false>{});
+                });
+            Unroll<D>::run([&](auto i) {                 // the flush after step t0 + 62
+                constexpr int I = decltype(i)::value;
+                step(t0 + 64 - D + I, Par<(I + 1) % D>{}, Tag<false>{}, Tag<64 - D + I == 62>{});
+            });
         } else {
             int t = t0;
 #pragma unroll 1
-            for (; t + 3 <= t1; t += 4) {
-                step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
-                step(t + 1, Par<2>{}, Tag<true>{}, Tag<true>{});
-                step(t + 2, Par<3>{}, Tag<true>{}, Tag<true>{});
-                step(t + 3, Par<0>{}, Tag<true>{}, Tag<true>{});
-            }
-            if (t <= t1) step(t, Par<1>{}, Tag<true>{}, Tag<true>{});
-            if (t + 1 <= t1) step(t + 1, Par<2>{}, Tag<true>{}, Tag<true>{});
-            if (t + 2 <= t1) step(t + 2, Par<3>{}, Tag<true>{}, Tag<true>{});
+            for (; t + D - 1 <= t1; t += D)
+                Unroll<D>::run([&](auto i) {
+                    constexpr int I = decltype(i)::value;
+                    step(t + I, Par<(I + 1) % D>{}, Tag<true>{}, Tag<true>{});
+                });
+            Unroll<D - 1>::run([&](auto i) {
+                constexpr int I = decltype(i)::value;
+                if (t + I <= t1) step(t + I, Par<(I + 1) % D>{}, Tag<true>{}, Tag<true>{});
+            });
         }
     }
 }

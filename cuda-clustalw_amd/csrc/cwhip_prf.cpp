@@ -840,36 +840,49 @@ int run_built_batch(cwhip_ctx* ctx, const cwhip_msa_params& P, const Seqs& seqs,
 // rewritten on the host pool.
 void apply_step(Step& st, Seqs& seqs, bool par = false)
 {
-    std::vector<uint8_t> path1, path2;
-    path1.reserve(st.L1 + st.L2 + 2);
-    path2.reserve(st.L1 + st.L2 + 2);
+    // the path of each group as runs: > 0 residues (path 2), < 0 gaps (path 1)
+    std::vector<int> runs1, runs2;
+    auto push = [](std::vector<int>& r, int v) {
+        if (!r.empty() && (r.back() > 0) == (v > 0)) r.back() += v;
+        else r.push_back(v);
+    };
     for (int i = 1; i <= st.job.n_displ; ++i) {
         const int k = st.displ[i];
-        if (k == 0) {
-            path1.push_back(2);
-            path2.push_back(2);
-        } else if (k > 0) {
-            for (int j = 0; j < k; ++j) { path1.push_back(1); path2.push_back(2); }
-        } else {
-            for (int j = 0; j < -k; ++j) { path1.push_back(2); path2.push_back(1); }
+        if (k == 0) {                                 // one column of both
+            push(runs1, 1);
+            push(runs2, 1);
+        } else {                                      // k > 0: k of group 2 against gaps
+            push(runs1, -k);                          // k < 0: -k of group 1 against gaps
+            push(runs2, k);
         }
     }
-    const int alen = (int)path1.size();
+    int alen = 0;
+    for (int v : runs1) alen += v > 0 ? v : -v;
+    // a row's residue ix is src[ix] below its length, GAP1 up to the profile
+    // length, ENDALN past it
     auto row = [&](size_t kk) {
         const int k = (int)kk;
         const bool g1 = k < st.n1;
-        const std::vector<uint8_t>& path = g1 ? path1 : path2;
+        const std::vector<int>& runs = g1 ? runs1 : runs2;
         const int rowlen = g1 ? st.L1 : st.L2;
         const std::vector<uint8_t>& src = seqs[st.members[k]];
+        const int lim1 = std::min(rowlen, (int)src.size());
         std::vector<uint8_t> out(alen);
+        uint8_t* o = out.data();
         int ix = 0;
-        for (int i = 0; i < alen; ++i) {
-            if (path[i] == 2) {
-                out[i] = ix < rowlen ? (uint8_t)row_at(src, ix) : (uint8_t)ENDALN;
-                ++ix;
-            } else {
-                out[i] = GAP1;
+        for (int v : runs) {
+            if (v < 0) {
+                std::memset(o, GAP1, -v);
+                o += -v;
+                continue;
             }
+            const int n1 = std::max(0, std::min(lim1 - ix, v));
+            if (n1 > 0) std::memcpy(o, src.data() + ix, n1);
+            const int n2 = std::max(0, std::min(rowlen - (ix + n1), v - n1));
+            std::memset(o + n1, GAP1, n2);
+            std::memset(o + n1 + n2, ENDALN, v - n1 - n2);
+            o += v;
+            ix += v;
         }
         seqs[st.members[k]].swap(out);
     };
@@ -1044,41 +1057,40 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
     std::vector<int> weights(seq_weight, seq_weight + nseq);
     const std::vector<int> rescnt = residue_counts(seqs);
     std::vector<int> aligned(nseq + 1, 0), maxid(nseq + 1, 0);
-    for (int i = 1; i <= nseq; ++i) {                     // MSA.cpp:67-78
-        maxid[i] = -1;
+    parallel_for(nseq, [&](size_t k) {                    // MSA.cpp:67-78
+        const int i = (int)k + 1;
+        int m = -1;
         for (int j = 1; j <= nseq; ++j)
-            if (j != i && maxid[i] < SIM(i, j)) maxid[i] = (int)SIM(i, j);
-    }
+            if (j != i && m < SIM(i, j)) m = (int)SIM(i, j);
+        maxid[i] = m;
+    });
+    std::vector<char> pass(nseq + 1, 0);                  // maxid above the cut-off
+    for (int i = 1; i <= nseq; ++i) pass[i] = maxid[i] > P.divergence_cutoff;
     int ix = 0, calls = 0, batches = 0;
     // First pass (:80-157).  A step only touches its members whose maxid passes
     // the cut-off, and a member's rows change only in steps that contain it,
     // so a step can run as soon as the last earlier step sharing a member is
     // done: steps are grouped into dependency levels, each one device batch.
     std::vector<int> level(nsteps, -1), last(nseq + 1, -1);
-    std::vector<std::vector<int>> groups(nsteps);
+    std::vector<std::vector<int>> members(nsteps);       // of each step, those past the cut-off
     int nlev = 0;
     for (int s = 0; s < nsteps; ++s) {
         const int32_t* row = steps + (size_t)s * (nseq + 1);
-        int entries = 0, lv = 0;
+        std::vector<int>& mem = members[s];
+        int lv = 0;
         for (int i = 1; i <= nseq; ++i) {
-            if (row[i] != 0 && maxid[i] > P.divergence_cutoff) {
-                ++entries;
-                if (aligned[i] == 0) {
-                    out_order[ix++] = i;                  // outputOrder ALIGNED
-                    aligned[i] = 1;
-                }
-                if (last[i] >= 0) lv = std::max(lv, level[last[i]] + 1);
+            if (row[i] == 0 || !pass[i]) continue;
+            mem.push_back(i);
+            if (aligned[i] == 0) {
+                out_order[ix++] = i;                      // outputOrder ALIGNED
+                aligned[i] = 1;
             }
+            if (last[i] >= 0) lv = std::max(lv, level[last[i]] + 1);
         }
-        if (entries == 0) continue;
+        if (mem.empty()) continue;
         level[s] = lv;
-        for (int i = 1; i <= nseq; ++i)
-            if (row[i] != 0 && maxid[i] > P.divergence_cutoff) last[i] = s;
+        for (int i : mem) last[i] = s;
         nlev = std::max(nlev, lv + 1);
-        // the member's aligned flag at this step is its own cut-off test
-        groups[s].assign(row, row + nseq + 1);
-        for (int i = 1; i <= nseq; ++i)
-            if (!(maxid[i] > P.divergence_cutoff)) groups[s][i] = 0;
     }
     std::vector<std::vector<int>> by_level(nlev);
     for (int s = 0; s < nsteps; ++s)
@@ -1092,7 +1104,13 @@ int cwhip_msa(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, const uint8_t
         // (and later apply their edit scripts) on several host threads; a level
         // of few steps builds them in turn, each on all threads
         auto prep = [&](size_t k) {
-            std::vector<int> grp = groups[by_level[lv][k]];
+            // the step's row of the group matrix, the members that fail the
+            // cut-off left out (their aligned flag at this step is that test)
+            const int s = by_level[lv][k];
+            const int32_t* row = steps + (size_t)s * (nseq + 1);
+            std::vector<int> grp(nseq + 1, 0);
+            grp[0] = row[0];
+            for (int i : members[s]) grp[i] = row[i];
             ok[k] = (signed char)prepare_step(P, seqs, rescnt, sim, weights, grp, all_aligned, sts[k]);
         };
         if (nst >= 4) parallel_for(nst, prep);
