@@ -1146,10 +1146,11 @@ __device__ __forceinline__ bool is_tiny(int M, int N)
     return M <= 1 || N <= 0 || (M <= kTinyRows && N <= kTinyCols);
 }
 
-// One node by one lane.  ctr[0]: records used, ctr[1]: queue tail.
+// One node by one lane.  ctr[0]: records used, ctr[1]: queue tail.  A split
+// node's children are records of the subtree (node-table slots come at the
+// subtree's write-out).
 __device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int* lq, int* ctr,
-                                          int rec_cap, int* s_alloc, int cap, int* s_err,
-                                          int* out_score)
+                                          int rec_cap, int* s_err, int* out_score)
 {
     int* rd = recs + kLocalRec * ri;
     const int A = rd[0], B = rd[1], M = rd[2], N = rd[3], fl = rd[4];
@@ -1283,18 +1284,17 @@ __device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int
         rd[6] = midj;
         rd[7] = -1;
         if (root) *out_score = midh;
-        const int slot = atomicAdd(s_alloc, 2);
         const int nr = atomicAdd(&ctr[0], 2);
         const int qp = atomicAdd(&ctr[1], 2);
-        if (slot + 2 > cap || nr + 2 > rec_cap || qp + 2 > rec_cap) {
+        if (nr + 2 > rec_cap || qp + 2 > rec_cap) {
             *s_err = 1;
         } else {
             int* c0 = recs + kLocalRec * nr;
             int* c1 = c0 + kLocalRec;
             make_children(c0, c1, A, B, M, N, fl, type, midj);
-            c0[8] = slot;
-            c1[8] = slot + 1;
-            rd[7] = slot;
+            c0[8] = -1;                              // not the problem's root
+            c1[8] = -1;
+            rd[7] = nr;                              // record index until the write-out
             lq[qp] = nr;
             lq[qp + 1] = nr + 1;
         }
@@ -1371,9 +1371,7 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
         wave_fence();
         const int type = uni(rd[5]);
         if (type != 0) {
-            const int got = atomicAdd(s_alloc, lane == 0 ? 2 : 0);
-            const int slot = uni(__shfl(got, 0, WAVE));
-            if (slot + 2 > cap || nrec + 2 > rec_cap || sp + 2 > kLocalStack) {
+            if (nrec + 2 > rec_cap || sp + 2 > kLocalStack) {
                 *s_err = 1;                          // same value from every lane
                 break;
             }
@@ -1381,9 +1379,9 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
             int* c1 = c0 + kLocalRec;
             make_children(c0, c1, uni(rd[0]), uni(rd[1]), uni(rd[2]), uni(rd[3]), uni(rd[4]), type,
                           uni(rd[6]));
-            c0[8] = slot;
-            c1[8] = slot + 1;
-            rd[7] = slot;
+            c0[8] = -1;                              // not the problem's root
+            c1[8] = -1;
+            rd[7] = nrec;                            // record index until the write-out
             stk[sp] = nrec + 1;
             stk[sp + 1] = nrec;
             sp += 2;
@@ -1401,17 +1399,27 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
         if (head >= tail) break;
         const int take = tail - head < WAVE ? tail - head : WAVE;
         if (lane < take)
-            tiny_node(q, recs, lq[head + lane], lq, ctr, rec_cap, s_alloc, cap, s_err, out_score);
+            tiny_node(q, recs, lq[head + lane], lq, ctr, rec_cap, s_err, out_score);
         head += take;
         wave_fence();
     }
     nrec = uni(ctr[0]);
-    // records out to the node table; the root keeps its slot, flagged as done
+    // records out to the node table: the root keeps its slot (flagged as
+    // done), records 1 .. nrec-1 take slots slot0 .. slot0+nrec-2, reserved by
+    // one atomic, and child links become slots
+    const int got = atomicAdd(s_alloc, lane == 0 ? nrec - 1 : 0);
+    const int slot0 = uni(__shfl(got, 0, WAVE));
+    if (slot0 + nrec - 1 > cap) {
+        *s_err = 1;                                  // same value from every lane
+        return;
+    }
     for (int k = lane; k < nrec * CWP_NODE_INTS; k += WAVE) {
         const int r = k >> 3, f = k & 7;
         const int* rc = recs + kLocalRec * r;
-        const int v = (f == 4 && r == 0) ? (rc[4] | kLocalFlag) : rc[f];
-        nodes[(int64_t)rc[8] * CWP_NODE_INTS + f] = v;
+        int v = rc[f];
+        if (f == 4 && r == 0) v |= kLocalFlag;
+        if (f == 7 && v >= 0) v = slot0 + v - 1;
+        nodes[(int64_t)(r == 0 ? root : slot0 + r - 1) * CWP_NODE_INTS + f] = v;
     }
     wave_fence();
 }
