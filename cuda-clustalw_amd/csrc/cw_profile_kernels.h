@@ -40,7 +40,8 @@ struct PrfArgs {
     const int32_t* tile_base; // S kernel: first 64x64 tile of each problem
     int32_t nprob;
     int32_t rmax;
-    int32_t* local_list;      // {problem, node} of every subtree cw_prf_mm leaves to cw_prf_local
+    int32_t* local_list;      // {problem, node, first reserved slot} of every subtree cw_prf_mm
+                              // leaves to cw_prf_local
     int32_t* local_ctr;       // [0] entries in local_list, [1] next entry to take (both 0 at launch)
     int32_t* state;           // per problem: lev_b, lev_e, nodes allocated, levels done
     int32_t resume;           // cw_prf_mm: continue at the level in state (else start at the root)

@@ -101,6 +101,10 @@ __device__ __forceinline__ int neg_after(int x, int after) {
 // LDS address space: reads through these compile to ds_read_* rather than flat loads
 #define CWP_LDS __attribute__((address_space(3)))
 typedef CWP_LDS int lint;
+__device__ __forceinline__ int lds_add(lint* q, int v)
+{
+    return __hip_atomic_fetch_add(q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Global address space: these arrays are read with global (not flat) loads,
 // which LDS waits (lgkmcnt) do not wait for.
@@ -827,7 +831,8 @@ __device__ __attribute__((noinline)) void do_mid(const Prob& p, int* nd, int is_
 // The two children of a split node (:796-809): type 1 splits at (midi, midj),
 // type 2 through the gap of rows midi..midi+1 (the del(2) between them is the
 // host's).  Records: A B M N flags type midj child.
-__device__ __forceinline__ void make_children(int* c0, int* c1, int A, int B, int M, int N, int fl,
+template <typename Ptr>
+__device__ __forceinline__ void make_children(Ptr c0, Ptr c1, int A, int B, int M, int N, int fl,
                                               int type, int midj)
 {
     const int go1 = fl & 1, go2 = (fl >> 1) & 1;
@@ -1054,7 +1059,7 @@ __device__ void lhalves(const LProb& q, int A, int B, int M, int N, int go1, int
 // One node of a local subtree (progDiff on LDS data): leaves as in do_leaf,
 // internal nodes by lhalves and the midpoint of :756-776.  Writes type and
 // midj to the record; *midh_out gets the score (used for the root only).
-__device__ __attribute__((noinline)) void local_node(const LProb& q, int* rd, int is_root, lint* rows4,
+__device__ __attribute__((noinline)) void local_node(const LProb& q, lint* rd, int is_root, lint* rows4,
                                                      int rstride, int* out_score)
 {
     const int lane = lane_id();
@@ -1149,10 +1154,10 @@ __device__ __forceinline__ bool is_tiny(int M, int N)
 // One node by one lane.  ctr[0]: records used, ctr[1]: queue tail.  A split
 // node's children are records of the subtree (node-table slots come at the
 // subtree's write-out).
-__device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int* lq, int* ctr,
+__device__ __forceinline__ void tiny_node(const LProb& q, lint* recs, int ri, lint* lq, lint* ctr,
                                           int rec_cap, int* s_err, int* out_score)
 {
-    int* rd = recs + kLocalRec * ri;
+    lint* rd = recs + kLocalRec * ri;
     const int A = rd[0], B = rd[1], M = rd[2], N = rd[3], fl = rd[4];
     const int go1 = fl & 1, go2 = (fl >> 1) & 1;
     const bool root = rd[8] == 0;
@@ -1284,13 +1289,13 @@ __device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int
         rd[6] = midj;
         rd[7] = -1;
         if (root) *out_score = midh;
-        const int nr = atomicAdd(&ctr[0], 2);
-        const int qp = atomicAdd(&ctr[1], 2);
+        const int nr = lds_add(&ctr[0], 2);
+        const int qp = lds_add(&ctr[1], 2);
         if (nr + 2 > rec_cap || qp + 2 > rec_cap) {
             *s_err = 1;
         } else {
-            int* c0 = recs + kLocalRec * nr;
-            int* c1 = c0 + kLocalRec;
+            lint* c0 = recs + kLocalRec * nr;
+            lint* c1 = c0 + kLocalRec;
             make_children(c0, c1, A, B, M, N, fl, type, midj);
             c0[8] = -1;                              // not the problem's root
             c1[8] = -1;
@@ -1302,10 +1307,10 @@ __device__ __forceinline__ void tiny_node(const LProb& q, int* recs, int ri, int
 }
 
 __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* nodes, int root, int* base,
-                                                        int* s_alloc, int cap, int* s_err, int* out_score)
+                                                        int slot0, int* s_err, int* out_score)
 {
     const int lane = lane_id();
-    const int* rnd = nodes + (int64_t)root * CWP_NODE_INTS;
+    gcint* rnd = (gcint*)nodes + (int64_t)root * CWP_NODE_INTS;
     const int A0 = uni(rnd[0]), B0 = uni(rnd[1]), M0 = uni(rnd[2]), N0 = uni(rnd[3]);
     const int fl0 = uni(rnd[4]);
     lint* lb = (lint*)base;
@@ -1315,11 +1320,11 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
     lint* lpl2 = lpl1 + (M0 + 2);
     lint* lS = lpl2 + (N0 + 2);                      // lS[(J-B0-1)*M0 + I-A0-1]
     lint* lout = lS + M0 * N0;                       // final rows of a node: 4 x (N0 + 2)
-    int* recs = base + (4 * (M0 + 2) + 4 * (N0 + 2) + (M0 + 2) + (N0 + 2) + M0 * N0 + 4 * (N0 + 2));
+    lint* recs = lb + (4 * (M0 + 2) + 4 * (N0 + 2) + (M0 + 2) + (N0 + 2) + M0 * N0 + 4 * (N0 + 2));
     const int rec_cap = 2 * M0 + 4;
-    int* stk = recs + kLocalRec * rec_cap;
-    int* lq = stk + kLocalStack;                     // tiny-node queue (record indices)
-    int* ctr = lq + rec_cap;                         // records used, queue tail
+    lint* stk = recs + kLocalRec * rec_cap;
+    lint* lq = stk + kLocalStack;                    // tiny-node queue (record indices)
+    lint* ctr = lq + rec_cap;                        // records used, queue tail
 
     // copy in, 16 loads in flight per lane before the LDS stores
     for (int x = lane; x < M0 + 2; x += WAVE) {
@@ -1360,7 +1365,7 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
     while (sp > 0) {
         sp -= 1;
         const int ri = uni(stk[sp]);
-        int* rd = recs + kLocalRec * ri;
+        lint* rd = recs + kLocalRec * ri;
         if (is_tiny(uni(rd[2]), uni(rd[3]))) {
             lq[nq] = ri;                             // same value from every lane
             nq += 1;
@@ -1375,8 +1380,8 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
                 *s_err = 1;                          // same value from every lane
                 break;
             }
-            int* c0 = recs + kLocalRec * nrec;
-            int* c1 = c0 + kLocalRec;
+            lint* c0 = recs + kLocalRec * nrec;
+            lint* c1 = c0 + kLocalRec;
             make_children(c0, c1, uni(rd[0]), uni(rd[1]), uni(rd[2]), uni(rd[3]), uni(rd[4]), type,
                           uni(rd[6]));
             c0[8] = -1;                              // not the problem's root
@@ -1405,21 +1410,19 @@ __device__ __attribute__((noinline)) void local_subtree(const Prob& p, int* node
     }
     nrec = uni(ctr[0]);
     // records out to the node table: the root keeps its slot (flagged as
-    // done), records 1 .. nrec-1 take slots slot0 .. slot0+nrec-2, reserved by
-    // one atomic, and child links become slots
-    const int got = atomicAdd(s_alloc, lane == 0 ? nrec - 1 : 0);
-    const int slot0 = uni(__shfl(got, 0, WAVE));
-    if (slot0 + nrec - 1 > cap) {
+    // done), records 1 .. nrec-1 take slots slot0 .. slot0+nrec-2 of the
+    // 2 M0 - 2 that cw_prf_mm reserved, and child links become slots
+    if (nrec - 1 > 2 * M0 - 2) {
         *s_err = 1;                                  // same value from every lane
         return;
     }
     for (int k = lane; k < nrec * CWP_NODE_INTS; k += WAVE) {
         const int r = k >> 3, f = k & 7;
-        const int* rc = recs + kLocalRec * r;
+        const lint* rc = recs + kLocalRec * r;
         int v = rc[f];
         if (f == 4 && r == 0) v |= kLocalFlag;
         if (f == 7 && v >= 0) v = slot0 + v - 1;
-        nodes[(int64_t)(r == 0 ? root : slot0 + r - 1) * CWP_NODE_INTS + f] = v;
+        ((gint*)nodes)[(int64_t)(r == 0 ? root : slot0 + r - 1) * CWP_NODE_INTS + f] = v;
     }
     wave_fence();
 }
@@ -1490,12 +1493,26 @@ extern "C" __global__ void __launch_bounds__(256) cw_prf_score(PrfArgs a)
 // Myers-Miller recursion of one profile pair per workgroup, level by level.
 // A subtree small enough for one wave's LDS is left to cw_prf_local: its root
 // goes on the batch's list (lane 0 of the wave appends it).
-__device__ __attribute__((noinline)) static void defer_local(const PrfArgs& a, int prob, int idx)
+// A subtree of a root with M >= 2 rows has at most 2M - 1 nodes (a split
+// node's children have at most its rows between them; leaves have M <= 1), so
+// its records other than the root fit 2M - 2 node-table slots, reserved here
+// from the workgroup's count.  Reserving that much for every local subtree
+// keeps the table at most 2 L1 - 1 nodes, as for the tree itself.
+__device__ __attribute__((noinline)) static void defer_local(const PrfArgs& a, int prob, int idx, int M,
+                                                             int* s_alloc, int cap, int* s_err)
 {
-    if (lane_id() == 0) {
+    const int lane = lane_id();
+    const int got = atomicAdd(s_alloc, lane == 0 ? 2 * M - 2 : 0);
+    const int slot0 = uni(__shfl(got, 0, WAVE));
+    if (slot0 + 2 * M - 2 > cap) {
+        *s_err = 1;                                  // same value from every lane
+        return;
+    }
+    if (lane == 0) {
         const int i = atomicAdd(&a.local_ctr[0], 1);
-        a.local_list[2 * i] = prob;
-        a.local_list[2 * i + 1] = idx;
+        a.local_list[3 * i] = prob;
+        a.local_list[3 * i + 1] = idx;
+        a.local_list[3 * i + 2] = slot0;
     }
 }
 
@@ -1575,7 +1592,7 @@ extern "C" __global__ void __launch_bounds__(CWP_THREADS) cw_prf_mm(PrfArgs a)
             int* nd = nodes + (int64_t)idx * CWP_NODE_INTS;
             const int M = uni(nd[2]), N = uni(nd[3]);
             if (local_fits(M, N)) {
-                if (within == 0) defer_local(a, prob, idx);
+                if (within == 0) defer_local(a, prob, idx, M, &s_alloc, cap, &s_err);
             } else if (N <= 0 || M <= 1) {
                 if (within == 0) do_leaf(p, nd, idx == 0, a.out + 4 * prob);
             } else if (!swept) {
@@ -1716,11 +1733,12 @@ extern "C" __global__ void __launch_bounds__(64 * CWP_LOCAL_WAVES) cw_prf_local(
         const int got = atomicAdd(&a.local_ctr[1], lane == 0 ? 1 : 0);
         const int i = uni(__shfl(got, 0, WAVE));
         if (i >= total) break;
-        const int prob = uni(a.local_list[2 * i]), idx = uni(a.local_list[2 * i + 1]);
+        const int prob = uni(a.local_list[3 * i]), idx = uni(a.local_list[3 * i + 1]);
+        const int slot0 = uni(a.local_list[3 * i + 2]);
         const PrfProb pp = a.probs[prob];
         const Prob p = make_prob(a, pp);
-        local_subtree(p, a.pool + pp.nodes, idx, s_local + wave * CWP_LOCAL_INTS, a.out + 4 * prob + 1,
-                      pp.node_cap, a.out + 4 * prob + 2, a.out + 4 * prob);
+        local_subtree(p, a.pool + pp.nodes, idx, s_local + wave * CWP_LOCAL_INTS, slot0,
+                      a.out + 4 * prob + 2, a.out + 4 * prob);
     }
 }
 

@@ -291,7 +291,7 @@ void layout_batch(const std::vector<DpJob>& jobs, BatchLayout& lo, const cwhip_c
     for (const auto& pp : lo.probs) roots += pp.node_cap;
     lo.in_sz = in_sz;
     lo.list = in_sz + work_sz;
-    lo.ctr = up4(lo.list + 2 * roots);
+    lo.ctr = up4(lo.list + 3 * roots);
     lo.state = lo.ctr + 4;
     lo.total = lo.state + up4(4 * (int64_t)n);
 }
