@@ -1795,40 +1795,46 @@ extern "C" __global__ void __launch_bounds__(CWP_BUILD_COLS * CWP_BUILD_SLICES) 
     const bool hydp = (sd.flags & CWP_F_HYD) != 0;
     int gapc = 0, hyd = 0;
     const int k0 = (int)((int64_t)n * sl / NS), k1 = (int)((int64_t)n * (sl + 1) / NS);
+    // A hydrophilic residue counts when it lies inside a stretch of more than
+    // 3 hydrophilic residues (the reference looks up to 3 to the left, then
+    // right for the rest).  A slice is one wave whose 64 lanes hold 64
+    // consecutive columns of the row, so the stretch test reads the row's
+    // hydrophilic flags from two ballots: the lanes' own columns and the 3
+    // columns past each end of the block (lanes 0-2 and 61-63 load those).
+    // Columns outside 0 .. L-1 count as not hydrophilic.
+    const int col0 = pos - t;                // the block's first column
+    const int ecol = t < 3 ? col0 - 3 + t : (t >= 61 ? col0 + 3 + t : -1);
+    const bool eload = ecol >= 0 && ecol < sd.L;
+    auto hyd_run = [&](const uint8_t* r, int c) -> bool {
+        const bool h = phyd(a.hyd_mask, c);
+        const uint64_t mid = __ballot(h);
+        const uint64_t edge = __ballot(eload && phyd(a.hyd_mask, r[eload ? ecol : 0]));
+        // bits 0-2: columns col0-3 .. col0-1; 3-66: the lanes; 67-69: col0+64 .. col0+66
+        const uint64_t lo = (edge & 7ull) | (mid << 3);
+        const uint64_t hi = (mid >> 61) | ((edge >> 61) << 3);
+        // this lane's window: columns pos-3 .. pos+3 at bits 0 .. 6
+        const uint64_t w = (t == 0 ? lo : ((lo >> t) | (hi << (64 - t)))) & 0x7Full;
+        return h && ((w & 0x0F) == 0x0F || (w & 0x1E) == 0x1E || (w & 0x3C) == 0x3C ||
+                     (w & 0x78) == 0x78);
+    };
     auto one = [&](const uint8_t* r, int c, int wk, int f, int e) {
         if (c < 32) {
             ws[c * C + t] += wk;
             cn[c * C + t] += 1;
         }
-        if (pgap(c)) {
-            if (pos >= f && pos < e) ++gapc;
-        } else if (hydp && phyd(a.hyd_mask, c)) {
-            // inside a stretch of more than 3 hydrophilic residues
-            int left = 0, right = 0;
-            for (int d = 1; d <= 3 && pos - d >= 0; ++d) {
-                if (!phyd(a.hyd_mask, r[pos - d])) break;
-                ++left;
-            }
-            for (int d = 1; d <= 3 - left && pos + d < sd.L; ++d) {
-                if (!phyd(a.hyd_mask, r[pos + d])) break;
-                ++right;
-            }
-            if (left + right >= 3) hyd += 100;
-        }
+        if (pgap(c) && pos >= f && pos < e) ++gapc;
+        if (hydp && hyd_run(r, c)) hyd += 100;
     };
     if (live) {
         int k = k0;
-        for (; k + 4 <= k1; k += 4) {        // four rows' loads in flight
-            const uint8_t* r0 = base + (int64_t)k * sd.stride;
-            const uint8_t* r1 = r0 + sd.stride;
-            const uint8_t* r2 = r1 + sd.stride;
-            const uint8_t* r3 = r2 + sd.stride;
-            const int c0 = r0[pos], c1 = r1[pos], c2 = r2[pos], c3 = r3[pos];
+        for (; k + 8 <= k1; k += 8) {        // eight rows' loads in flight
+            int cv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cv[u] = base[(int64_t)(k + u) * sd.stride + pos];
             const int* ri = rinfo + 3 * k;
-            one(r0, c0, ri[0], ri[1], ri[2]);
-            one(r1, c1, ri[3], ri[4], ri[5]);
-            one(r2, c2, ri[6], ri[7], ri[8]);
-            one(r3, c3, ri[9], ri[10], ri[11]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                one(base + (int64_t)(k + u) * sd.stride, cv[u], ri[3 * u], ri[3 * u + 1], ri[3 * u + 2]);
         }
         for (; k < k1; ++k) {
             const uint8_t* r = base + (int64_t)k * sd.stride;
