@@ -856,35 +856,42 @@ void apply_step(Step& st, Seqs& seqs, bool par = false)
             push(runs2, k);
         }
     }
-    int alen = 0;
-    for (int v : runs1) alen += v > 0 ? v : -v;
+    int alen = 0, res1 = 0, res2 = 0;                 // columns; residue places of each group
+    for (int v : runs1) {
+        alen += v > 0 ? v : -v;
+        res1 += v > 0 ? v : 0;
+    }
+    for (int v : runs2) res2 += v > 0 ? v : 0;
     // a row's residue ix is src[ix] below its length, GAP1 up to the profile
-    // length, ENDALN past it
+    // length, ENDALN past it.  The row grows in place from its end: an output
+    // position is never below the source position it takes, so a run's source
+    // bytes are still there when it is written.
     auto row = [&](size_t kk) {
         const int k = (int)kk;
         const bool g1 = k < st.n1;
         const std::vector<int>& runs = g1 ? runs1 : runs2;
         const int rowlen = g1 ? st.L1 : st.L2;
-        const std::vector<uint8_t>& src = seqs[st.members[k]];
-        const int lim1 = std::min(rowlen, (int)src.size());
-        std::vector<uint8_t> out(alen);
-        uint8_t* o = out.data();
-        int ix = 0;
-        for (int v : runs) {
-            if (v < 0) {
-                std::memset(o, GAP1, -v);
-                o += -v;
+        std::vector<uint8_t>& v = seqs[st.members[k]];
+        const int lim1 = std::min(rowlen, (int)v.size());
+        if ((int)v.capacity() < alen) v.reserve((size_t)alen + alen / 4);   // room for later steps
+        v.resize(alen);
+        uint8_t* o = v.data();
+        int op = alen, ix = g1 ? res1 : res2;
+        for (size_t q = runs.size(); q-- > 0;) {
+            const int r = runs[q];
+            if (r < 0) {
+                op += r;
+                std::memset(o + op, GAP1, -r);
                 continue;
             }
-            const int n1 = std::max(0, std::min(lim1 - ix, v));
-            if (n1 > 0) std::memcpy(o, src.data() + ix, n1);
-            const int n2 = std::max(0, std::min(rowlen - (ix + n1), v - n1));
-            std::memset(o + n1, GAP1, n2);
-            std::memset(o + n1 + n2, ENDALN, v - n1 - n2);
-            o += v;
-            ix += v;
+            op -= r;
+            ix -= r;
+            const int n1 = std::max(0, std::min(lim1 - ix, r));
+            const int n2 = std::max(0, std::min(rowlen - (ix + n1), r - n1));
+            std::memset(o + op + n1 + n2, ENDALN, r - n1 - n2);
+            std::memset(o + op + n1, GAP1, n2);
+            if (n1 > 0) std::memmove(o + op, o + ix, n1);
         }
-        seqs[st.members[k]].swap(out);
     };
     const size_t nrow = (size_t)(st.n1 + st.n2);
     if (par && nrow * (size_t)alen > (1u << 16)) parallel_for(nrow, row);
