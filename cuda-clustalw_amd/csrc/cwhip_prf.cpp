@@ -1014,6 +1014,18 @@ int cwhip_step_profiles(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     return CWHIP_OK;
 }
 
+int cwhip_set_msa_tuning(cwhip_ctx* ctx, int wide_rows, int wide_levels, int wide_pairs)
+{
+    if (!ctx) return CWHIP_E_ARG;
+    if (wide_rows < -1 || wide_levels < -1 || wide_levels == 0 || wide_levels > kWideMaxLevels ||
+        wide_pairs < -1)
+        return fail(ctx, CWHIP_E_ARG, "msa tuning value out of range");
+    if (wide_rows >= 0) ctx->prf_wide_rows = wide_rows;
+    if (wide_levels >= 0) ctx->prf_wide_levels = wide_levels;
+    if (wide_pairs >= 0) ctx->prf_wide_pairs = wide_pairs;
+    return CWHIP_OK;
+}
+
 int cwhip_msa_params_default(int dna, cwhip_msa_params* p)
 {
     if (!p) return CWHIP_E_ARG;

@@ -20,7 +20,8 @@ EXPORTS = [
     "cwhip_create", "cwhip_destroy", "cwhip_last_error", "cwhip_pair_params_default",
     "cwhip_load_sequences", "cwhip_range_npairs", "cwhip_pairwise_range",
     "cwhip_pairwise_pairs", "cwhip_distance", "cwhip_stage_pairs", "cwhip_run_staged",
-    "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning", "cwhip_get_tune_table",
+    "cwhip_fetch_counts", "cwhip_fetch_info", "cwhip_set_tuning", "cwhip_set_msa_tuning",
+    "cwhip_get_tune_table",
     "cwhip_set_tune_table", "cwhip_autotune", "cwhip_set_graph",
     "cwhip_profile_dp", "cwhip_msa_params_default", "cwhip_msa", "cwhip_msa_rows",
     "cwhip_msa_stats", "cwhip_profile_align", "cwhip_step_profiles", "cwhip_fast_params_default",
@@ -145,6 +146,7 @@ def load() -> C.CDLL:
     L.cwhip_fetch_counts.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_fetch_info.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.cwhip_set_msa_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.cwhip_get_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
     L.cwhip_set_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
     L.cwhip_autotune.argtypes = [vp, C.POINTER(PairParams), P(np.int32, flags="C"), C.c_int64,
@@ -260,6 +262,11 @@ class Context:
     def set_tuning(self, rmax=0, waves_per_group=0, groups_per_cu=0):
         self._chk(self._L.cwhip_set_tuning(self._h, rmax, waves_per_group, groups_per_cu),
                   "cwhip_set_tuning")
+
+    def set_msa_tuning(self, wide_rows=-1, wide_levels=-1, wide_pairs=-1):
+        """Schedule of the profile alignments' Myers-Miller levels (-1 keeps a value)."""
+        self._chk(self._L.cwhip_set_msa_tuning(self._h, wide_rows, wide_levels, wide_pairs),
+                  "cwhip_set_msa_tuning")
 
     def tune_table(self):
         t = TuneTable()

@@ -176,6 +176,16 @@ int cwhip_fetch_info(cwhip_ctx *ctx, int32_t *out_info, int64_t npairs);
  * waves per workgroup, persistent workgroups per CU. */
 int cwhip_set_tuning(cwhip_ctx *ctx, int rmax, int waves_per_group, int groups_per_cu);
 
+/* Schedule of the profile alignments' Myers-Miller levels (no counterpart in
+ * the reference, which runs progDiff on one thread; for callers and tests that
+ * compare schedules, all of which give the same alignment): about how many
+ * rows each pipelined stripe of a large root's half sweeps takes when those
+ * run over several CUs (0: the root stays in its pair's workgroup), how many
+ * levels sweep their halves one wave per workgroup before the rest run in the
+ * pair's workgroup (1..6; 1: none), and the most profile pairs a batch may
+ * hold for either (0: none).  -1 keeps a value. */
+int cwhip_set_msa_tuning(cwhip_ctx *ctx, int wide_rows, int wide_levels, int wide_pairs);
+
 /* Shapes of the batched kernel by tuning class (the longer sequence of a pair:
  * <=128, <=256, <=512, <=1024, <=2048, longer): stripe height in rows per lane
  * (a stripe is 64*rmax rows) and the widest Myers-Miller node that runs on the

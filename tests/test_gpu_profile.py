@@ -229,3 +229,37 @@ def test_delayed_pass_is_one_profile_align_step(ctx):
     assert info[0] >= 0
     for i in range(n):
         np.testing.assert_array_equal(new[i], rows[i])
+
+
+def test_msa_schedules_agree():
+    """Every schedule of the profile alignments' Myers-Miller levels gives the
+    oracle's alignment: the root's half sweeps as 2..5 pipelined stripes on
+    separate workgroups or in the pair's workgroup, 1..6 levels swept over the
+    chip, and batches left entirely to one workgroup per pair (the schedule of
+    batches of more than 512 pairs).  Two families of ~600 residues make roots
+    of 256+ rows from the first level on."""
+    a = seqgen.family(901, 14, 640, branch_mean=0.3, name_prefix="sa")
+    b = seqgen.family(902, 10, 560, branch_mean=0.5, name_prefix="sb")
+    seqs, names = a.seqs + b.seqs, a.names + b.names
+    n = len(seqs)
+    codes = np.concatenate(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    pairs = O.range_pairs(n, 0, n, 0, n)
+    cnt, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring(False))
+    dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    r_rows, r_order, info = O.msa([s.astype(np.int32) for s in seqs], sim, w, steps, O.msa_params(False))
+    schedules = [(256, 5, 512), (64, 5, 512), (128, 6, 512), (0, 3, 512), (256, 1, 512), (256, 5, 0)]
+    c = cwhip.Context(0)
+    try:
+        for wide_rows, wide_levels, wide_pairs in schedules:
+            print("msa schedule", wide_rows, wide_levels, wide_pairs, flush=True)
+            c.set_msa_tuning(wide_rows, wide_levels, wide_pairs)
+            rows, order, stats = c.msa(seqs, sim, w, steps)
+            np.testing.assert_array_equal(order, r_order)
+            for i in range(n):
+                np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
+    finally:
+        c.close()
