@@ -803,6 +803,15 @@ int cwhip_clustal_text(int nseq, const char* const* names, const uint8_t* rows, 
                                        "SNDEQK", "NDEQHK", "NEQHRK", "FVLIM", "HFY"};
     static const char kCodes[] = "ABCDEFGHIKLMNOPQRSTUVWXYZ-";
     const int maxAA = 24, nStrong = 9, nWeak = 11;
+    // per code 0..25: bit k set when the code's letter is in group k
+    uint32_t strongBits[26], weakBits[26];
+    for (int c = 0; c < 26; ++c) {
+        strongBits[c] = weakBits[c] = 0;
+        for (int k = 0; k < nStrong; ++k)
+            if (std::strchr(strong[k], kCodes[c])) strongBits[c] |= 1u << k;
+        for (int k = 0; k < nWeak; ++k)
+            if (std::strchr(weak[k], kCodes[c])) weakBits[c] |= 1u << k;
+    }
     int maxNames = 0, length = 0;
     for (int i = 0; i < nseq; ++i) {
         if (!names[i] || lens[i] < 0 || lens[i] > stride) return CWHIP_E_ARG;
@@ -821,7 +830,15 @@ int cwhip_clustal_text(int nseq, const char* const* names, const uint8_t* rows, 
     auto at = [&](int seq, int col) -> int {                    // 1-based sequence and column
         return col <= lens[seq - 1] ? rows[(int64_t)(seq - 1) * stride + col - 1] : -3;
     };
+    // each name cut or padded to namesWidth, then 6 blanks
+    std::vector<std::string> label(nseq);
+    for (int k = 0; k < nseq; ++k) {
+        label[k] = names[k];
+        if ((int)label[k].size() > namesWidth) label[k].resize(namesWidth);
+        label[k].resize(namesWidth + 6, ' ');
+    }
     std::string s = "CLUSTAL 2.0.11 multiple sequence alignment\n\n";
+    s.reserve(s.size() + (size_t)chunks * (1 + (size_t)(nseq + 1) * (namesWidth + 7 + lineLength)));
     std::string line;
     for (int lv1 = 1; lv1 <= chunks; ++lv1) {
         const int pos = (lv1 - 1) * lineLength + 1;
@@ -829,45 +846,46 @@ int cwhip_clustal_text(int nseq, const char* const* names, const uint8_t* rows, 
         s += "\n";
         for (int ii = 1; ii <= nseq; ++ii) {
             const int i = order[ii - 1];
-            line.assign(ptr - pos + 1, '-');
-            for (int j = pos; j <= ptr; ++j) {
-                const int val = at(i, j);
-                if (val == -3 || val == 253) break;
-                line[j - pos] = (val < 0 || val > maxAA) ? '-' : kCodes[val];
+            s += label[i - 1];
+            const size_t at0 = s.size();
+            s.append(ptr - pos + 1, '-');
+            char* o = &s[at0];
+            const uint8_t* r = rows + (int64_t)(i - 1) * stride;
+            const int end = std::min(ptr, (int)lens[i - 1]);        // past a row's end: '-'
+            for (int j = pos; j <= end; ++j) {
+                const int val = r[j - 1];
+                if (val == 253) break;
+                o[j - pos] = val > maxAA ? '-' : kCodes[val];
             }
-            std::string nm = names[i - 1];
-            if ((int)nm.size() > namesWidth) nm.resize(namesWidth);
-            nm.resize(namesWidth + 6, ' ');
-            s += nm;
-            s += line;
-            s += "\n";
+            s += '\n';
         }
+        // A column is '*' when every sequence counts and equals the first, ':'
+        // when every sequence counts and has a residue of one strong group,
+        // '.' likewise for a weak group; a sequence counts when both it and
+        // the first reach the column and the first's code is a residue.  So a
+        // group's count reaches nseq exactly when the AND over the sequences
+        // of their codes' group bits keeps its bit.
         line.assign(ptr - pos + 1, ' ');
         for (int i = pos; i <= ptr; ++i) {
-            int ident = 0, cat1[9] = {0}, cat2[11] = {0};
-            for (int j = 1; j <= nseq; ++j) {
-                if (!(i <= lens[j - 1] && i <= lens[0])) continue;
-                const int r1 = at(1, i);
-                if (r1 < 0 || r1 > maxAA) continue;
-                const int rj = at(j, i);
-                if (r1 == rj) ++ident;
-                if (rj <= maxAA + 1) {
-                    const char c = kCodes[rj];
-                    for (int k = 0; k < nStrong; ++k)
-                        if (std::strchr(strong[k], c)) ++cat1[k];
-                    for (int k = 0; k < nWeak; ++k)
-                        if (std::strchr(weak[k], c)) ++cat2[k];
+            const int r1 = i <= lens[0] ? at(1, i) : -1;
+            if (r1 < 0 || r1 > maxAA) continue;
+            bool ident = true;
+            uint32_t sAnd = ~0u, wAnd = ~0u;
+            for (int j = 1; j <= nseq && (ident || sAnd || wAnd); ++j) {
+                if (i > lens[j - 1]) {
+                    ident = false;
+                    sAnd = wAnd = 0;
+                    break;
                 }
+                const int rj = at(j, i);
+                ident = ident && rj == r1;
+                sAnd &= rj <= maxAA + 1 ? strongBits[rj] : 0u;
+                wAnd &= rj <= maxAA + 1 ? weakBits[rj] : 0u;
             }
             char mark = ' ';
-            if (ident == nseq) {
-                mark = '*';
-            } else if (!dna) {
-                for (int k = 0; k < nStrong && mark == ' '; ++k)
-                    if (cat1[k] == nseq) mark = ':';
-                for (int k = 0; k < nWeak && mark == ' '; ++k)
-                    if (cat2[k] == nseq) mark = '.';
-            }
+            if (ident) mark = '*';
+            else if (!dna && sAnd) mark = ':';
+            else if (!dna && wAnd) mark = '.';
             line[i - pos] = mark;
         }
         s += std::string(namesWidth + 6, ' ');

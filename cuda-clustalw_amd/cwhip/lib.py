@@ -484,14 +484,20 @@ def clustal_text(names, rows, order, dna=False) -> str:
     order = np.ascontiguousarray(order, np.int32)
     arr = _names(names)
     ln = C.c_int64()
-    L.cwhip_clustal_text(n, C.cast(arr, C.c_void_p), buf, stride, lens, order, int(dna), None, 0,
-                         C.byref(ln))
-    out = C.create_string_buffer(ln.value + 1)
+    # room for the text in one call: lines of at most 30 + 6 name columns and 60
+    # residues, at least 50 residues per block
+    blocks = -(-stride // 50)
+    cap = 64 + blocks * (1 + (n + 1) * 98)
+    out = C.create_string_buffer(cap)
     rc = L.cwhip_clustal_text(n, C.cast(arr, C.c_void_p), buf, stride, lens, order, int(dna),
-                              C.cast(out, C.c_void_p), ln.value + 1, C.byref(ln))
+                              C.cast(out, C.c_void_p), cap, C.byref(ln))
+    if rc and ln.value + 1 > cap:
+        out = C.create_string_buffer(ln.value + 1)
+        rc = L.cwhip_clustal_text(n, C.cast(arr, C.c_void_p), buf, stride, lens, order, int(dna),
+                                  C.cast(out, C.c_void_p), ln.value + 1, C.byref(ln))
     if rc:
         raise CwhipError("cwhip_clustal_text failed: %d" % rc)
-    return out.value.decode()
+    return out.raw[:ln.value].decode()
 
 
 def profile_align(ctx, seqs, sim, weights, group, aligned, params):
