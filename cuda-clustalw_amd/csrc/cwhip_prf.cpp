@@ -236,16 +236,22 @@ constexpr int kWideMaxLevels = 6;
 static_assert(2 << (kWideMaxLevels - 1) <= CWP_WIDE_SETS, "scratch sets for cw_prf_level");
 int wide_levels(const cwhip_ctx* ctx) { return std::max(1, std::min(ctx->prf_wide_levels, kWideMaxLevels)); }
 
+// Batches of more pairs than this already give cw_prf_wide more stripes than
+// the chip has SIMDs: their roots take as few stripes as fit (less pipeline
+// lag for the same work; 0.55 ms less per cwhip_msa on 1000 x ~1002).
+constexpr int kWideFewPairs = 64;
+
 // cw_prf_wide's members per half for a root of L1 x L2 (0: leave it to
 // cw_prf_mm): about `rows` rows per stripe, 2..CWP_WIDE_MAXG stripes of
-// 64..512 rows in both halves.
-int wide_members(int L1, int L2, int rows)
+// 64..512 rows in both halves; with `tallest`, the fewest stripes of at most
+// 512 rows.
+int wide_members(int L1, int L2, int rows, bool tallest = false)
 {
     const int h = L1 / 2;                            // forward half; the reverse has h or h + 1
     if (rows <= 0 || h < kWideMinHalf || L2 < kWideMinCols) return 0;
-    int G = std::min(CWP_WIDE_MAXG, std::max(2, h / rows));
+    int G = tallest ? (L1 - h + 511) / 512 : std::min(CWP_WIDE_MAXG, std::max(2, h / rows));
     G = std::min(G, h / 64);
-    if (G < 2 || (L1 - h + G - 1) / G > 512) return 0;
+    if (G < (tallest ? 1 : 2) || (L1 - h + G - 1) / G > 512) return 0;
     return G;
 }
 
@@ -307,7 +313,7 @@ int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
     std::vector<int32_t> wide;
     for (int k = 0; k < n; ++k) {
         PrfProb& pp = lo.probs[k];
-        const int G = lo.wide ? wide_members(pp.L1, pp.L2, ctx->prf_wide_rows) : 0;
+        const int G = lo.wide ? wide_members(pp.L1, pp.L2, ctx->prf_wide_rows, n > kWideFewPairs) : 0;
         if (G == 0) continue;
         pp.flags |= CWP_ROOT_WIDE;
         for (int half = 0; half < 2; ++half)
