@@ -634,14 +634,15 @@ int prepare_step(const cwhip_msa_params& p, const Seqs& seqs, const std::vector<
     }
     double pcid = 0.0;                                    // :124-137
     int count = 0;
+    std::vector<int> in2;                                 // group 2, ascending: the same order of adds
+    in2.reserve(n2);
+    for (int j = 0; j < nseq; ++j)
+        if (group[j + 1] == 2) in2.push_back(j + 1);
     for (int i = 0; i < nseq; ++i) {
         if (group[i + 1] != 1) continue;
         const double* row = sim + (size_t)(i + 1) * (nseq + 1);
-        for (int j = 0; j < nseq; ++j)
-            if (group[j + 1] == 2) {
-                ++count;
-                pcid += row[j + 1];
-            }
+        for (int j : in2) pcid += row[j];
+        count += (int)in2.size();
     }
     pcid = pcid / (float)count;
     st.members.clear();
