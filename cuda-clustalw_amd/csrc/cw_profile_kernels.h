@@ -54,6 +54,7 @@ struct WideArgs {
     const int32_t* tasks;     // 4 per task: problem, half, member m, members G of the half
     int32_t* prog;            // per task: chunks its stripe has flushed (0 at launch)
     int32_t* err;             // set to 1 when a stripe's wait timed out (0 at launch)
+    int32_t wait_limit;       // sleeps a stripe waits for the one above before it gives up
 };
 
 // ---- profile construction on the device ------------------------------------

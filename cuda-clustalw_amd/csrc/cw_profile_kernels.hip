@@ -195,7 +195,9 @@ struct StripeSync {
     const int* wait;   // generic pointers, tested against null.  Stripes of one
     int* pub;          // workgroup: the counters and the error flag are in LDS
     int* err;          // (offset 0 is a valid LDS address) and are accessed as
-};                     // such; stripes of different workgroups: global memory
+                       // such; stripes of different workgroups: global memory
+    int limit;         // the latter's wait bound, in sleeps
+};
 
 // Stripes of different workgroups (cw_prf_wide) may run on different XCDs,
 // whose L2 caches are not coherent with each other.  Everything they hand over
@@ -234,7 +236,7 @@ __device__ __forceinline__ void wait_chunks_glb(const StripeSync& sy, int need)
     while (uni(ld_agent(sy.wait)) < need) {
         if (uni(ld_agent(sy.err)) != 0) break;
         __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1 << 20)) {
+        if (++spins > sy.limit) {
             st_agent(sy.err, 1);
             break;
         }
@@ -642,7 +644,7 @@ __device__ void psweep_part(const PSweep& W, int rmax, int m, int S, int* const*
 // only on stripe m-1, i.e. on a workgroup of lower index.
 template <int MODE>
 __device__ void psweep_wide(const PSweep& W, int m, int G, int* reg0, int rowcap, int* finH, int* finE,
-                            int* prog, int* err)
+                            int* prog, int* err, int limit)
 {
     const int nrows = W.nrows;
     const int base = nrows / G, rem = nrows % G;
@@ -654,7 +656,7 @@ __device__ void psweep_wide(const PSweep& W, int m, int G, int* reg0, int rowcap
     const int* inE = inH + rowcap;
     int* oH = m == G - 1 ? finH : reg0 + m * stride + 2 * rowcap;
     int* oE = m == G - 1 ? finE : oH + rowcap;
-    const StripeSync sy{m > 0 ? prog + (m - 1) : nullptr, m < G - 1 ? prog + m : nullptr, err};
+    const StripeSync sy{m > 0 ? prog + (m - 1) : nullptr, m < G - 1 ? prog + m : nullptr, err, limit};
     run_pstripe_x<MODE>(R, W, row0, P, m == 0, inH, inE, oH, oE, sy);
 }
 
@@ -1715,10 +1717,10 @@ extern "C" __global__ void __launch_bounds__(WAVE) cw_prf_wide(PrfArgs a, WideAr
     const PSweep W = half_def(p, 0, 0, M, N, pp.flags & 3, half, &s_err);
     if (half == 0) {
         if (m == 0) fill_top<MODE_FWD>(p, 0, 0, M, N, reg0);
-        psweep_wide<MODE_FWD>(W, m, G, reg0, rowcap, lv.fH, lv.fE, prog, w.err);
+        psweep_wide<MODE_FWD>(W, m, G, reg0, rowcap, lv.fH, lv.fE, prog, w.err, w.wait_limit);
     } else {
         if (m == 0) fill_top<MODE_REV>(p, 0, 0, M, N, reg0);
-        psweep_wide<MODE_REV>(W, m, G, reg0, rowcap, lv.rH, lv.rK, prog, w.err);
+        psweep_wide<MODE_REV>(W, m, G, reg0, rowcap, lv.rH, lv.rK, prog, w.err, w.wait_limit);
     }
     wave_fence();
     if (uni(s_err) != 0) st_agent(w.err, 1);

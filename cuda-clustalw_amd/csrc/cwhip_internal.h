@@ -141,6 +141,7 @@ struct cwhip_ctx {
     int prf_wide_rows = 256;                    // rows per cw_prf_wide stripe (0: no cw_prf_wide)
     int prf_wide_levels = 5;                    // levels before cw_prf_mm's last launch (<= 6; 1: none)
     int prf_wide_pairs = CWP_WIDE_MAXP;         // batches of at most this many pairs use them
+    int prf_wide_wait = 1 << 20;                // sleeps a cw_prf_wide stripe waits before it gives up
     float prf_score_ms = 0.f, prf_mm_ms = 0.f;
     double prf_cells = 0.0, prf_entries = 0.0;
     std::vector<std::vector<uint8_t>> msa_rows;   // rows of the last cwhip_msa

@@ -348,6 +348,7 @@ int dp_launch_collect(cwhip_ctx* ctx, BatchLayout& lo, std::vector<DpJob>& jobs)
         wa.tasks = ctx->p_wide.p;
         wa.prog = ctx->p_wide.p + 4 * nwide;
         wa.err = wa.prog + nwide;
+        wa.wait_limit = ctx->prf_wide_wait;
     }
     if (lo.tile_base[n] > 0) HIPCHK(cwp_launch_score(lo.tile_base[n], ctx->stream, a));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1021,15 +1022,16 @@ int cwhip_step_profiles(cwhip_ctx* ctx, const cwhip_msa_params* p, int nseq, con
     return CWHIP_OK;
 }
 
-int cwhip_set_msa_tuning(cwhip_ctx* ctx, int wide_rows, int wide_levels, int wide_pairs)
+int cwhip_set_msa_tuning(cwhip_ctx* ctx, int wide_rows, int wide_levels, int wide_pairs, int wait_limit)
 {
     if (!ctx) return CWHIP_E_ARG;
     if (wide_rows < -1 || wide_levels < -1 || wide_levels == 0 || wide_levels > kWideMaxLevels ||
-        wide_pairs < -1)
+        wide_pairs < -1 || wait_limit < -1)
         return fail(ctx, CWHIP_E_ARG, "msa tuning value out of range");
     if (wide_rows >= 0) ctx->prf_wide_rows = wide_rows;
     if (wide_levels >= 0) ctx->prf_wide_levels = wide_levels;
     if (wide_pairs >= 0) ctx->prf_wide_pairs = wide_pairs;
+    if (wait_limit >= 0) ctx->prf_wide_wait = wait_limit > 0 ? wait_limit : 1 << 20;
     return CWHIP_OK;
 }
 

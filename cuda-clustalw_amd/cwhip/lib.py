@@ -146,7 +146,7 @@ def load() -> C.CDLL:
     L.cwhip_fetch_counts.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_fetch_info.argtypes = [vp, P(np.int32, flags="C"), C.c_int64]
     L.cwhip_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
-    L.cwhip_set_msa_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.cwhip_set_msa_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.cwhip_get_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
     L.cwhip_set_tune_table.argtypes = [vp, C.POINTER(TuneTable)]
     L.cwhip_autotune.argtypes = [vp, C.POINTER(PairParams), P(np.int32, flags="C"), C.c_int64,
@@ -263,9 +263,9 @@ class Context:
         self._chk(self._L.cwhip_set_tuning(self._h, rmax, waves_per_group, groups_per_cu),
                   "cwhip_set_tuning")
 
-    def set_msa_tuning(self, wide_rows=-1, wide_levels=-1, wide_pairs=-1):
+    def set_msa_tuning(self, wide_rows=-1, wide_levels=-1, wide_pairs=-1, wait_limit=-1):
         """Schedule of the profile alignments' Myers-Miller levels (-1 keeps a value)."""
-        self._chk(self._L.cwhip_set_msa_tuning(self._h, wide_rows, wide_levels, wide_pairs),
+        self._chk(self._L.cwhip_set_msa_tuning(self._h, wide_rows, wide_levels, wide_pairs, wait_limit),
                   "cwhip_set_msa_tuning")
 
     def tune_table(self):

@@ -182,9 +182,11 @@ int cwhip_set_tuning(cwhip_ctx *ctx, int rmax, int waves_per_group, int groups_p
  * rows each pipelined stripe of a large root's half sweeps takes when those
  * run over several CUs (0: the root stays in its pair's workgroup), how many
  * levels sweep their halves one wave per workgroup before the rest run in the
- * pair's workgroup (1..6; 1: none), and the most profile pairs a batch may
- * hold for either (0: none).  -1 keeps a value. */
-int cwhip_set_msa_tuning(cwhip_ctx *ctx, int wide_rows, int wide_levels, int wide_pairs);
+ * pair's workgroup (1..6; 1: none), the most profile pairs a batch may hold
+ * for either (0: none), and how many sleeps such a stripe waits for the one
+ * above it before it gives up and the call fails with CWHIP_E_KERNEL (0: the
+ * built-in 2^20, about a second).  -1 keeps a value. */
+int cwhip_set_msa_tuning(cwhip_ctx *ctx, int wide_rows, int wide_levels, int wide_pairs, int wait_limit);
 
 /* Shapes of the batched kernel by tuning class (the longer sequence of a pair:
  * <=128, <=256, <=512, <=1024, <=2048, longer): stripe height in rows per lane

@@ -263,3 +263,36 @@ def test_msa_schedules_agree():
                 np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
     finally:
         c.close()
+
+
+def test_msa_stripe_wait_limit_reports_error():
+    """A pipelined root stripe whose wait for the stripe above runs out gives up
+    (every access stays in bounds; the others stop on the shared error flag),
+    and cwhip_msa fails with a kernel error instead of hanging; the same context
+    then aligns as the oracle does.  A one-sleep limit runs out before the
+    stripe above has flushed its first 64 columns (~127 of its steps)."""
+    a = seqgen.family(911, 6, 700, branch_mean=0.3, name_prefix="wa")
+    b = seqgen.family(912, 5, 650, branch_mean=0.4, name_prefix="wb")
+    seqs, names = a.seqs + b.seqs, a.names + b.names
+    n = len(seqs)
+    codes = np.concatenate(seqs)
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1])
+    pairs = O.range_pairs(n, 0, n, 0, n)
+    cnt, dist = O.pairwise_pairs(codes, offs, lens, pairs, O.Scoring(False))
+    dnd = O.nj_dnd(O.distance_matrix(n, pairs, dist), names)
+    w, sim, steps = O.guide_from_dnd(dnd, names)
+    r_rows, r_order, info = O.msa([s.astype(np.int32) for s in seqs], sim, w, steps, O.msa_params(False))
+    c = cwhip.Context(0)
+    try:
+        c.set_msa_tuning(wide_rows=256, wide_levels=5, wide_pairs=512, wait_limit=1)
+        with pytest.raises(cwhip.CwhipError):
+            c.msa(seqs, sim, w, steps)
+        c.set_msa_tuning(wait_limit=0)
+        rows, order, stats = c.msa(seqs, sim, w, steps)
+        np.testing.assert_array_equal(order, r_order)
+        for i in range(n):
+            np.testing.assert_array_equal(rows[i], r_rows[i].astype(np.uint8))
+    finally:
+        c.close()
